@@ -1,0 +1,52 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+def _split(arr, lens):
+    out, o = [], 0
+    for n in lens:
+        out.append(arr[o:o + n])
+        o += n
+    return out
+
+
+def load_decode_cases(name):
+    """Golden decode cases written by tests/golden/gen_golden.py."""
+    z = np.load(os.path.join(GOLDEN, name))
+    llr = _split(z["llr"], z["llr_len"])
+    hard = _split(z["hard"], z["hard_len"])
+    soft = _split(z["soft"], z["buf_len"])
+    lq = _split(z["lq"], z["buf_len"])
+    harq = _split(z["harq_in"], z["harq_len"])
+    cases = []
+    for i in range(len(z["cfg"])):
+        cases.append(dict(idx=i, cfg=tuple(int(v) for v in z["cfg"][i]),
+                          row=[int(v) for v in z["row"][i]],
+                          status=int(z["status"][i]), iters=int(z["iters"][i]),
+                          llr=llr[i], hard=hard[i], soft=soft[i], lq=lq[i],
+                          harq_in=harq[i] if len(harq[i]) else None))
+    return cases
+
+
+def case_id(c):
+    bgn, A, crc, E, rv, Qm, h = c["cfg"]
+    return f"{c['idx']}-bg{bgn}-A{A}-E{E}-rv{rv}-Qm{Qm}-h{h}-algo{c['row'][0]}-it{c['row'][1]}"
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    from oracle import oracle as O
+    O.lib()
+    return O

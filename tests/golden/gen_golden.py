@@ -1,0 +1,375 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ from the reference C library.
+
+Runs ONLY in the development container: it loads oracle/_ref/libnrLDPC_ref.so,
+which oracle/Makefile.ref builds from the reference sources
+(/root/reference/c_lib) plus the two probe units that expose the decoders'
+file-static posterior arrays.  The fixtures are plain data (inputs and the
+reference's outputs) in .npz files; nothing from the reference is stored.
+
+    make -f oracle/Makefile.ref && python tests/golden/gen_golden.py
+
+Iteration counts are not returned by the reference; they are recovered as
+min{k : decode with max_iterations = k returns 0} (else max_iterations), which
+is exact because the decoders are deterministic and prefix-identical in k.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402  (input generation only)
+
+REF = os.path.join(ROOT, "oracle", "_ref", "libnrLDPC_ref.so")
+
+
+class InCfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint16) for n in
+                ("bgn", "A", "crctype", "E", "rv", "Qm", "HARQ_on")]
+
+
+class DecCfg(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_uint16), ("max_iterations", ctypes.c_uint16),
+                ("alpha", ctypes.c_uint16), ("beta", ctypes.c_uint16),
+                ("filling_bit_value", ctypes.c_int16), ("max_llr", ctypes.c_int16),
+                ("min_llr", ctypes.c_int16)]
+
+
+class LdpcCfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint16) for n in
+                ("bgn", "A", "crctype", "E", "rv", "Qm", "HARQ_on",
+                 "B", "Zc", "iLS", "K", "F", "N", "k0")] + [
+        ("Vij_array", ctypes.c_void_p), ("HBG_array", ctypes.c_void_p),
+        ("HBG_offset_array", ctypes.c_void_p)]
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Ref:
+    def __init__(self):
+        self.L = ctypes.CDLL(REF)
+        self.L.c_ldpc_5g_decode.restype = ctypes.c_int32
+        self.L.c_ldpc_5g_encode.restype = ctypes.c_int32
+        self.L.calculate_crc.restype = ctypes.c_uint32
+        self.L.calculate_crc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        self.L.tx_rx_process.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.c_uint16, ctypes.c_float,
+                                         ctypes.c_void_p]
+        self.L.ref_probe_fixed_lq.restype = ctypes.POINTER(ctypes.c_int16)
+        self.L.ref_probe_float_lq.restype = ctypes.POINTER(ctypes.c_float)
+        self.harq = np.ctypeslib.as_array(
+            (ctypes.c_int16 * (68 * 384)).in_dll(self.L, "HARQ_memory"))
+        self.libc = ctypes.CDLL("libc.so.6")
+
+    def params(self, bgn, A, crctype, E, rv, Qm, harq_on=0):
+        c = LdpcCfg()
+        ic = InCfg(bgn, A, crctype, E, rv, Qm, harq_on)
+        st = self.L.gen_ldpc_parameters(ctypes.byref(c), ctypes.byref(ic))
+        return st, [c.B, c.Zc, c.iLS, c.K, c.F, c.N, c.k0]
+
+    def decode(self, llr, cfg, row):
+        """One c_ldpc_5g_decode call; returns status, hard, soft, lq."""
+        bgn, A, crctype, E, rv, Qm, harq_on = cfg
+        st, p = self.params(*cfg)
+        B = A + (16 if crctype == 0 else 24)
+        hard = np.zeros(max(B // 8, 1), np.uint8)
+        llr = np.ascontiguousarray(llr, np.int16)
+        status = self.L.c_ldpc_5g_decode(P(llr), P(hard),
+                                         ctypes.byref(InCfg(*cfg)),
+                                         ctypes.byref(DecCfg(*row)))
+        if st != 0:
+            return status, hard, None, None
+        n = p[5] + 2 * p[1]
+        soft = self.harq[:n].copy()
+        if row[0] == 1:
+            lq = np.ctypeslib.as_array(self.L.ref_probe_fixed_lq(), (n,)).copy()
+        elif row[0] in (0, 4):
+            lq = np.ctypeslib.as_array(self.L.ref_probe_float_lq(), (n,)).copy()
+        else:
+            lq = None
+        return status, hard, soft, lq
+
+    def decode_with_iters(self, llr, cfg, row):
+        """Full decode plus the executed iteration count (min-k search).
+        HARQ_on inputs are replayed from the same starting buffer."""
+        start = self.harq.copy()
+        status, hard, soft, lq = self.decode(llr, cfg, row)
+        end = self.harq.copy()
+        iters = row[1]
+        if status == 0:
+            for k in range(1, row[1] + 1):
+                self.harq[:] = start
+                r2 = list(row)
+                r2[1] = k
+                st2, _, _, _ = self.decode(llr, cfg, r2)
+                if st2 == 0:
+                    iters = k
+                    break
+        self.harq[:] = end
+        return status, hard, soft, lq, iters
+
+
+def gen_crc(ref, rng):
+    cases = []
+    for nbytes in (29, 30, 1, 2, 3, 16, 127, 1056, 480):
+        for crctype in range(4):
+            data = rng.integers(0, 256, nbytes).astype(np.uint8)
+            cases.append((data, crctype, ref.L.calculate_crc(P(data), nbytes, crctype)))
+    # invalid crctype returns 0
+    data = rng.integers(0, 256, 8).astype(np.uint8)
+    cases.append((data, 7, ref.L.calculate_crc(P(data), 8, 7)))
+    # ragged payloads are stored concatenated, with their lengths
+    lens = np.array([len(c[0]) for c in cases], np.int32)
+    np.savez_compressed(os.path.join(HERE, "crc_vectors.npz"),
+                        data=np.concatenate([c[0] for c in cases]), lens=lens,
+                        crctype=np.array([c[1] for c in cases], np.int32),
+                        crc=np.array([c[2] for c in cases], np.uint32))
+
+
+def gen_params(ref):
+    rows = []
+    for bgn in (1, 2):
+        for crctype in range(4):
+            for A in range(0, 8448 + 8, 8):
+                for rv in range(4):
+                    st, p = ref.params(bgn, A, crctype, 25344, rv, 2, 0)
+                    rows.append([bgn, A, crctype, 25344, rv, 2, 0, st] + p)
+    # invalid configs: each validation rule (c_lib/ldpc_internal.c:86-131)
+    bad = [(3, 8424, 2, 25344, 0, 2, 0), (1, 8424, 4, 25344, 0, 2, 0),
+           (1, 8424, 2, 25344, 0, 3, 0), (1, 8424, 2, 25344, 0, 2, 2),
+           (1, 8424, 2, 25344, 4, 2, 0), (1, 8424, 2, 25345, 0, 2, 0),
+           (1, 8420, 2, 25344, 0, 2, 0), (1, 8432, 2, 25344, 0, 2, 0),
+           (2, 3824, 2, 25344, 0, 2, 0), (2, 3816, 2, 25344, 0, 2, 0),
+           (0, 8, 0, 100, 0, 2, 0), (1, 8, 0, 99, 0, 2, 0)]
+    for cfg in bad:
+        st, p = ref.params(*cfg)
+        rows.append(list(cfg) + [st] + (p if st == 0 else [0] * 7))
+    np.savez_compressed(os.path.join(HERE, "params_table.npz"),
+                        rows=np.array(rows, np.int32))
+
+
+def gen_encoder(ref, rng):
+    """Reference encoder outputs (packed rate-matched bytes)."""
+    cfgs = []
+    for Zc, bgn in [(2, 1), (3, 2), (8, 1), (15, 2), (52, 1), (104, 2), (160, 2),
+                    (208, 1), (384, 1), (320, 1), (56, 2), (7, 1)]:
+        A = 24 * Zc
+        if bgn == 1 and A > 8448 - 24:
+            A = 8424
+        if bgn == 2 and A > 3840 - 24:
+            A = 3816
+        for crctype, rv, rate, Qm in [(0, 0, 1 / 3, 2), (1, 1, 1 / 5, 4), (2, 2, 1 / 2, 6),
+                                      (3, 3, 1 / 8, 8), (2, 0, 1.0, 1)]:
+            E = int((A + 48) / rate)
+            E -= E % Qm
+            if E > 65535:
+                continue
+            cfgs.append((bgn, A, crctype, E, rv, Qm))
+    payloads, outs, meta = [], [], []
+    for cfg in cfgs:
+        bgn, A, crctype, E, rv, Qm = cfg
+        bits = rng.integers(0, 2, A).astype(np.uint8)
+        info = np.packbits(bits)
+        rm = np.zeros((E + 7) // 8, np.uint8)
+        st = ref.L.c_ldpc_5g_encode(P(info), P(rm), ctypes.byref(InCfg(*cfg, 0)))
+        assert st == 0
+        payloads.append(bits)
+        outs.append(rm)
+        meta.append(list(cfg))
+    np.savez_compressed(os.path.join(HERE, "encoder_vectors.npz"),
+                        meta=np.array(meta, np.int32),
+                        payload=np.concatenate(payloads),
+                        rm=np.concatenate(outs))
+
+
+def channel_llr(cb_cfg, snr_noise_db, rng):
+    bgn, A, crctype, E, rv, Qm = cb_cfg
+    return O.make_codeblock(bgn, A, crctype, E, rv, Qm, snr_noise_db, rng)
+
+
+def gen_rate_recovery(ref, rng):
+    """HARQ_memory after rate recovery (decode with max_iterations = 0)."""
+    cases = []
+    row0 = [1, 0, 22936, 0, 1280, 32767, -32767]
+
+    def run(cfg, llr, filler=1280):
+        row = list(row0)
+        row[4] = filler
+        st, _, soft, _ = ref.decode(llr, cfg, row)
+        assert st == 1
+        cases.append((list(cfg) + [filler], llr.copy(), soft.copy()))
+
+    for cfg in [(1, 8424, 2, 25344, 0, 2), (1, 488, 1, 1536, 1, 1), (2, 1016, 2, 5200, 2, 4),
+                (2, 2056, 2, 4158, 3, 6), (1, 3816, 0, 6000, 3, 8), (2, 176, 1, 512, 1, 2),
+                (1, 8, 0, 30000, 2, 2), (2, 16, 3, 9000, 0, 6)]:
+        llr = rng.integers(-12000, 12000, cfg[3]).astype(np.int16)
+        run(tuple(cfg) + (0,), llr, filler=int(rng.integers(-3000, 3000)))
+    # repetition + int16 wrap: E >> N - F with large LLRs
+    cfg = (2, 16, 3, 60000, 0, 4, 0)
+    run(cfg, rng.integers(20000, 32767, 60000).astype(np.int16))
+    # HARQ chain rv 0 -> 2 -> 3 -> 1 with combining (large values: wraps)
+    ref.harq[:] = 0
+    for i, rv in enumerate((0, 2, 3, 1)):
+        cfg = (1, 2024, 2, 7000, rv, 2, 0 if i == 0 else 1)
+        run(cfg, rng.integers(-32768, 32767, 7000).astype(np.int16))
+    meta = np.array([c[0] for c in cases], np.int32)
+    np.savez_compressed(os.path.join(HERE, "rate_recovery.npz"), meta=meta,
+                        llr=np.concatenate([c[1] for c in cases]),
+                        soft=np.concatenate([c[2] for c in cases]))
+
+
+DEFAULT_ROW = [1, 8, 22936, 0, 1280, 32767, -32767]
+
+
+def gen_decode(ref, rng):
+    """Decoder fixtures: algo 1 (int16) and algo 4 / algo 0 (FP32)."""
+    cases = []  # (cfg7, row7, llr, status, hard, soft, lq, iters, harq_in)
+
+    def add(cfg7, row, llr, harq_in=None):
+        if harq_in is not None:
+            ref.harq[:] = 0
+            ref.harq[:len(harq_in)] = harq_in
+        st, hard, soft, lq, iters = ref.decode_with_iters(llr, cfg7, row)
+        cases.append((list(cfg7), list(row), llr.copy(), st, hard.copy(),
+                      soft.copy(), lq.copy(), iters,
+                      None if harq_in is None else np.asarray(harq_in, np.int16).copy()))
+
+    # 1) the reference CUDA decoder test's 8 configs (CUDA/test_ldpc_5g_decoder.cu
+    #    :318-327): glibc rand() payload + reference encoder + reference channel
+    #    at noise 0 dB, srand(1000 + A + E + 17*Qm), algo 1, 32 iterations.
+    cuda_cases = [(1, 488, 1, 1536, 0, 1), (1, 1024, 0, 3072, 0, 1),
+                  (1, 2400, 2, 8192, 0, 1), (1, 4072, 1, 16384, 0, 1),
+                  (2, 176, 1, 512, 0, 1), (2, 640, 3, 2048, 0, 1),
+                  (2, 1200, 0, 4096, 0, 1), (2, 2400, 2, 8192, 0, 1)]
+    row32 = [1, 32, 22936, 0, 1280, 32767, -32767]
+    for cfg in cuda_cases:
+        bgn, A, crctype, E, rv, Qm = cfg
+        ref.libc.srand(1000 + A + E + 17 * Qm)
+        bits = np.array([ref.libc.rand() & 1 for _ in range(A)], np.uint8)
+        info = np.packbits(bits)
+        rm = np.zeros((E + 7) // 8 + 8, np.uint8)
+        assert ref.L.c_ldpc_5g_encode(P(info), P(rm), ctypes.byref(InCfg(*cfg, 0))) == 0
+        llr = np.zeros(E, np.int16)
+        assert ref.L.tx_rx_process(P(rm), E, Qm, ctypes.c_float(0.0), P(llr)) == 0
+        add(cfg + (0,), row32, llr)
+
+    # 2) the benchmark workload: BG1 Zc=384 A=8424 CRC24B E=25344 QPSK,
+    #    Es/N0 = 1 dB (noise -1 dB) and two harder points
+    for seed, noise in [(11, -1.0), (12, -1.0), (13, -1.0), (14, 0.5), (15, 1.5)]:
+        r = np.random.default_rng(seed)
+        cb = O.make_codeblock(1, 8424, 2, 25344, 0, 2, noise, r)
+        add((1, 8424, 2, 25344, 0, 2, 0), DEFAULT_ROW, cb["llr"])
+
+    # 3) assorted sizes, rates, rv, Qm (tiny Zc included)
+    assorted = [((2, 8, 0, 200, 0, 2), -3.0), ((1, 16, 0, 300, 1, 2), -3.0),
+                ((2, 1016, 2, 5200, 2, 4), -4.0), ((2, 2056, 2, 4160, 0, 4), -6.0),
+                ((1, 3816, 1, 11448, 3, 6), -10.0), ((2, 3816, 3, 9600, 1, 8), -16.0),
+                ((1, 8424, 2, 14000, 0, 8), -18.0), ((2, 56, 1, 240, 0, 2), -1.0),
+                ((1, 64, 0, 720, 2, 2), -2.0), ((2, 3000, 2, 7600, 0, 2), -0.5)]
+    for cfg, noise in assorted:
+        cb = O.make_codeblock(*cfg, noise, rng)
+        add(cfg + (0,), DEFAULT_ROW, cb["llr"])
+        add(cfg + (0,), [1, 12, 26000, 32, -900, 32767, -32767], cb["llr"])
+
+    # 4) decoder-row edge cases on one mid-size code
+    cfg = (2, 1016, 2, 3120, 0, 2)
+    cb = O.make_codeblock(*cfg, -2.5, rng)
+    for row in ([1, 0, 22936, 0, 1280, 32767, -32767],       # no iterations
+                [1, 1, 22936, 0, 1280, 32767, -32767],
+                [1, 8, 32767, 0, 1280, 32767, -32767],
+                [1, 8, 40000, 0, 1280, 32767, -32767],       # alpha wraps negative
+                [1, 8, 65535, 100, 1280, 32767, -32767],
+                [1, 8, 22936, 200, 1280, 32767, -32767],     # offset min-sum
+                [1, 8, 22936, 40000, 1280, 32767, -32767],   # beta wraps negative
+                [1, 8, 22936, 0, 1280, 2000, -1500],         # tight clipping
+                [1, 8, 22936, 0, 1280, -200, -32768],        # max_llr < 0
+                [1, 8, 22936, 0, -1280, 32767, -32767],      # negative filler
+                [1, 20, 16384, 64, 32767, 32767, -32768]):
+        add(cfg + (0,), row, cb["llr"])
+    # saturated channel values: exercises int16 wrap of rel = LQ - Lr
+    llr = np.where(cb["rm"] == 0, 32767, -32768).astype(np.int16)
+    llr[::7] = -llr[::7]
+    add(cfg + (0,), [1, 8, 65535, 0, 32767, 32767, -32768], llr)
+    add(cfg + (0,), [1, 8, 22936, 0, 32767, 32767, -32768], llr)
+
+    # 5) HARQ chains: rv0 -> rv2 -> rv3 -> rv1 soft-combined, decoded each time
+    for seed, noise in [(21, -5.5), (22, 3.0)]:
+        r = np.random.default_rng(seed)
+        payload = r.integers(0, 2, 2024).astype(np.uint8)
+        b = O.attach_crc(payload, 2)
+        harq = None
+        for i, rv in enumerate((0, 2, 3, 1)):
+            cfg = (1, 2024, 2, 3000, rv, 2, 0 if i == 0 else 1)
+            c = O.code(*cfg)
+            cw = O.encode(c, b)
+            llr = O.channel_llr(O.rate_match(c, cw), 2, noise, r)
+            start = ref.harq[:c.N + 2 * c.Zc].copy() if i else None
+            add(cfg, DEFAULT_ROW, llr, harq_in=start)
+            harq = ref.harq[:c.N + 2 * c.Zc].copy()
+        del harq
+
+    # 6) float decoders: algo 4 (FP32 min-sum) and algo 0 (FP32 BP), BG2
+    #    Zc 104/208 at R 0.2/0.5 (the FP32 workload) plus BG1 small codes
+    rows_f = [[4, 20, 22936, 0, 1280, 32767, -32767],
+              [0, 20, 1, 0, 1280, 20, -32768]]
+    fcfgs = [((2, 1016, 2, 5200, 0, 2), -4.5), ((2, 2056, 2, 4160, 0, 4), -7.0),
+             ((2, 1016, 2, 2080, 0, 4), -8.5), ((2, 2056, 2, 10400, 0, 2), -5.5),
+             ((1, 488, 1, 1536, 1, 2), -2.0), ((2, 16, 0, 160, 0, 2), -2.0),
+             ((1, 8424, 2, 25344, 0, 2), -1.0)]
+    for cfg, noise in fcfgs:
+        cb = O.make_codeblock(*cfg, noise, rng)
+        for row in rows_f:
+            add(cfg + (0,), row, cb["llr"])
+    cfg = (2, 1016, 2, 3120, 0, 2)
+    cb = O.make_codeblock(*cfg, -2.0, rng)
+    for row in ([4, 8, 30000, 64, 1280, 4000, -4000], [4, 8, 50000, 0, -640, 32767, -32767],
+                [0, 1, 1, 0, 1280, 20, -32768], [0, 8, 1, 0, 0, 20, -32768],
+                [4, 0, 22936, 0, 1280, 32767, -32767]):
+        add(cfg + (0,), row, cb["llr"])
+
+    split = {1: [], 0: [], 4: []}
+    for c in cases:
+        split[c[1][0]].append(c)
+    for algo, name in ((1, "decode_algo1.npz"), (4, "decode_algo4.npz"),
+                       (0, "decode_algo0.npz")):
+        cs = split[algo]
+        harq_in = [c[8] if c[8] is not None else np.zeros(0, np.int16) for c in cs]
+        np.savez_compressed(
+            os.path.join(HERE, name),
+            cfg=np.array([c[0] for c in cs], np.int32),
+            row=np.array([c[1] for c in cs], np.int32),
+            status=np.array([c[3] for c in cs], np.int32),
+            iters=np.array([c[7] for c in cs], np.int32),
+            llr=np.concatenate([c[2] for c in cs]),
+            llr_len=np.array([len(c[2]) for c in cs], np.int32),
+            hard=np.concatenate([c[4] for c in cs]),
+            hard_len=np.array([len(c[4]) for c in cs], np.int32),
+            soft=np.concatenate([c[5] for c in cs]),
+            lq=np.concatenate([c[6] for c in cs]),
+            buf_len=np.array([len(c[5]) for c in cs], np.int32),
+            harq_in=np.concatenate(harq_in),
+            harq_len=np.array([len(h) for h in harq_in], np.int32))
+        print(name, len(cs), "cases")
+
+
+def main():
+    if not os.path.exists(REF):
+        raise SystemExit(f"{REF} missing: run make -f oracle/Makefile.ref first")
+    ref = Ref()
+    rng = np.random.default_rng(20261203)
+    gen_crc(ref, rng)
+    gen_params(ref)
+    gen_encoder(ref, rng)
+    gen_rate_recovery(ref, rng)
+    gen_decode(ref, rng)
+
+
+if __name__ == "__main__":
+    main()
