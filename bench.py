@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark: decoded info Gbps of the 5G NR LDPC decode path on one MI355X.
+
+Workload (one "step" = one batched decode launch over the resident batch):
+BG1 Zc=384 (K=8448), A=8424 + CRC24B, rv0, E=25344 (R~1/3), QPSK over AWGN at
+Es/N0 = 1 dB, LLR x128 quantized to int16 (clip +-10000), 256 codeblocks,
+decoder row [1, 8, 22936, 0, 1280, 32767, -32767] (int16 layered min-sum,
+8 iterations max, CRC24B early stop).  Inputs are synthesized on the host,
+copied to HBM once, and stay resident; the timed region is K launches.
+
+Prints ONE JSON line.  Also reported: per-launch kernel time from HIP events
+on the launch stream, roofline (algorithmic HBM bytes / kernel time vs
+8 TB/s), host-visible batch latency p50/p99, the per-codeblock ABI path
+(c_ldpc_5g_decode, one launch per codeblock) and the CPU baseline (the
+oracle's -O3 restatement of the reference decoder on the host cores).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CFG = (1, 8424, 2, 25344, 0, 2, 0)
+ROW = [1, 8, 22936, 0, 1280, 32767, -32767]
+BATCH = 256
+NOISE_DB = -1.0        # Es/N0 = 1 dB with unit-energy symbols
+EDGES_BG1 = 316
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes(iters, E=25344, B=8448, zc=384, w=8):
+    """Per launch: sum over codeblocks of 2E + B/8 + I_exec * w * nnz * Zc
+    (LQ and Lr read + written once per edge per iteration, int16)."""
+    iters = np.asarray(iters, np.int64)
+    return int(np.sum(2 * E + B // 8 + iters * w * EDGES_BG1 * zc))
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_minsum_i16_bg1_zc384_b256.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(llr, threads):
+    """Oracle restatement of the reference algo-1 decoder (-O3, scalar loop
+    structure of c_lib) on the host cores, one codeblock per task."""
+    from oracle import oracle as O
+    L = O.lib()
+    n = llr.shape[0]
+    hard = np.zeros((n, 1056), np.uint8)
+    st = np.zeros(n, np.int32)
+    its = np.zeros(n, np.int32)
+    d = O.dec(ROW)
+    secs = L.orc_decode_batch_i16(O._p(llr), O._p(hard), O._p(st), O._p(its), n, *CFG[:6],
+                                  ctypes.byref(d), threads)
+    return secs, st, its, hard
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--latency-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(codeblocks shard by index across GPUs with no exchange, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    from deeplearning_5g_ldpc_decoder_amd._lib import DecoderConfig, LdpcInConfig, ptr
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, synth_batch
+
+    n = args.batch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    llr, payload = synth_batch(CFG, n, NOISE_DB, seed=1, threads=os.cpu_count() or 8)
+    d_llr = torch.from_numpy(llr.reshape(-1)).to(dev)
+    plan = BatchPlan([CFG] * n, ROW)
+    out = plan.alloc_outputs(dev)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    for _ in range(args.warmup):
+        plan.launch(d_llr, out, stream=sh)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        plan.launch(d_llr, out, stream=sh)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+
+    status = out["status"].cpu().numpy()
+    iters = out["iters"].cpu().numpy()
+    hard = out["hard"].cpu().numpy().reshape(n, 1056)
+    decoded = np.unpackbits(hard, axis=1)[:, :CFG[1]]
+    good = int(np.sum(np.all(decoded == payload, axis=1)))
+
+    info_bits = n * CFG[1]
+    gbps = info_bits * args.steps / wall / 1e9
+    ms_per_step = wall * 1e3 / args.steps
+
+    # host-visible latency of one batch (launch -> results synchronized)
+    lat = []
+    for _ in range(args.latency_reps):
+        t = time.perf_counter()
+        plan.launch(d_llr, out, stream=sh)
+        stream.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+
+    # per-codeblock reference ABI path: one launch + copies per codeblock
+    L = _lib.load()
+    ic = LdpcInConfig(*CFG)
+    dc = DecoderConfig(*ROW)
+    hb = np.zeros(1056, np.uint8)
+    n_pc = min(n, 64)
+    for i in range(4):
+        L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
+    t = time.perf_counter()
+    for i in range(n_pc):
+        L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
+    per_cb_gbps = n_pc * CFG[1] / (time.perf_counter() - t) / 1e9
+
+    avg_kernel_s = float(kernel_ms.mean()) / 1e3
+    alg = algorithmic_bytes(iters)
+    achieved = alg / avg_kernel_s
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": load_traffic(),
+            "algorithmic_bytes_per_launch": alg,
+            "compulsory_bytes_per_launch": int(n * (2 * 25344 + 1056)),
+            "kernel_ms_avg": float(kernel_ms.mean())}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8
+        threads = max(1, min(threads, 16))
+        reps = 3
+        secs = 0.0
+        for _ in range(reps):
+            s, cst, cits, chard = cpu_baseline(llr, threads)
+            secs += s
+        same = bool(np.array_equal(cst, status) and np.array_equal(cits, iters) and
+                    np.array_equal(chard, hard))
+        cpu = {"value": reps * info_bits / secs / 1e9, "unit": "Gbps", "cores": threads,
+               "kind": "port",
+               "sample": f"{reps} passes over the same {n}-codeblock batch, one codeblock per "
+                         f"thread task; bit-exact with the GPU result: {same}"}
+
+    line = {
+        "metric": "decoded info Gbps (BG1 Zc=384, 8 it) + p99 decode latency, BLER-matched",
+        "value": gbps, "unit": "Gbps", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "bg1_zc384_minsum_i16_8it_batch256", "global_batch": n,
+                   "codeblock": {"bgn": 1, "A": 8424, "crc": "24B", "E": 25344, "Qm": 2,
+                                 "rv": 0, "EsN0_dB": 1.0},
+                   "decoder_row": ROW, "parallelism": "single-gpu"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                       "p99": float(np.percentile(lat, 99))},
+        "mean_iterations": float(iters.mean()), "max_iterations_executed": int(iters.max()),
+        "bler": float(np.mean(status != 0)), "payload_correct": good,
+        "per_codeblock_abi_gbps": per_cb_gbps,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,135 @@
+"""ctypes binding of libnrldpc_mi355.so (declared in include/nr_ldpc_mi355.h).
+
+The library is built in-tree (``make -C deeplearning_5g_ldpc_decoder_amd/csrc``
+or ``__graft_entry__.build()``).  Loading fails loudly when it is missing:
+there is no CPU fallback for the decode path.
+"""
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libnrldpc_mi355.so")
+CSRC_DIR = os.path.join(PKG_DIR, "csrc")
+
+# Every exported symbol of include/nr_ldpc_mi355.h (functions and data).
+EXPORTED_FUNCTIONS = (
+    "calculate_crc", "gen_ldpc_parameters", "dequantize_int16_to_llr",
+    "c_ldpc_5g_decode", "c_ldpc_5g_encode", "tx_rx_process",
+    "ldpc_mi355_decode_ex", "ldpc_mi355_channel_seed", "ldpc_mi355_plan_create",
+    "ldpc_mi355_plan_layout", "ldpc_mi355_plan_decode", "ldpc_mi355_plan_destroy",
+    "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
+    "ldpc_mi355_synth_batch",
+)
+EXPORTED_DATA = (
+    "BG1_HBG_offset_array", "BG1_HBG_array", "BG1_HBG_Vij_array",
+    "BG2_HBG_offset_array", "BG2_HBG_array", "BG2_HBG_Vij_array", "HARQ_memory",
+)
+
+
+class LdpcInConfig(ctypes.Structure):
+    """ldpc_in_config_t (c_lib/nr_ldpc.h:42-50)."""
+    _fields_ = [(n, ctypes.c_uint16) for n in
+                ("bgn", "A", "crctype", "E", "rv", "Qm", "HARQ_on")]
+
+
+class DecoderConfig(ctypes.Structure):
+    """ldpc_decoder_cfg_t (c_lib/nr_ldpc.h:82-90)."""
+    _fields_ = [("algo", ctypes.c_uint16), ("max_iterations", ctypes.c_uint16),
+                ("alpha", ctypes.c_uint16), ("beta", ctypes.c_uint16),
+                ("filling_bit_value", ctypes.c_int16), ("max_llr", ctypes.c_int16),
+                ("min_llr", ctypes.c_int16)]
+
+
+class LdpcConfig(ctypes.Structure):
+    """ldpc_config_t (c_lib/nr_ldpc.h:54-79)."""
+    _fields_ = [(n, ctypes.c_uint16) for n in
+                ("bgn", "A", "crctype", "E", "rv", "Qm", "HARQ_on",
+                 "B", "Zc", "iLS", "K", "F", "N", "k0")] + [
+        ("Vij_array", ctypes.POINTER(ctypes.c_uint16)),
+        ("HBG_array", ctypes.POINTER(ctypes.c_uint16)),
+        ("HBG_offset_array", ctypes.POINTER(ctypes.c_uint16))]
+
+
+class CbDescriptor(ctypes.Structure):
+    """ldpc_mi355_cb_t: one codeblock of a batch."""
+    _fields_ = [("cfg", LdpcInConfig), ("reserved", ctypes.c_int16),
+                ("llr_offset", ctypes.c_uint32), ("soft_slot", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def build(quiet=True):
+    """Compile the HIP library for gfx950 (hipcc cross-compiles without a GPU)."""
+    import subprocess
+    out = subprocess.run(["make", "-C", CSRC_DIR, "-j8"], capture_output=quiet, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("libnrldpc_mi355 build failed:\n" + (out.stdout or "") +
+                           (out.stderr or ""))
+
+
+def load():
+    """Load the library (once) and declare argument types."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
+            f"make -C {CSRC_DIR}; the decode path has no CPU fallback")
+    # PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7) and
+    # reaches it under another file name; loading torch first makes this
+    # library bind to that same HIP runtime, so device pointers and streams
+    # are shared instead of living in two runtimes.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    L.calculate_crc.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.calculate_crc.restype = ctypes.c_uint32
+    L.gen_ldpc_parameters.argtypes = [ctypes.POINTER(LdpcConfig), ctypes.POINTER(LdpcInConfig)]
+    L.gen_ldpc_parameters.restype = ctypes.c_int
+    L.dequantize_int16_to_llr.argtypes = [ctypes.c_int16]
+    L.dequantize_int16_to_llr.restype = ctypes.c_float
+    L.c_ldpc_5g_decode.argtypes = [P, P, ctypes.POINTER(LdpcInConfig),
+                                   ctypes.POINTER(DecoderConfig)]
+    L.c_ldpc_5g_decode.restype = ctypes.c_int32
+    L.ldpc_mi355_decode_ex.argtypes = [P, P, ctypes.POINTER(LdpcInConfig),
+                                       ctypes.POINTER(DecoderConfig), P, P]
+    L.ldpc_mi355_decode_ex.restype = ctypes.c_int32
+    L.c_ldpc_5g_encode.argtypes = [P, P, ctypes.POINTER(LdpcInConfig)]
+    L.c_ldpc_5g_encode.restype = ctypes.c_int32
+    L.tx_rx_process.argtypes = [P, ctypes.c_size_t, ctypes.c_uint16, ctypes.c_float, P]
+    L.tx_rx_process.restype = ctypes.c_int
+    L.ldpc_mi355_channel_seed.argtypes = [ctypes.c_uint64]
+    L.ldpc_mi355_channel_seed.restype = None
+    L.ldpc_mi355_plan_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(CbDescriptor),
+                                         ctypes.c_int32, ctypes.POINTER(DecoderConfig),
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
+    L.ldpc_mi355_plan_create.restype = ctypes.c_int32
+    L.ldpc_mi355_plan_layout.argtypes = [P, P, P, P, P, P]
+    L.ldpc_mi355_plan_layout.restype = ctypes.c_int32
+    L.ldpc_mi355_plan_decode.argtypes = [P, P, P, P, P, P, P, P]
+    L.ldpc_mi355_plan_decode.restype = ctypes.c_int32
+    L.ldpc_mi355_plan_destroy.argtypes = [P]
+    L.ldpc_mi355_plan_destroy.restype = None
+    L.ldpc_mi355_sync.argtypes = []
+    L.ldpc_mi355_sync.restype = ctypes.c_int32
+    L.ldpc_mi355_soft_slot_elems.argtypes = []
+    L.ldpc_mi355_soft_slot_elems.restype = ctypes.c_int32
+    L.ldpc_mi355_version.argtypes = []
+    L.ldpc_mi355_version.restype = ctypes.c_char_p
+    L.ldpc_mi355_synth_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32,
+                                         ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, P, P]
+    L.ldpc_mi355_synth_batch.restype = ctypes.c_int32
+    _lib = L
+    return L
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor as c_void_p."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)

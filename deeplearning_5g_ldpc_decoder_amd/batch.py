@@ -1,0 +1,157 @@
+"""Batched decode on the MI355X through the C ABI plan entry points.
+
+PyTorch is only the device-memory and stream plumbing here: buffers are torch
+tensors on ``cuda`` (HIP), their addresses go to
+``ldpc_mi355_plan_decode``, and the work is the library's HIP kernel.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import CbDescriptor, DecoderConfig, LdpcInConfig, ptr
+
+ALGO_LQ_DTYPE = {1: "int16", 0: "float32", 4: "float32"}
+
+
+def soft_slot_elems():
+    return int(_lib.load().ldpc_mi355_soft_slot_elems())
+
+
+def _decoder(row):
+    return DecoderConfig(*[int(v) for v in row])
+
+
+class BatchPlan:
+    """A fixed batch of codeblocks and one decoder row.
+
+    cfgs: sequence of (bgn, A, crctype, E, rv, Qm, HARQ_on).
+    llr_offsets: element offset of each codeblock's E LLRs in the LLR buffer
+    (default: packed back to back).  soft_slots: HARQ soft-buffer slot per
+    codeblock or -1 (default: none).  want_lq: also return posteriors.
+    """
+
+    def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False):
+        L = _lib.load()
+        n = len(cfgs)
+        if n == 0:
+            raise ValueError("empty batch")
+        self.n = n
+        self.cfgs = [tuple(int(v) for v in c) for c in cfgs]
+        self.row = [int(v) for v in decoder_row]
+        arr = (CbDescriptor * n)()
+        off = 0
+        self.llr_offsets = []
+        for i, c in enumerate(self.cfgs):
+            arr[i].cfg = LdpcInConfig(*c)
+            o = off if llr_offsets is None else int(llr_offsets[i])
+            arr[i].llr_offset = o
+            self.llr_offsets.append(o)
+            off += c[3]
+            arr[i].soft_slot = -1 if soft_slots is None else int(soft_slots[i])
+        self.llr_elems = max(o + c[3] for o, c in zip(self.llr_offsets, self.cfgs))
+        self._handle = ctypes.c_void_p()
+        bad = ctypes.c_int32(-1)
+        dec = _decoder(self.row)
+        st = L.ldpc_mi355_plan_create(ctypes.byref(self._handle), arr, n, ctypes.byref(dec),
+                                      1 if want_lq else 0, ctypes.byref(bad))
+        if st != 0:
+            raise ValueError(f"plan_create failed ({st}), codeblock {bad.value}")
+        hb, le, es = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int32()
+        hoff = np.zeros(n, np.uint32)
+        loff = np.zeros(n, np.uint32)
+        L.ldpc_mi355_plan_layout(self._handle, ctypes.byref(hb), ctypes.byref(le),
+                                 ctypes.byref(es), ptr(hoff), ptr(loff))
+        self.hard_bytes = int(hb.value)
+        self.lq_elems = int(le.value)
+        self.lq_elem_size = int(es.value)
+        self.hard_offsets = hoff.astype(np.int64)
+        self.lq_offsets = loff.astype(np.int64)
+        self.want_lq = want_lq
+
+    def close(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            _lib.load().ldpc_mi355_plan_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_outputs(self, device="cuda"):
+        import torch
+        out = dict(
+            hard=torch.zeros(max(self.hard_bytes, 1), dtype=torch.uint8, device=device),
+            status=torch.zeros(self.n, dtype=torch.int32, device=device),
+            iters=torch.zeros(self.n, dtype=torch.int32, device=device))
+        if self.want_lq:
+            dt = torch.int16 if self.lq_elem_size == 2 else torch.float32
+            out["lq"] = torch.zeros(self.lq_elems, dtype=dt, device=device)
+        return out
+
+    def launch(self, llr, out, soft_pool=None, stream=None):
+        """Asynchronous decode of device tensor ``llr`` (int16) into ``out``."""
+        if llr.dtype.itemsize != 2 or llr.numel() < self.llr_elems:
+            raise ValueError("llr must be int16 with every codeblock's E values")
+        if not llr.is_cuda:
+            raise ValueError("llr must be a device tensor")
+        st = _lib.load().ldpc_mi355_plan_decode(
+            self._handle, ptr(llr), ptr(soft_pool) if soft_pool is not None else None,
+            ptr(out["hard"]), ptr(out["status"]), ptr(out["iters"]),
+            ptr(out["lq"]) if self.want_lq else None,
+            ctypes.c_void_p(stream) if stream is not None else None)
+        if st != 0:
+            raise RuntimeError(f"plan_decode failed ({st})")
+        return out
+
+    def decode(self, llr, soft_pool=None):
+        """Synchronous convenience: torch device tensors in and out."""
+        import torch
+        out = self.alloc_outputs(llr.device)
+        stream = torch.cuda.current_stream(llr.device).cuda_stream
+        self.launch(llr, out, soft_pool, stream)
+        torch.cuda.current_stream(llr.device).synchronize()
+        return out
+
+    def split_hard(self, hard):
+        """Per-codeblock B/8-byte views of a host copy of ``hard``."""
+        res = []
+        for i, c in enumerate(self.cfgs):
+            B = c[1] + (16 if c[2] == 0 else 24)
+            o = int(self.hard_offsets[i])
+            res.append(hard[o:o + B // 8])
+        return res
+
+    def split_lq(self, lq):
+        res = []
+        for i, c in enumerate(self.cfgs):
+            zc, N = code_dims(c)
+            o = int(self.lq_offsets[i])
+            res.append(lq[o:o + N + 2 * zc])
+        return res
+
+
+def code_dims(cfg):
+    """(Zc, N) of a config via the library's gen_ldpc_parameters."""
+    L = _lib.load()
+    pc = _lib.LdpcConfig()
+    st = L.gen_ldpc_parameters(ctypes.byref(pc), ctypes.byref(LdpcInConfig(*cfg)))
+    if st != 0:
+        raise ValueError(f"invalid config {cfg}")
+    return int(pc.Zc), int(pc.N)
+
+
+def synth_batch(cfg, n, noise_power_db, seed=0, threads=8, want_payload=True):
+    """Host synthesis of n codeblocks (numpy int16 LLRs [n, E], payload bits)."""
+    L = _lib.load()
+    ic = LdpcInConfig(*[int(v) for v in cfg])
+    llr = np.zeros((n, ic.E), np.int16)
+    pay = np.zeros((n, ic.A), np.uint8) if want_payload else None
+    st = L.ldpc_mi355_synth_batch(ctypes.byref(ic), n, ctypes.c_float(noise_power_db),
+                                  ctypes.c_uint64(seed), threads, ptr(llr),
+                                  ptr(pay) if pay is not None else None)
+    if st != 0:
+        raise ValueError(f"invalid config {cfg}")
+    return llr, pay

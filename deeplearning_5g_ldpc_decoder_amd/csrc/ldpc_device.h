@@ -1,0 +1,61 @@
+// Device-side data layout shared by the HIP kernels and the host runtime.
+//
+// One decode launch processes a batch of codeblocks.  Everything a
+// workgroup needs about its codeblock is in one CbDesc (read once into
+// SGPRs); the arrays it reads and writes are flat device buffers addressed
+// by the offsets in the descriptor:
+//
+//   llr   int16  E values per codeblock, rate-matched + Qm-interleaved order
+//   soft  int16  N+2Zc values per codeblock (HARQ soft buffer, in/out)
+//   hard  uint8  B/8 bytes per codeblock, MSB-first
+//   lq    int16 | float  N+2Zc posteriors per codeblock (optional)
+//   work  float  nnz*Zc check-to-variable messages (BP only)
+//   etab  uint32 per (BG, Zc): one word per base-graph edge,
+//                low 16 bits = column*Zc, high 16 bits = V_ij mod Zc
+#pragma once
+#include <stdint.h>
+
+namespace nrldpc {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kMaxZc = 384;
+constexpr int kMaxThreads = 384;         // one lane per lifted check
+constexpr int kMaxSoft = 68 * kMaxZc;    // N + 2Zc for BG1 Zc=384
+
+struct CbDesc {
+    uint32_t llr_off;    // int16 element offset of the E channel LLRs
+    uint32_t hard_off;   // byte offset of the B/8 hard-decision bytes
+    uint32_t soft_off;   // int16 element offset of the soft buffer or kNone
+    uint32_t lq_off;     // element offset of the posterior output or kNone
+    uint32_t etab_off;   // uint32 element offset of the (BG, Zc) edge table
+    uint32_t work_off;   // float element offset of the BP message store
+    uint16_t bgn, Zc, B, K, N, E, k0, Qm;
+    uint8_t crctype, harq_on, pad0, pad1;
+};
+static_assert(sizeof(CbDesc) == 44, "CbDesc layout");
+
+// Decoder row (c_lib/nr_ldpc.h:82-90) pre-converted for the kernels.
+struct DecParams {
+    int32_t max_it;
+    int32_t alpha, beta;        // algo 1: uint16 fields wrapped to int16
+    int32_t max_llr, min_llr;   // int16 clip bounds
+    int32_t filler;             // filler-bit LLR (int16)
+    float falpha, fbeta;        // algo 4: alpha/32768, beta/128
+    float fmax, fmin;           // algo 4: max_llr/128, min_llr/128
+};
+
+// Per-codeblock result flags (status) follow c_ldpc_5g_decode: 0 CRC pass,
+// 1 not converged.
+struct DecOut {
+    const CbDesc *desc;
+    const int16_t *llr;
+    int16_t *soft;
+    uint8_t *hard;
+    int32_t *status;
+    int32_t *iters;
+    void *lq;
+    float *work;
+    const uint32_t *etab;
+};
+
+}  // namespace nrldpc

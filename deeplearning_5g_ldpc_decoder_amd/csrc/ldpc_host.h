@@ -1,0 +1,30 @@
+// Host-side helpers shared by the library's translation units.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/nr_ldpc_mi355.h"
+
+namespace nrldpc {
+
+// Derived code parameters (c_lib/nr_ldpc.h:54-79 without table pointers).
+struct Params {
+    int bgn = 0, A = 0, crctype = 0, E = 0, rv = 0, Qm = 0, harq_on = 0;
+    int B = 0, Zc = 0, iLS = 0, K = 0, F = 0, N = 0, k0 = 0;
+    int rows = 0, edges = 0;
+};
+
+// nullptr on success, else the reason the config is invalid.
+const char *derive(const ldpc_in_config_t &in, Params &p);
+void print_invalid(const ldpc_in_config_t &in, const char *why);
+
+// Lifting-size set index of zc (Table 5.3.2-1), or -1.
+int lifting_set(int zc);
+const uint16_t *all_lifting_sizes(int *count);
+
+// Device edge table of (bgn, zc): per edge (column * zc) | (V_ij mod zc) << 16.
+void edge_table(int bgn, int zc, int ils, uint32_t *out);
+
+void encode_codeword(const Params &p, const uint8_t *bits_b, uint8_t *cw);
+void rate_match_bits(const Params &p, const uint8_t *cw, uint8_t *out_bits);
+
+}  // namespace nrldpc

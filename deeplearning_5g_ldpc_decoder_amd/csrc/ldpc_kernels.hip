@@ -1,0 +1,589 @@
+// HIP kernels (gfx950) for the 5G NR LDPC decode path.
+//
+// Mapping: one workgroup decodes one codeblock; lane zx owns lifted check zx
+// of every base-graph row.  The layered schedule of the reference
+// (c_lib/fixedpoint_ldpc_decoder.c:182-212) walks the rows in order and the
+// Zc lifted checks of one row touch disjoint variable nodes, so running them
+// in parallel and putting a workgroup barrier between rows reproduces the
+// reference's serial order bit for bit.
+//
+// State:
+//   * variable-node beliefs LQ: LDS (int16 or float, N+2Zc entries), filled by
+//     the fused rate recovery / HARQ combine;
+//   * min-sum check-to-variable messages: VGPRs, compressed per lifted check
+//     to (min1', min2', argmin, one sign bit per edge) -- the reference's
+//     message of edge j is exactly sign_j * (j == argmin ? min2' : min1'),
+//     so the compressed form re-expands bit-exactly;
+//   * BP messages (no compressed form): a float array in HBM, edge-major so
+//     one row edge of all lanes is one coalesced 4*Zc-byte access.
+// Rows are unrolled at compile time per base graph so the per-row state sits
+// at fixed register indices.
+//
+// After every iteration the hard decisions of the first B beliefs are CRC'd
+// in parallel: each lane runs the byte-table CRC over its slice of bytes,
+// shifts its remainder by x^(8*bytes_after) mod g (GF(2) linearity), and the
+// remainders are XOR-reduced across the workgroup.  A zero remainder stops the
+// codeblock (c_lib/fixedpoint_ldpc_decoder.c:214-219).
+#include <hip/hip_runtime.h>
+
+#include "ldpc_device.h"
+#include "nr_bg_tables.h"
+
+namespace nrldpc {
+
+template <int BG> struct Graph;
+template <> struct Graph<1> {
+    static constexpr int kRows = NR_BG1_ROWS;
+    __host__ __device__ static constexpr int start(int r) { return kBG1RowStart[r]; }
+};
+template <> struct Graph<2> {
+    static constexpr int kRows = NR_BG2_ROWS;
+    __host__ __device__ static constexpr int start(int r) { return kBG2RowStart[r]; }
+};
+
+// ---------------------------------------------------------------------------
+// CRC (TS 38.212 5.1; c_lib/nr_crc.c:129-179): MSB-first, init 0.
+// ---------------------------------------------------------------------------
+struct Crc {
+    uint32_t poly, mask;
+    int w;
+};
+
+__device__ inline Crc crc_spec(int type) {
+    switch (type) {
+    case 0: return {0x1021u, 0xFFFFu, 16};
+    case 1: return {0x864CFBu, 0xFFFFFFu, 24};
+    case 2: return {0x800063u, 0xFFFFFFu, 24};
+    default: return {0xB2B117u, 0xFFFFFFu, 24};
+    }
+}
+
+__device__ inline uint32_t crc_xtimes(uint32_t r, const Crc &c) {
+    const uint32_t top = (r >> (c.w - 1)) & 1u;
+    r = (r << 1) & c.mask;
+    return top ? r ^ c.poly : r;
+}
+
+// a(x) * b(x) mod g(x) for remainders a, b of degree < w.
+__device__ inline uint32_t crc_mulmod(uint32_t a, uint32_t b, const Crc &c) {
+    uint32_t r = 0;
+    for (int i = c.w - 1; i >= 0; --i) {
+        r = crc_xtimes(r, c);
+        if ((b >> i) & 1u) r ^= a;
+    }
+    return r;
+}
+
+// x^(8m) mod g(x)
+__device__ inline uint32_t crc_x8pow(int m, const Crc &c) {
+    uint32_t res = 1u, base = 0x100u;
+    while (m) {
+        if (m & 1) res = crc_mulmod(res, base, c);
+        base = crc_mulmod(base, base, c);
+        m >>= 1;
+    }
+    return res;
+}
+
+__device__ inline uint32_t wave_xor(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v ^= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Hard-decision byte from 8 consecutive beliefs (bit 7 = first belief).
+__device__ inline uint32_t hard_byte(const int16_t *lq, int b) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(lq + 8 * b);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t out = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        out = (out << 1) | ((w[i] >> 15) & 1u);
+        out = (out << 1) | ((w[i] >> 31) & 1u);
+    }
+    return out;
+}
+
+__device__ inline uint32_t hard_byte(const float *lq, int b) {
+    const float4 a = *reinterpret_cast<const float4 *>(lq + 8 * b);
+    const float4 c = *reinterpret_cast<const float4 *>(lq + 8 * b + 4);
+    const float f[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+    uint32_t out = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out = (out << 1) | (f[i] < 0.0f ? 1u : 0u);
+    return out;
+}
+
+// Shared per-workgroup CRC context.
+struct CrcCtx {
+    Crc spec;
+    uint32_t mult;     // x^(8 * bytes after this lane's slice) mod g
+    int b_lo, b_hi;    // this lane's byte slice [b_lo, b_hi)
+};
+
+__device__ inline CrcCtx crc_setup(int crctype, int nbytes, uint32_t *tab) {
+    CrcCtx ctx;
+    ctx.spec = crc_spec(crctype);
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int i = t; i < 256; i += nt) {
+        uint32_t r = (uint32_t)i << (ctx.spec.w - 8);
+        for (int k = 0; k < 8; ++k) r = crc_xtimes(r, ctx.spec);
+        tab[i] = r;
+    }
+    const int per = (nbytes + nt - 1) / nt;
+    ctx.b_lo = t * per < nbytes ? t * per : nbytes;
+    ctx.b_hi = (t + 1) * per < nbytes ? (t + 1) * per : nbytes;
+    ctx.mult = (ctx.b_hi > ctx.b_lo) ? crc_x8pow(nbytes - ctx.b_hi, ctx.spec) : 0u;
+    return ctx;
+}
+
+// Whole-codeblock CRC of the hard decisions; every lane gets the result.
+// Contains one workgroup barrier.  Caller guarantees lq is settled.
+template <typename T>
+__device__ inline uint32_t block_crc(const T *lq, const CrcCtx &ctx,
+                                     const uint32_t *tab, uint32_t *slots) {
+    const Crc &c = ctx.spec;
+    uint32_t r = 0;
+    for (int b = ctx.b_lo; b < ctx.b_hi; ++b) {
+        const uint32_t byte = hard_byte(lq, b);
+        r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
+    }
+    uint32_t part = r ? crc_mulmod(r, ctx.mult, c) : 0u;
+    part = wave_xor(part);
+    const int wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) slots[wave] = part;
+    __syncthreads();
+    uint32_t total = 0;
+    for (int w = 0; w < nwaves; ++w) total ^= slots[w];
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// Rate recovery + HARQ combine fused into the LDS fill
+// (c_lib/ldpc_rate_recovery.c:24-65).  Gather form: buffer position p takes
+// the sum of every LLR k with compact index q(p) = (ck0 + k) mod (N - F),
+// k < E; the sum wraps mod 2^16 exactly like the reference's int16 +=.
+// ---------------------------------------------------------------------------
+__device__ inline int16_t recover_one(int p, const CbDesc &d, const int16_t *llr,
+                                      const int16_t *soft_in, int filler) {
+    const int Zc = d.Zc, B = d.B, K = d.K, F = K - B;
+    int acc = (d.harq_on && soft_in) ? (int)soft_in[p] : 0;
+    if (p < 2 * Zc) return (int16_t)acc;
+    if (p >= B && p < K) return (int16_t)filler;
+    const int pos = p - 2 * Zc;
+    const int lo = B - 2 * Zc, hi = K - 2 * Zc;
+    const int nc = d.N - F;
+    const int ck0 = d.k0 < lo ? d.k0 : (d.k0 < hi ? lo : d.k0 - F);
+    const int q = pos < lo ? pos : pos - F;
+    int k = q - ck0;
+    if (k < 0) k += nc;
+    const int S = d.E / d.Qm;
+    for (; k < d.E; k += nc) {
+        int sym = k, bit = 0;
+        while (sym >= S) { sym -= S; ++bit; }
+        acc += llr[sym * d.Qm + bit];
+    }
+    return (int16_t)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Algo 1: int16 normalized/offset min-sum
+// (c_lib/fixedpoint_ldpc_decoder.c:28-66, 103-144).
+// ---------------------------------------------------------------------------
+template <int BG>
+struct StateI16 {
+    uint32_t mag[Graph<BG>::kRows];   // min1' | min2' << 16
+    uint32_t sgn[Graph<BG>::kRows];   // sign bit per edge | argmin << 24
+};
+
+__device__ inline int16_t scale_i16(int m, const DecParams &p) {
+    int off = m - p.beta;
+    if (off < 0) off = 0;
+    const int scaled = p.alpha * off;
+    return (int16_t)((scaled + (1 << 14)) >> 15);
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &st,
+                                        const uint32_t *__restrict__ etab, int zx,
+                                        int Zc, const DecParams &p) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    const uint32_t mag = st.mag[R], sb = st.sgn[R];
+    const int16_t o1 = (int16_t)(mag & 0xFFFFu), o2 = (int16_t)(mag >> 16);
+    const int oidx = (int)(sb >> 24);
+    int addr[deg];
+    int16_t rel[deg];
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const uint32_t t = etab[e0 + j];
+        int c = zx + (int)(t >> 16);
+        c = (c >= Zc) ? c - Zc : c;
+        addr[j] = (int)(t & 0xFFFFu) + c;
+        const int16_t ov = (j == oidx) ? o2 : o1;
+        const int16_t lr = ((sb >> j) & 1u) ? (int16_t)(-(int)ov) : ov;
+        rel[j] = (int16_t)((int)lq[addr[j]] - (int)lr);
+    }
+    int m1 = 32767, m2 = 32767, idx = 255;
+    uint32_t negs = 0;
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const int v = rel[j];
+        const bool neg = v < 0;
+        negs |= (uint32_t)neg << j;
+        const int a = neg ? (v == -32768 ? 32767 : -v) : v;
+        const bool lt = a < m1;
+        m2 = lt ? m1 : (a < m2 ? a : m2);
+        idx = lt ? j : idx;
+        m1 = lt ? a : m1;
+    }
+    if (m1 > p.max_llr) m1 = p.max_llr;
+    if (m2 > p.max_llr) m2 = p.max_llr;
+    const int16_t n1 = scale_i16(m1, p), n2 = scale_i16(m2, p);
+    const uint32_t mask = (deg >= 32) ? 0xFFFFFFFFu : ((1u << deg) - 1u);
+    const uint32_t bits = ((__popc(negs) & 1) ? ~negs : negs) & mask;
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const int16_t nv = (j == idx) ? n2 : n1;
+        const int16_t lr = ((bits >> j) & 1u) ? (int16_t)(-(int)nv) : nv;
+        int v = (int)rel[j] + (int)lr;
+        if (v > p.max_llr) v = p.max_llr;
+        else if (v < p.min_llr) v = p.min_llr;
+        lq[addr[j]] = (int16_t)v;
+    }
+    st.mag[R] = (uint32_t)(uint16_t)n1 | ((uint32_t)(uint16_t)n2 << 16);
+    st.sgn[R] = bits | ((uint32_t)idx << 24);
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void rows_i16(int16_t *lq, StateI16<BG> &st, const uint32_t *etab,
+                                         int zx, int Zc, const DecParams &p) {
+    if constexpr (R < Graph<BG>::kRows) {
+        if (zx < Zc) row_i16<BG, R>(lq, st, etab, zx, Zc, p);
+        __syncthreads();
+        rows_i16<BG, R + 1>(lq, st, etab, zx, Zc, p);
+    }
+}
+
+template <int BG>
+__device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
+                       const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
+    StateI16<BG> st;
+#pragma unroll
+    for (int r = 0; r < Graph<BG>::kRows; ++r) {
+        st.mag[r] = 0u;
+        st.sgn[r] = 255u << 24;
+    }
+    const int zx = threadIdx.x;
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        rows_i16<BG, 0>(lq, st, etab, zx, d.Zc, p);
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Algo 4: FP32 min-sum (c_lib/floatpoint_ldpc_decoder.c:153-220).  Single
+// IEEE operations only (built with -ffp-contract=off): bit-exact with the
+// reference's FP32 arithmetic.
+// ---------------------------------------------------------------------------
+template <int BG>
+struct StateF32 {
+    float n1[Graph<BG>::kRows];
+    float n2[Graph<BG>::kRows];
+    uint32_t sgn[Graph<BG>::kRows];   // sign bit per edge | argmin << 24
+};
+
+template <int BG, int R>
+__device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st,
+                                          const uint32_t *__restrict__ etab, int zx,
+                                          int Zc, const DecParams &p) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    const float o1 = st.n1[R], o2 = st.n2[R];
+    const uint32_t sb = st.sgn[R];
+    const int oidx = (int)(sb >> 24);
+    int addr[deg];
+    float rel[deg];
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const uint32_t t = etab[e0 + j];
+        int c = zx + (int)(t >> 16);
+        c = (c >= Zc) ? c - Zc : c;
+        addr[j] = (int)(t & 0xFFFFu) + c;
+        const float ov = (j == oidx) ? o2 : o1;
+        const float lr = ((sb >> j) & 1u) ? -ov : ov;
+        rel[j] = lq[addr[j]] - lr;
+    }
+    float m1 = 3.402823466e+38f, m2 = 3.402823466e+38f;
+    int idx = 0;
+    uint32_t negs = 0;
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const bool neg = rel[j] < 0.0f;
+        negs |= (uint32_t)neg << j;
+        const float a = fabsf(rel[j]);
+        const bool lt = a < m1;
+        m2 = lt ? m1 : (a < m2 ? a : m2);
+        idx = lt ? j : idx;
+        m1 = lt ? a : m1;
+    }
+    if (m1 > p.fmax) m1 = p.fmax;
+    if (m1 < 0.0f) m1 = 0.0f;
+    if (m2 > p.fmax) m2 = p.fmax;
+    if (m2 < 0.0f) m2 = 0.0f;
+    float off1 = m1 - p.fbeta, off2 = m2 - p.fbeta;
+    if (off1 < 0.0f) off1 = 0.0f;
+    if (off2 < 0.0f) off2 = 0.0f;
+    const float n1 = p.falpha * off1, n2 = p.falpha * off2;
+    const uint32_t mask = (1u << deg) - 1u;
+    const uint32_t bits = ((__popc(negs) & 1) ? ~negs : negs) & mask;
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const float nv = (j == idx) ? n2 : n1;
+        const float lr = ((bits >> j) & 1u) ? -nv : nv;
+        float v = rel[j] + lr;
+        if (v > p.fmax) v = p.fmax;
+        if (v < p.fmin) v = p.fmin;
+        lq[addr[j]] = v;
+    }
+    st.n1[R] = n1;
+    st.n2[R] = n2;
+    st.sgn[R] = bits | ((uint32_t)idx << 24);
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void rows_f32ms(float *lq, StateF32<BG> &st, const uint32_t *etab,
+                                           int zx, int Zc, const DecParams &p) {
+    if constexpr (R < Graph<BG>::kRows) {
+        if (zx < Zc) row_f32ms<BG, R>(lq, st, etab, zx, Zc, p);
+        __syncthreads();
+        rows_f32ms<BG, R + 1>(lq, st, etab, zx, Zc, p);
+    }
+}
+
+template <int BG>
+__device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
+                         const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
+    StateF32<BG> st;
+#pragma unroll
+    for (int r = 0; r < Graph<BG>::kRows; ++r) {
+        st.n1[r] = 0.0f;
+        st.n2[r] = 0.0f;
+        st.sgn[r] = 255u << 24;
+    }
+    const int zx = threadIdx.x;
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        rows_f32ms<BG, 0>(lq, st, etab, zx, d.Zc, p);
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Algo 0: FP32 layered BP (c_lib/floatpoint_ldpc_decoder.c:53-151).  Exact
+// zeros are special-cased like the reference; messages saturate at
+// +-2*19.07 when the extrinsic product reaches +-1; LQ is not clipped.
+// Device tanhf/atanhf: results track the reference within tolerance.
+// ---------------------------------------------------------------------------
+__device__ inline float bp_msg(float prod) {
+    const float sat = 2.0f * 19.07f;
+    if (prod >= 1.0f) return sat;
+    if (prod <= -1.0f) return -sat;
+    return 2.0f * atanhf(prod);
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void row_bp(float *__restrict__ lq, float *__restrict__ lr_mem,
+                                       const uint32_t *__restrict__ etab, int zx, int Zc,
+                                       bool first) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    int addr[deg];
+    float rel[deg], t[deg];
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const uint32_t tt = etab[e0 + j];
+        int c = zx + (int)(tt >> 16);
+        c = (c >= Zc) ? c - Zc : c;
+        addr[j] = (int)(tt & 0xFFFFu) + c;
+        const float old = first ? 0.0f : lr_mem[(e0 + j) * Zc + zx];
+        rel[j] = lq[addr[j]] - old;
+    }
+    int nz = 0, zidx = -1;
+    float prod = 1.0f;
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const bool z = rel[j] == 0.0f;
+        t[j] = z ? 0.0f : tanhf(rel[j] / 2.0f);
+        if (z) { ++nz; zidx = j; }
+        else prod *= t[j];
+    }
+    float msg[deg];
+    if (nz >= 1) {
+        const float m = (nz == 1) ? bp_msg(prod) : 0.0f;
+#pragma unroll
+        for (int j = 0; j < deg; ++j) msg[j] = (j == zidx) ? m : 0.0f;
+    } else {
+        float pre[deg + 1], suf[deg + 1];
+        pre[0] = 1.0f;
+#pragma unroll
+        for (int j = 0; j < deg; ++j) pre[j + 1] = pre[j] * t[j];
+        suf[deg] = 1.0f;
+#pragma unroll
+        for (int j = deg - 1; j >= 0; --j) suf[j] = suf[j + 1] * t[j];
+#pragma unroll
+        for (int j = 0; j < deg; ++j) msg[j] = bp_msg(pre[j] * suf[j + 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        lr_mem[(e0 + j) * Zc + zx] = msg[j];
+        lq[addr[j]] = rel[j] + msg[j];
+    }
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void rows_bp(float *lq, float *lr_mem, const uint32_t *etab, int zx,
+                                        int Zc, bool first) {
+    if constexpr (R < Graph<BG>::kRows) {
+        if (zx < Zc) row_bp<BG, R>(lq, lr_mem, etab, zx, Zc, first);
+        __syncthreads();
+        rows_bp<BG, R + 1>(lq, lr_mem, etab, zx, Zc, first);
+    }
+}
+
+template <int BG>
+__device__ int run_bp(float *lq, float *lr_mem, const uint32_t *etab, const CbDesc &d,
+                      const DecParams &p, const CrcCtx &crc, const uint32_t *tab,
+                      uint32_t *slots, int *status) {
+    const int zx = threadIdx.x;
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        rows_bp<BG, 0>(lq, lr_mem, etab, zx, d.Zc, it == 0);
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4, kBpF32 = 0 };
+
+template <typename T> __device__ inline T from_soft(int16_t v);
+template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
+template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
+
+template <Algo A, typename T>
+__global__ void __launch_bounds__(kMaxThreads)
+ldpc_decode_kernel(DecOut o, DecParams p) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t slots[kMaxThreads / 64];
+    T *lq = reinterpret_cast<T *>(smem);
+
+    const CbDesc d = o.desc[blockIdx.x];
+    const int n = d.N + 2 * d.Zc;
+    const int t = threadIdx.x, nt = blockDim.x;
+
+    // fused rate recovery + HARQ combine into LDS; soft buffer write-back
+    const int16_t *llr = o.llr + d.llr_off;
+    int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+    for (int i = t; i < n; i += nt) {
+        const int16_t v = recover_one(i, d, llr, soft, p.filler);
+        if (soft) soft[i] = v;
+        lq[i] = from_soft<T>(v);
+    }
+    const CrcCtx crc = crc_setup(d.crctype, d.B / 8, crc_tab);
+    __syncthreads();
+
+    const uint32_t *etab = o.etab + d.etab_off;
+    int status = 1, it = 0;
+    if (p.max_it > 0) {
+        if constexpr (A == Algo::kMinSumI16) {
+            it = (d.bgn == 1) ? run_i16<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
+                              : run_i16<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
+        } else if constexpr (A == Algo::kMinSumF32) {
+            it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
+                              : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
+        } else {
+            float *lr_mem = o.work + d.work_off;
+            it = (d.bgn == 1) ? run_bp<1>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status)
+                              : run_bp<2>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status);
+        }
+        // hard decisions of the final iteration (left untouched when no
+        // iteration ran, like the reference)
+        for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
+    }
+    if (t == 0) {
+        o.status[blockIdx.x] = status;
+        o.iters[blockIdx.x] = it;
+    }
+    if (d.lq_off != kNone) {
+        T *out = reinterpret_cast<T *>(o.lq) + d.lq_off;
+        for (int i = t; i < n; i += nt) out[i] = lq[i];
+    }
+}
+
+}  // namespace nrldpc
+
+// Host-callable launchers (C linkage inside the library only).
+extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
+                                           const nrldpc::DecParams *p, int ncb, int threads,
+                                           int max_soft, hipStream_t stream) {
+    using namespace nrldpc;
+    const size_t esz = (algo == 1) ? sizeof(int16_t) : sizeof(float);
+    const size_t lds = ((size_t)max_soft * esz + 15) & ~(size_t)15;
+    const dim3 grid(ncb), block(threads);
+    switch (algo) {
+    case 1:
+        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t>), grid, block, lds,
+                           stream, *o, *p);
+        break;
+    case 4:
+        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumF32, float>), grid, block, lds,
+                           stream, *o, *p);
+        break;
+    case 0:
+        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kBpF32, float>), grid, block, lds, stream,
+                           *o, *p);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t nrldpc_configure_kernels() {
+    using namespace nrldpc;
+    const int max_lds = 160 * 1024 - 2048;
+    hipError_t e;
+    e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+}

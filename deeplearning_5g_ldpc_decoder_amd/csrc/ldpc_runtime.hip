@@ -1,0 +1,331 @@
+// Device runtime of libnrldpc_mi355.so: device context, the per-codeblock
+// reference ABI (c_ldpc_5g_decode) and batched decode plans.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/nr_ldpc_mi355.h"
+#include "ldpc_device.h"
+#include "ldpc_host.h"
+
+using namespace nrldpc;
+
+extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecParams *p,
+                                           int ncb, int threads, int max_soft,
+                                           hipStream_t stream);
+extern "C" hipError_t nrldpc_configure_kernels();
+
+namespace {
+
+#define HIP_OK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            fprintf(stderr, "libnrldpc_mi355: %s failed: %s (%s:%d)\n", #expr,        \
+                    hipGetErrorString(_e), __FILE__, __LINE__);                       \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+constexpr int kStatusBytes = 16;                  // status, iterations
+constexpr int kHardBytes = 8448 / 8 + 16;         // B/8 max, padded
+constexpr size_t kSoftBytes = (size_t)kMaxSoft * 2;
+
+// Per-codeblock staging layouts (one H2D and one D2H copy per call).
+struct InLayout {
+    static constexpr size_t desc = 0;
+    static constexpr size_t llr = 64;
+    static constexpr size_t soft = llr + 65536 * 2;
+    static constexpr size_t total = soft + kSoftBytes;
+};
+struct OutLayout {
+    static constexpr size_t status = 0;
+    static constexpr size_t hard = kStatusBytes;
+    static constexpr size_t soft = hard + kHardBytes;
+    static constexpr size_t lq = soft + kSoftBytes;
+    static constexpr size_t total = lq + (size_t)kMaxSoft * 4;
+};
+
+struct Runtime {
+    std::mutex mu;             // serializes the per-codeblock path
+    std::once_flag once;
+    int init_status = -1;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *d_etab = nullptr;
+    uint32_t etab_off[2][kMaxZc + 1] = {};
+    unsigned char *h_in = nullptr, *h_out = nullptr;   // pinned
+    unsigned char *d_in = nullptr, *d_out = nullptr;
+    float *d_work = nullptr;                           // BP messages, one codeblock
+
+    int init() {
+        HIP_OK(hipGetDevice(&device));
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_OK(nrldpc_configure_kernels());
+        int nz = 0;
+        const uint16_t *zs = all_lifting_sizes(&nz);
+        std::vector<uint32_t> tab;
+        for (int bg = 1; bg <= 2; ++bg) {
+            const int ne = bg == 1 ? BG1_MAX_NON_NULL_ELEMENTS : BG2_MAX_NON_NULL_ELEMENTS;
+            for (int i = 0; i < nz; ++i) {
+                const int zc = zs[i];
+                etab_off[bg - 1][zc] = (uint32_t)tab.size();
+                tab.resize(tab.size() + ne);
+                edge_table(bg, zc, lifting_set(zc), tab.data() + etab_off[bg - 1][zc]);
+            }
+        }
+        HIP_OK(hipMalloc(&d_etab, tab.size() * sizeof(uint32_t)));
+        HIP_OK(hipMemcpy(d_etab, tab.data(), tab.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+        HIP_OK(hipHostMalloc(&h_in, InLayout::total, hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(&h_out, OutLayout::total, hipHostMallocDefault));
+        HIP_OK(hipMalloc(&d_in, InLayout::total));
+        HIP_OK(hipMalloc(&d_out, OutLayout::total));
+        HIP_OK(hipMalloc(&d_work, (size_t)BG1_MAX_NON_NULL_ELEMENTS * kMaxZc * sizeof(float)));
+        return 0;
+    }
+
+    int ensure() {
+        std::call_once(once, [this] { init_status = init(); });
+        return init_status;
+    }
+};
+
+Runtime &rt() {
+    static Runtime r;
+    return r;
+}
+
+bool algo_supported(int algo) { return algo == 0 || algo == 1 || algo == 4; }
+
+DecParams make_params(const ldpc_decoder_cfg_t &c) {
+    DecParams p;
+    p.max_it = c.max_iterations;
+    p.alpha = (int16_t)c.alpha;        // int16 view, as the reference's helpers
+    p.beta = (int16_t)c.beta;
+    p.max_llr = c.max_llr;
+    p.min_llr = c.min_llr;
+    p.filler = c.filling_bit_value;
+    p.falpha = (float)c.alpha / 32768.0f;
+    p.fbeta = (float)c.beta / 128.0f;
+    p.fmax = (float)c.max_llr / 128.0f;
+    p.fmin = (float)c.min_llr / 128.0f;
+    return p;
+}
+
+CbDesc make_desc(const Params &p, const Runtime &r) {
+    CbDesc d;
+    memset(&d, 0, sizeof d);
+    d.bgn = (uint16_t)p.bgn; d.Zc = (uint16_t)p.Zc; d.B = (uint16_t)p.B;
+    d.K = (uint16_t)p.K; d.N = (uint16_t)p.N; d.E = (uint16_t)p.E;
+    d.k0 = (uint16_t)p.k0; d.Qm = (uint16_t)p.Qm;
+    d.crctype = (uint8_t)p.crctype; d.harq_on = (uint8_t)p.harq_on;
+    d.etab_off = r.etab_off[p.bgn - 1][p.Zc];
+    d.soft_off = kNone; d.lq_off = kNone;
+    return d;
+}
+
+int threads_for(int zc) { return ((zc + 63) / 64) * 64; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Per-codeblock path (the reference ABI): one H2D, one launch, one D2H.
+// ---------------------------------------------------------------------------
+extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
+                                        ldpc_in_config_t *inconfig,
+                                        ldpc_decoder_cfg_t *decoder_config, int32_t *iterations,
+                                        void *posteriors) {
+    if (iterations) *iterations = 0;
+    Params p;
+    if (const char *why = derive(*inconfig, p)) {
+        print_invalid(*inconfig, why);
+        return -2;
+    }
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    std::lock_guard<std::mutex> guard(r.mu);
+
+    const bool valid_algo = algo_supported(decoder_config->algo);
+    const int algo = valid_algo ? decoder_config->algo : 1;
+    DecParams dp = make_params(*decoder_config);
+    if (!valid_algo) dp.max_it = 0;   // rate recovery only, like the reference
+    const int n = p.N + 2 * p.Zc;
+
+    CbDesc d = make_desc(p, r);
+    d.llr_off = 0;
+    d.hard_off = 0;
+    d.soft_off = 0;
+    d.lq_off = (posteriors && valid_algo) ? 0u : kNone;
+    d.work_off = 0;
+    memcpy(r.h_in + InLayout::desc, &d, sizeof d);
+    memcpy(r.h_in + InLayout::llr, LLr, (size_t)p.E * 2);
+    if (p.harq_on) memcpy(r.h_in + InLayout::soft, HARQ_memory, (size_t)n * 2);
+    const size_t in_bytes = p.harq_on ? InLayout::soft + (size_t)n * 2
+                                      : InLayout::llr + (size_t)p.E * 2;
+    HIP_OK(hipMemcpyAsync(r.d_in, r.h_in, in_bytes, hipMemcpyHostToDevice, r.stream));
+    // the kernel reads the soft buffer only when HARQ_on; it always writes it
+    DecOut o;
+    o.desc = reinterpret_cast<const CbDesc *>(r.d_in + InLayout::desc);
+    o.llr = reinterpret_cast<const int16_t *>(r.d_in + InLayout::llr);
+    o.soft = reinterpret_cast<int16_t *>(r.d_in + InLayout::soft);
+    o.hard = r.d_out + OutLayout::hard;
+    o.status = reinterpret_cast<int32_t *>(r.d_out + OutLayout::status);
+    o.iters = o.status + 1;
+    o.lq = r.d_out + OutLayout::lq;
+    o.work = r.d_work;
+    o.etab = r.d_etab;
+    HIP_OK(nrldpc_launch_decode(algo, &o, &dp, 1, threads_for(p.Zc), n, r.stream));
+    // results: status/iterations/hard contiguous, then the soft buffer
+    HIP_OK(hipMemcpyAsync(r.h_out, r.d_out, OutLayout::soft, hipMemcpyDeviceToHost, r.stream));
+    HIP_OK(hipMemcpyAsync(r.h_out + OutLayout::soft, r.d_in + InLayout::soft, (size_t)n * 2,
+                          hipMemcpyDeviceToHost, r.stream));
+    const size_t lq_bytes = (size_t)n * (algo == 1 ? 2 : 4);
+    if (d.lq_off != kNone)
+        HIP_OK(hipMemcpyAsync(r.h_out + OutLayout::lq, r.d_out + OutLayout::lq, lq_bytes,
+                              hipMemcpyDeviceToHost, r.stream));
+    HIP_OK(hipStreamSynchronize(r.stream));
+
+    int32_t st, its;
+    memcpy(&st, r.h_out + OutLayout::status, 4);
+    memcpy(&its, r.h_out + OutLayout::status + 4, 4);
+    memcpy(HARQ_memory, r.h_out + OutLayout::soft, (size_t)n * 2);
+    if (!valid_algo) {
+        fprintf(stderr, "Invalid decoding algorithm selection: %u\n", decoder_config->algo);
+        return -3;
+    }
+    if (its > 0) memcpy(hardbits, r.h_out + OutLayout::hard, (size_t)p.B / 8);
+    if (iterations) *iterations = its;
+    if (posteriors) memcpy(posteriors, r.h_out + OutLayout::lq, lq_bytes);
+    return st;
+}
+
+extern "C" int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                                    ldpc_decoder_cfg_t *decoder_config) {
+    return ldpc_mi355_decode_ex(LLr, hardbits, inconfig, decoder_config, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Batched plans
+// ---------------------------------------------------------------------------
+struct ldpc_mi355_plan {
+    int n = 0, algo = 1, threads = 64, max_soft = 0;
+    DecParams dp;
+    CbDesc *d_desc = nullptr;
+    float *d_work = nullptr;
+    uint64_t hard_bytes = 0, lq_elems = 0;
+    std::vector<uint32_t> hard_off, lq_off;
+    bool want_lq = false;
+};
+
+extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs,
+                                          int32_t n, const ldpc_decoder_cfg_t *dec,
+                                          uint32_t flags, int32_t *bad_index) {
+    *plan = nullptr;
+    if (bad_index) *bad_index = -1;
+    if (n <= 0) return -2;
+    if (!algo_supported(dec->algo)) return -3;
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    auto *pl = new ldpc_mi355_plan;
+    pl->n = n;
+    pl->algo = dec->algo;
+    pl->dp = make_params(*dec);
+    pl->want_lq = (flags & 1u) != 0;
+    std::vector<CbDesc> descs((size_t)n);
+    pl->hard_off.resize(n);
+    pl->lq_off.resize(n);
+    uint64_t work = 0;
+    int max_zc = 2;
+    for (int i = 0; i < n; ++i) {
+        Params p;
+        if (const char *why = derive(cbs[i].cfg, p)) {
+            print_invalid(cbs[i].cfg, why);
+            if (bad_index) *bad_index = i;
+            delete pl;
+            return -2;
+        }
+        CbDesc d = make_desc(p, r);
+        d.llr_off = cbs[i].llr_offset;
+        d.hard_off = (uint32_t)pl->hard_bytes;
+        d.soft_off = cbs[i].soft_slot >= 0 ? (uint32_t)cbs[i].soft_slot * (uint32_t)kMaxSoft
+                                           : kNone;
+        d.lq_off = pl->want_lq ? (uint32_t)pl->lq_elems : kNone;
+        d.work_off = (uint32_t)work;
+        pl->hard_off[i] = d.hard_off;
+        pl->lq_off[i] = pl->want_lq ? (uint32_t)pl->lq_elems : 0u;
+        pl->hard_bytes += (uint64_t)p.B / 8;
+        pl->lq_elems += (uint64_t)(p.N + 2 * p.Zc);
+        if (pl->algo == 0) work += (uint64_t)p.edges * p.Zc;
+        if (p.Zc > max_zc) max_zc = p.Zc;
+        if (p.N + 2 * p.Zc > pl->max_soft) pl->max_soft = p.N + 2 * p.Zc;
+        descs[i] = d;
+    }
+    pl->threads = threads_for(max_zc);
+    if (hipMalloc(&pl->d_desc, sizeof(CbDesc) * (size_t)n) != hipSuccess ||
+        hipMemcpy(pl->d_desc, descs.data(), sizeof(CbDesc) * (size_t)n,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        (work && hipMalloc(&pl->d_work, work * sizeof(float)) != hipSuccess)) {
+        fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
+        ldpc_mi355_plan_destroy(pl);
+        return -1;
+    }
+    *plan = pl;
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_plan_layout(const ldpc_mi355_plan_t *plan, uint64_t *hard_bytes,
+                                          uint64_t *lq_elems, int32_t *lq_elem_size,
+                                          uint32_t *hard_offsets, uint32_t *lq_offsets) {
+    if (!plan) return -1;
+    if (hard_bytes) *hard_bytes = plan->hard_bytes;
+    if (lq_elems) *lq_elems = plan->want_lq ? plan->lq_elems : 0;
+    if (lq_elem_size) *lq_elem_size = plan->algo == 1 ? 2 : 4;
+    if (hard_offsets) memcpy(hard_offsets, plan->hard_off.data(), 4 * (size_t)plan->n);
+    if (lq_offsets) memcpy(lq_offsets, plan->lq_off.data(), 4 * (size_t)plan->n);
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const int16_t *llr,
+                                          int16_t *soft_pool, uint8_t *hard, int32_t *status,
+                                          int32_t *iterations, void *lq, void *stream) {
+    if (!plan) return -1;
+    if (plan->want_lq && !lq) return -1;
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    DecOut o;
+    o.desc = plan->d_desc;
+    o.llr = llr;
+    o.soft = soft_pool;
+    o.hard = hard;
+    o.status = status;
+    o.iters = iterations;
+    o.lq = lq;
+    o.work = plan->d_work;
+    o.etab = r.d_etab;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : r.stream;
+    HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
+                                plan->max_soft, s));
+    return 0;
+}
+
+extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
+    if (!plan) return;
+    if (plan->d_desc) (void)hipFree(plan->d_desc);
+    if (plan->d_work) (void)hipFree(plan->d_work);
+    delete plan;
+}
+
+extern "C" int32_t ldpc_mi355_sync(void) {
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    HIP_OK(hipStreamSynchronize(r.stream));
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_soft_slot_elems(void) { return kMaxSoft; }
+
+extern "C" const char *ldpc_mi355_version(void) { return "nrldpc-mi355 0.1 (gfx950)"; }

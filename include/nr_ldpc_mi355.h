@@ -1,0 +1,196 @@
+/*
+ * nr_ldpc_mi355.h — C ABI of libnrldpc_mi355.so, the MI355X (gfx950) 5G NR
+ * LDPC decode path.
+ *
+ * Part 1 is a drop-in for the reference library's ctypes surface
+ * (hahaliu2001/deeplearning_5g_ldpc_decoder, c_lib/nr_ldpc.h): the same
+ * struct layouts, symbol names, argument meanings and return codes, so
+ * scripts/ldpc_c_sim_common.py:load_c_library and
+ * deep_learning/ldpc_table_config.py bind it unchanged.  Decoding runs on the
+ * GPU; parameter derivation, CRC, encoding, rate matching and the test
+ * channel are host code (they generate inputs; they are not the hot path).
+ *
+ * Part 2 is new: a batched, allocation-free-per-call decode over device
+ * buffers (plans), device-resident HARQ soft buffers, and queries.
+ *
+ * Plain C types only; the library owns its HIP stream and device tables.
+ */
+#ifndef NR_LDPC_MI355_H
+#define NR_LDPC_MI355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Part 1: reference-compatible surface                                */
+/* ------------------------------------------------------------------ */
+
+/* c_lib/nr_ldpc.h:8-21 */
+#define BG1_NUM_ROWS 46
+#define BG1_NUM_COLS 68
+#define BG1_MAX_NON_NULL_ELEMENTS 316
+#define BG2_NUM_ROWS 42
+#define BG2_NUM_COLS 52
+#define BG2_MAX_NON_NULL_ELEMENTS 197
+#define MAX_DEGREE 19
+#define MAX_ZC 384
+
+/* c_lib/nr_ldpc.h:42-50 — 7 x uint16, 14 bytes, no padding. */
+typedef struct {
+    uint16_t bgn;     /* base graph 1 or 2 */
+    uint16_t A;       /* payload bits before CRC (multiple of 8) */
+    uint16_t crctype; /* 0 CRC16, 1 CRC24A, 2 CRC24B, 3 CRC24C */
+    uint16_t E;       /* rate-matched bits */
+    uint16_t rv;      /* redundancy version 0..3 */
+    uint16_t Qm;      /* 1, 2, 4, 6, 8 */
+    uint16_t HARQ_on; /* 1 = combine into the soft buffer, 0 = fresh */
+} ldpc_in_config_t;
+
+/* c_lib/nr_ldpc.h:54-79 — derived parameters plus host table pointers. */
+typedef struct {
+    uint16_t bgn, A, crctype, E, rv, Qm, HARQ_on;
+    uint16_t B, Zc, iLS, K, F, N, k0;
+    uint16_t *Vij_array;
+    uint16_t *HBG_array;
+    uint16_t *HBG_offset_array;
+} ldpc_config_t;
+
+/* c_lib/nr_ldpc.h:82-90 — the 7-field decoder row
+ * [algo, max_iterations, alpha, beta, filling_bit_value, max_llr, min_llr].
+ * algo: 0 FP32 layered BP, 1 int16 min-sum, 4 FP32 min-sum (2 and 3 are
+ * not provided by this library and return -3). */
+typedef struct {
+    uint16_t algo;
+    uint16_t max_iterations;
+    uint16_t alpha;   /* Q15 scale (algo 1 reads it as int16, algo 4 /32768) */
+    uint16_t beta;    /* offset in int16 LLR units (algo 4: /128) */
+    int16_t filling_bit_value;
+    int16_t max_llr;
+    int16_t min_llr;
+} ldpc_decoder_cfg_t;
+
+/* Compressed base graphs, reference layout (c_lib/nr_ldpc.h:93-99); the
+ * pointers gen_ldpc_parameters returns point into these host arrays. */
+extern uint16_t BG1_HBG_offset_array[BG1_NUM_ROWS + 1];
+extern uint16_t BG1_HBG_array[BG1_MAX_NON_NULL_ELEMENTS];
+extern uint16_t BG1_HBG_Vij_array[8][BG1_MAX_NON_NULL_ELEMENTS];
+extern uint16_t BG2_HBG_offset_array[BG2_NUM_ROWS + 1];
+extern uint16_t BG2_HBG_array[BG2_MAX_NON_NULL_ELEMENTS];
+extern uint16_t BG2_HBG_Vij_array[8][BG2_MAX_NON_NULL_ELEMENTS];
+
+/* Host mirror of the process soft buffer (c_lib/nr_ldpc.h:105,
+ * c_lib/ldpc_internal.c:11).  After each c_ldpc_5g_decode it holds the
+ * rate-recovered buffer of that call; a call with HARQ_on = 1 combines into
+ * whatever it holds (the caller may edit it between calls). */
+extern int16_t HARQ_memory[BG1_NUM_COLS * MAX_ZC];
+
+/* c_lib/nr_ldpc.h:102, c_lib/nr_crc.c:170 — CRC remainder over len bytes;
+ * 0 for an unknown crctype. */
+uint32_t calculate_crc(const uint8_t *data, int len, int crctype);
+
+/* c_lib/nr_ldpc.h:108, c_lib/ldpc_internal.c:66 — 0 or -1 (message on
+ * stderr). */
+int gen_ldpc_parameters(ldpc_config_t *params, ldpc_in_config_t *inconfig);
+
+/* c_lib/nr_ldpc.h:124, c_lib/tx_rx_process.c:42 — x / 128. */
+float dequantize_int16_to_llr(int16_t quantized_llr);
+
+/* c_lib/nr_ldpc.h:127, c_lib/c_ldpc_5g_decoder.c:34 — rate recovery + HARQ
+ * combine + decode of one codeblock on the GPU.  LLr: E int16 (x128);
+ * hardbits: B/8 bytes MSB-first (untouched when max_iterations = 0).
+ * Returns 0 CRC pass, 1 not converged, -2 invalid config, -3 invalid algo
+ * (rate recovery still updates HARQ_memory, as in the reference), -1 GPU or
+ * runtime error.  Serialized internally (safe from several threads). */
+int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                         ldpc_decoder_cfg_t *decoder_config);
+
+/* c_lib/nr_ldpc.h:129, c_lib/c_ldpc_5g_encoder.c:66 — CRC attach, encode,
+ * rate match; rm_bytes receives ceil(E/8) bytes MSB-first.  0 or -2.
+ * (Host code; the encoder buffer is sized 68*Zc, so Zc >= 373 is safe.) */
+int32_t c_ldpc_5g_encode(uint8_t *infobytes, uint8_t *rm_bytes, ldpc_in_config_t *inconfig);
+
+/* c_lib/nr_ldpc.h:123, c_lib/tx_rx_process.c:286 — test channel: Gray QAM,
+ * complex AWGN of power 10^(noise_power_db/10), max-log LLRs x128 rounded
+ * and clipped to +-10000.  The noise comes from this library's own
+ * generator (seed with ldpc_mi355_channel_seed), not glibc rand().
+ * 0 or -1 for invalid arguments. */
+int tx_rx_process(const uint8_t *inbytes, size_t num_bits, uint16_t Qm, float noise_power_db,
+                  int16_t *llr_out);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: MI355X extensions (no counterpart in the reference)         */
+/* ------------------------------------------------------------------ */
+
+/* c_ldpc_5g_decode plus what the reference keeps internal: the number of
+ * iterations executed and the N+2Zc posteriors at exit (int16 for algo 1,
+ * float for algos 0/4); either output may be NULL. */
+int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                             ldpc_decoder_cfg_t *decoder_config, int32_t *iterations,
+                             void *posteriors);
+
+/* Seeds the tx_rx_process noise generator (default seed 0). */
+void ldpc_mi355_channel_seed(uint64_t seed);
+
+/* Input synthesis for simulations and benchmarks (host, worker threads):
+ * n codeblocks of one config with random payloads, CRC, encoding, rate
+ * matching and the test channel at noise_power_db.  Each codeblock has its
+ * own noise stream derived from (seed, index), so results do not depend on
+ * the thread count.  llr_out: n*E int16; payload_out (may be NULL): n*A
+ * payload bits, one per byte.  0, or -2 for an invalid config. */
+int32_t ldpc_mi355_synth_batch(const ldpc_in_config_t *cfg, int32_t n, float noise_power_db,
+                               uint64_t seed, int32_t threads, int16_t *llr_out,
+                               uint8_t *payload_out);
+
+/* One codeblock of a batch. */
+typedef struct {
+    ldpc_in_config_t cfg;  /* HARQ_on: combine into the soft buffer slot */
+    int16_t reserved;
+    uint32_t llr_offset;   /* element offset of its E LLRs in the LLR buffer */
+    int32_t soft_slot;     /* soft-buffer slot index, or -1 for none */
+} ldpc_mi355_cb_t;
+
+typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
+
+/* Validates n codeblock descriptors against one decoder row and uploads
+ * them to the device.  flags: bit 0 = produce posteriors.  Returns 0, -2 if
+ * any codeblock config is invalid (index in *bad_index when non-NULL), -3
+ * for an unsupported algo, -1 on a GPU error. */
+int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs, int32_t n,
+                               const ldpc_decoder_cfg_t *decoder_config, uint32_t flags,
+                               int32_t *bad_index);
+
+/* Output layout of a plan: total hard bytes, total posterior elements,
+ * element size of a posterior (2 or 4), and per-codeblock offsets (arrays
+ * of n, may be NULL). */
+int32_t ldpc_mi355_plan_layout(const ldpc_mi355_plan_t *plan, uint64_t *hard_bytes,
+                               uint64_t *lq_elems, int32_t *lq_elem_size,
+                               uint32_t *hard_offsets, uint32_t *lq_offsets);
+
+/* Launches the batch on `stream` (a hipStream_t, or NULL for the library's
+ * stream).  All pointers are device pointers: llr (int16), soft_pool (int16,
+ * slots of 68*384 elements; may be NULL if no codeblock uses a slot),
+ * hard (hard_bytes), status and iterations (n int32 each), lq (optional).
+ * Asynchronous and allocation-free (capturable in a hipGraph). */
+int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const int16_t *llr,
+                               int16_t *soft_pool, uint8_t *hard, int32_t *status,
+                               int32_t *iterations, void *lq, void *stream);
+
+void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan);
+
+/* Blocks until the library stream is idle.  0 or -1. */
+int32_t ldpc_mi355_sync(void);
+
+/* Elements per soft-buffer slot (68*384). */
+int32_t ldpc_mi355_soft_slot_elems(void);
+
+/* Version string of this library. */
+const char *ldpc_mi355_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NR_LDPC_MI355_H */
