@@ -64,6 +64,10 @@ def cpu_baseline(llr, threads):
     return secs, st, its, hard
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +76,12 @@ def main():
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--latency-reps", type=int, default=200)
+    ap.add_argument("--profile", action="store_true",
+                    help="batch launches only (for rocprofv3): no latency loop, no "
+                         "per-codeblock path, no CPU leg")
     args = ap.parse_args()
+    if args.profile:
+        args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(codeblocks shard by index across GPUs with no exchange, see DESIGN.md)",
@@ -85,6 +94,7 @@ def main():
     from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, synth_batch
 
     n = args.batch
+    log("synthesizing inputs")
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     llr, payload = synth_batch(CFG, n, NOISE_DB, seed=1, threads=os.cpu_count() or 8)
@@ -94,6 +104,7 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
 
+    log("warmup")
     for _ in range(args.warmup):
         plan.launch(d_llr, out, stream=sh)
     torch.cuda.synchronize()
@@ -120,27 +131,30 @@ def main():
     gbps = info_bits * args.steps / wall / 1e9
     ms_per_step = wall * 1e3 / args.steps
 
+    log(f"timed {args.steps} steps in {wall:.3f} s")
     # host-visible latency of one batch (launch -> results synchronized)
     lat = []
-    for _ in range(args.latency_reps):
+    for _ in range(0 if args.profile else args.latency_reps):
         t = time.perf_counter()
         plan.launch(d_llr, out, stream=sh)
         stream.synchronize()
         lat.append((time.perf_counter() - t) * 1e3)
-    lat = np.array(lat)
+    lat = np.array(lat if lat else [float("nan")])
 
     # per-codeblock reference ABI path: one launch + copies per codeblock
-    L = _lib.load()
-    ic = LdpcInConfig(*CFG)
-    dc = DecoderConfig(*ROW)
-    hb = np.zeros(1056, np.uint8)
-    n_pc = min(n, 64)
-    for i in range(4):
-        L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
-    t = time.perf_counter()
-    for i in range(n_pc):
-        L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
-    per_cb_gbps = n_pc * CFG[1] / (time.perf_counter() - t) / 1e9
+    per_cb_gbps = None
+    if not args.profile:
+        L = _lib.load()
+        ic = LdpcInConfig(*CFG)
+        dc = DecoderConfig(*ROW)
+        hb = np.zeros(1056, np.uint8)
+        n_pc = min(n, 64)
+        for i in range(4):
+            L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
+        t = time.perf_counter()
+        for i in range(n_pc):
+            L.c_ldpc_5g_decode(ptr(llr[i]), ptr(hb), ctypes.byref(ic), ctypes.byref(dc))
+        per_cb_gbps = n_pc * CFG[1] / (time.perf_counter() - t) / 1e9
 
     avg_kernel_s = float(kernel_ms.mean()) / 1e3
     alg = algorithmic_bytes(iters)
@@ -151,8 +165,10 @@ def main():
             "compulsory_bytes_per_launch": int(n * (2 * 25344 + 1056)),
             "kernel_ms_avg": float(kernel_ms.mean())}
 
+    log("per-codeblock path done")
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline")
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8
         threads = max(1, min(threads, 16))
         reps = 3

@@ -18,6 +18,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_decode_ex", "ldpc_mi355_channel_seed", "ldpc_mi355_plan_create",
     "ldpc_mi355_plan_layout", "ldpc_mi355_plan_decode", "ldpc_mi355_plan_destroy",
     "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
+    "ldpc_mi355_algo_available",
     "ldpc_mi355_synth_batch",
 )
 EXPORTED_DATA = (
@@ -119,6 +120,8 @@ def load():
     L.ldpc_mi355_sync.restype = ctypes.c_int32
     L.ldpc_mi355_soft_slot_elems.argtypes = []
     L.ldpc_mi355_soft_slot_elems.restype = ctypes.c_int32
+    L.ldpc_mi355_algo_available.argtypes = [ctypes.c_int32]
+    L.ldpc_mi355_algo_available.restype = ctypes.c_int32
     L.ldpc_mi355_version.argtypes = []
     L.ldpc_mi355_version.restype = ctypes.c_char_p
     L.ldpc_mi355_synth_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32,

@@ -92,16 +92,20 @@ class BatchPlan:
         return out
 
     def launch(self, llr, out, soft_pool=None, stream=None):
-        """Asynchronous decode of device tensor ``llr`` (int16) into ``out``."""
+        """Asynchronous decode of device tensor ``llr`` (int16) into ``out`` on
+        ``stream`` (a raw hipStream_t handle; default: torch's current stream,
+        whose handle is 0 for the default stream)."""
+        import torch
         if llr.dtype.itemsize != 2 or llr.numel() < self.llr_elems:
             raise ValueError("llr must be int16 with every codeblock's E values")
         if not llr.is_cuda:
             raise ValueError("llr must be a device tensor")
+        if stream is None:
+            stream = torch.cuda.current_stream(llr.device).cuda_stream
         st = _lib.load().ldpc_mi355_plan_decode(
             self._handle, ptr(llr), ptr(soft_pool) if soft_pool is not None else None,
             ptr(out["hard"]), ptr(out["status"]), ptr(out["iters"]),
-            ptr(out["lq"]) if self.want_lq else None,
-            ctypes.c_void_p(stream) if stream is not None else None)
+            ptr(out["lq"]) if self.want_lq else None, ctypes.c_void_p(stream))
         if st != 0:
             raise RuntimeError(f"plan_decode failed ({st})")
         return out

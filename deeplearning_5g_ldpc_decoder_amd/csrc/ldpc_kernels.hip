@@ -29,6 +29,13 @@
 #include "ldpc_device.h"
 #include "nr_bg_tables.h"
 
+// The FP32 BP kernel (algo 0) did not return on its first MI355X run and no
+// defect has been found in its source, so it is compiled out until one is:
+// algo 0 then fails with an explicit error instead of running.
+#ifndef NRLDPC_BP_KERNEL
+#define NRLDPC_BP_KERNEL 0
+#endif
+
 namespace nrldpc {
 
 template <int BG> struct Graph;
@@ -392,6 +399,7 @@ __device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const
     return it;
 }
 
+#if NRLDPC_BP_KERNEL
 // ---------------------------------------------------------------------------
 // Algo 0: FP32 layered BP (c_lib/floatpoint_ldpc_decoder.c:53-151).  Exact
 // zeros are special-cased like the reference; messages saturate at
@@ -481,6 +489,7 @@ __device__ int run_bp(float *lq, float *lr_mem, const uint32_t *etab, const CbDe
     }
     return it;
 }
+#endif  // NRLDPC_BP_KERNEL
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -524,9 +533,11 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
             it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
         } else {
+#if NRLDPC_BP_KERNEL
             float *lr_mem = o.work + d.work_off;
             it = (d.bgn == 1) ? run_bp<1>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status)
                               : run_bp<2>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status);
+#endif
         }
         // hard decisions of the final iteration (left untouched when no
         // iteration ran, like the reference)
@@ -561,10 +572,12 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
         hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumF32, float>), grid, block, lds,
                            stream, *o, *p);
         break;
+#if NRLDPC_BP_KERNEL
     case 0:
         hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kBpF32, float>), grid, block, lds, stream,
                            *o, *p);
         break;
+#endif
     default:
         return hipErrorInvalidValue;
     }
@@ -582,8 +595,13 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+#if NRLDPC_BP_KERNEL
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(
+    e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+#endif
+    return e;
 }
+
+extern "C" int nrldpc_bp_kernel_enabled() { return NRLDPC_BP_KERNEL; }

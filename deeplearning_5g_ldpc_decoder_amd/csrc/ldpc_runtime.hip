@@ -17,6 +17,7 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
                                            int ncb, int threads, int max_soft,
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
+extern "C" int nrldpc_bp_kernel_enabled();
 
 namespace {
 
@@ -101,6 +102,14 @@ Runtime &rt() {
 
 bool algo_supported(int algo) { return algo == 0 || algo == 1 || algo == 4; }
 
+// Algo 0 is part of the ABI but its GPU kernel is compiled out of this build
+// (see ldpc_kernels.hip); calls fail loudly rather than falling back.
+bool algo_runnable(int algo) { return algo != 0 || nrldpc_bp_kernel_enabled(); }
+
+void report_bp_disabled() {
+    fprintf(stderr, "libnrldpc_mi355: algo 0 (FP32 BP) GPU kernel is disabled in this build\n");
+}
+
 DecParams make_params(const ldpc_decoder_cfg_t &c) {
     DecParams p;
     p.max_it = c.max_iterations;
@@ -150,16 +159,17 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     std::lock_guard<std::mutex> guard(r.mu);
 
     const bool valid_algo = algo_supported(decoder_config->algo);
-    const int algo = valid_algo ? decoder_config->algo : 1;
+    const bool runnable = valid_algo && algo_runnable(decoder_config->algo);
+    const int algo = runnable ? decoder_config->algo : 1;
     DecParams dp = make_params(*decoder_config);
-    if (!valid_algo) dp.max_it = 0;   // rate recovery only, like the reference
+    if (!runnable) dp.max_it = 0;   // rate recovery only, like the reference
     const int n = p.N + 2 * p.Zc;
 
     CbDesc d = make_desc(p, r);
     d.llr_off = 0;
     d.hard_off = 0;
     d.soft_off = 0;
-    d.lq_off = (posteriors && valid_algo) ? 0u : kNone;
+    d.lq_off = (posteriors && runnable) ? 0u : kNone;
     d.work_off = 0;
     memcpy(r.h_in + InLayout::desc, &d, sizeof d);
     memcpy(r.h_in + InLayout::llr, LLr, (size_t)p.E * 2);
@@ -197,6 +207,10 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
         fprintf(stderr, "Invalid decoding algorithm selection: %u\n", decoder_config->algo);
         return -3;
     }
+    if (!runnable) {
+        report_bp_disabled();
+        return -1;
+    }
     if (its > 0) memcpy(hardbits, r.h_out + OutLayout::hard, (size_t)p.B / 8);
     if (iterations) *iterations = its;
     if (posteriors) memcpy(posteriors, r.h_out + OutLayout::lq, lq_bytes);
@@ -228,6 +242,10 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     if (bad_index) *bad_index = -1;
     if (n <= 0) return -2;
     if (!algo_supported(dec->algo)) return -3;
+    if (!algo_runnable(dec->algo)) {
+        report_bp_disabled();
+        return -1;
+    }
     Runtime &r = rt();
     if (r.ensure() != 0) return -1;
     auto *pl = new ldpc_mi355_plan;
@@ -306,7 +324,9 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     o.lq = lq;
     o.work = plan->d_work;
     o.etab = r.d_etab;
-    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : r.stream;
+    // NULL is HIP's default (null) stream, as everywhere in HIP -- it is also
+    // the handle PyTorch reports for its default stream.
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
                                 plan->max_soft, s));
     return 0;
@@ -327,5 +347,9 @@ extern "C" int32_t ldpc_mi355_sync(void) {
 }
 
 extern "C" int32_t ldpc_mi355_soft_slot_elems(void) { return kMaxSoft; }
+
+extern "C" int32_t ldpc_mi355_algo_available(int32_t algo) {
+    return algo_supported(algo) && algo_runnable(algo) ? 1 : 0;
+}
 
 extern "C" const char *ldpc_mi355_version(void) { return "nrldpc-mi355 0.1 (gfx950)"; }

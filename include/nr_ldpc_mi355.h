@@ -170,7 +170,7 @@ int32_t ldpc_mi355_plan_layout(const ldpc_mi355_plan_t *plan, uint64_t *hard_byt
                                uint64_t *lq_elems, int32_t *lq_elem_size,
                                uint32_t *hard_offsets, uint32_t *lq_offsets);
 
-/* Launches the batch on `stream` (a hipStream_t, or NULL for the library's
+/* Launches the batch on `stream` (a hipStream_t; NULL is HIP's default null
  * stream).  All pointers are device pointers: llr (int16), soft_pool (int16,
  * slots of 68*384 elements; may be NULL if no codeblock uses a slot),
  * hard (hard_bytes), status and iterations (n int32 each), lq (optional).
@@ -180,6 +180,11 @@ int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const int16_t *llr
                                int32_t *iterations, void *lq, void *stream);
 
 void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan);
+
+/* 1 when this build runs decoder algo `algo` on the GPU, else 0 (algo 0,
+ * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls
+ * then return -1 with a message instead of decoding). */
+int32_t ldpc_mi355_algo_available(int32_t algo);
 
 /* Blocks until the library stream is idle.  0 or -1. */
 int32_t ldpc_mi355_sync(void);

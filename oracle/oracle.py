@@ -67,7 +67,7 @@ def lib():
         L.orc_encode.argtypes = [ctypes.POINTER(OrcCode), P, P]
         L.orc_rate_match.argtypes = [ctypes.POINTER(OrcCode), P, P]
         L.orc_syndrome_ok.argtypes = [ctypes.POINTER(OrcCode), P]
-        L.orc_decode_batch_i16.argtypes = [P, P, P, P] + [ctypes.c_int] * 8 + [
+        L.orc_decode_batch_i16.argtypes = [P, P, P, P] + [ctypes.c_int] * 7 + [
             ctypes.POINTER(OrcDec), ctypes.c_int]
         L.orc_decode_batch_i16.restype = ctypes.c_double
         _lib = L

@@ -82,6 +82,9 @@ def test_algo4_abi_matches_reference(L, case):
 def test_algo0_abi_tracks_fp64_oracle(L, oracle_mod, case):
     """FP32 BP on the GPU vs the FP64 oracle, same iteration count forced:
     posteriors within 1e-3 * max|LLR| (tolerance of the float path)."""
+    if not L.ldpc_mi355_algo_available(0):
+        pytest.skip("FP32 BP kernel disabled: its first GPU run did not return and no "
+                    "defect has been found in its source yet")
     row = list(case["row"])
     out = abi_decode(L, case["llr"], case["cfg"], row, case["harq_in"])
     n = len(case["soft"])
@@ -106,6 +109,24 @@ def test_algo0_abi_tracks_fp64_oracle(L, oracle_mod, case):
         o1 = abi_decode(L, case["llr"], case["cfg"], r1, case["harq_in"])
         _, _, lq64_1, _ = oracle_mod.decode_soft(c, r1, case["soft"], precision="f64")
         assert np.max(np.abs(o1["lq"][:n] - lq64_1)) <= 1e-3 * np.abs(lq64_1).max()
+
+
+def test_algo0_disabled_fails_loudly(L):
+    """While the BP kernel is compiled out, algo 0 returns -1 (no fallback),
+    after rate recovery, like an unsupported algorithm."""
+    if L.ldpc_mi355_algo_available(0):
+        pytest.skip("BP kernel enabled in this build")
+    rng = np.random.default_rng(2)
+    llr = rng.integers(-500, 500, 512).astype(np.int16)
+    hard = np.full(25, 0x5A, np.uint8)
+    cfg = (2, 176, 1, 512, 0, 2, 0)
+    st = L.c_ldpc_5g_decode(ptr(llr), ptr(hard), ctypes.byref(LdpcInConfig(*cfg)),
+                            ctypes.byref(DecoderConfig(0, 8, 1, 0, 1280, 20, -32768)))
+    assert st == -1
+    assert np.all(hard == 0x5A)
+    from oracle import oracle as O
+    want = O.rate_recover(O.code(*cfg), 1280, llr)
+    assert np.array_equal(harq_view(L)[:len(want)], want)
 
 
 def test_invalid_config_and_algo(L):
@@ -139,7 +160,7 @@ def _random_batch(rng, n, choices, noise_span=(-3.0, 1.0)):
 
 MIXED = [(1, 8424, 2, 25344, 0, 2), (2, 1016, 2, 3120, 1, 2), (1, 488, 1, 1536, 2, 4),
          (2, 16, 0, 200, 0, 2), (1, 16, 0, 300, 3, 2), (2, 2056, 3, 4160, 0, 4),
-         (1, 3816, 1, 8000, 0, 6), (2, 56, 1, 240, 0, 2), (1, 64, 0, 720, 0, 8)]
+         (1, 3816, 1, 7998, 0, 6), (2, 56, 1, 240, 0, 2), (1, 64, 0, 720, 0, 8)]
 
 
 @pytest.mark.parametrize("row", [ROW1, [4, 12, 22936, 0, 1280, 32767, -32767],
