@@ -142,6 +142,34 @@ void print_invalid(const ldpc_in_config_t &in, const char *why) {
             in.crctype, in.E, in.rv, in.Qm, in.HARQ_on);
 }
 
+DecParams decoder_params(const ldpc_decoder_cfg_t &c) {
+    DecParams p;
+    p.max_it = c.max_iterations;
+    p.alpha = (int16_t)c.alpha;   // the int16 view the reference's helpers take
+    p.beta = (int16_t)c.beta;
+    p.max_llr = c.max_llr;
+    p.min_llr = c.min_llr;
+    p.filler = c.filling_bit_value;
+    p.falpha = (float)c.alpha / 32768.0f;
+    p.fbeta = (float)c.beta / 128.0f;
+    p.fmax = (float)c.max_llr / 128.0f;
+    p.fmin = (float)c.min_llr / 128.0f;
+    return p;
+}
+
+CbDesc codeblock_desc(const Params &p, uint32_t etab_off) {
+    CbDesc d;
+    memset(&d, 0, sizeof d);
+    d.bgn = (uint16_t)p.bgn; d.Zc = (uint16_t)p.Zc; d.B = (uint16_t)p.B;
+    d.K = (uint16_t)p.K; d.N = (uint16_t)p.N; d.E = (uint16_t)p.E;
+    d.k0 = (uint16_t)p.k0; d.Qm = (uint16_t)p.Qm;
+    d.crctype = (uint8_t)p.crctype; d.harq_on = (uint8_t)p.harq_on;
+    d.etab_off = etab_off;
+    d.soft_off = kNone;
+    d.lq_off = kNone;
+    return d;
+}
+
 void edge_table(int bgn, int zc, int ils, uint32_t *out) {
     const int ne = bgn == 1 ? NR_BG1_EDGES : NR_BG2_EDGES;
     const uint8_t *col = bgn == 1 ? kBG1EdgeCol : kBG2EdgeCol;

@@ -3,6 +3,7 @@
 #include <stdint.h>
 
 #include "../../include/nr_ldpc_mi355.h"
+#include "ldpc_device.h"
 
 namespace nrldpc {
 
@@ -26,5 +27,12 @@ void edge_table(int bgn, int zc, int ils, uint32_t *out);
 
 void encode_codeword(const Params &p, const uint8_t *bits_b, uint8_t *cw);
 void rate_match_bits(const Params &p, const uint8_t *cw, uint8_t *out_bits);
+
+// Kernel-side views of a decoder row and of one codeblock (ldpc_device.h).
+// The descriptor's buffer offsets are set to "none"; callers fill them.
+DecParams decoder_params(const ldpc_decoder_cfg_t &c);
+CbDesc codeblock_desc(const Params &p, uint32_t etab_off);
+// Workgroup size for lifting size zc: one lane per lifted check, whole waves.
+inline int workgroup_threads(int zc) { return ((zc + 63) / 64) * 64; }
 
 }  // namespace nrldpc

@@ -24,7 +24,14 @@
 // shifts its remainder by x^(8*bytes_after) mod g (GF(2) linearity), and the
 // remainders are XOR-reduced across the workgroup.  A zero remainder stops the
 // codeblock (c_lib/fixedpoint_ldpc_decoder.c:214-219).
+#ifdef NRLDPC_CPU_EMU
+// Host build of the kernel source for CPU tests (tests/kernel_emu/).
+#include "hip_cpu_emu.h"
+#define NRLDPC_DYNAMIC_LDS(name) unsigned char *name = emu_dynamic_lds()
+#else
 #include <hip/hip_runtime.h>
+#define NRLDPC_DYNAMIC_LDS(name) extern __shared__ __align__(16) unsigned char name[]
+#endif
 
 #include "ldpc_device.h"
 #include "nr_bg_tables.h"
@@ -503,7 +510,7 @@ template <> __device__ inline float from_soft<float>(int16_t v) { return (float)
 template <Algo A, typename T>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_decode_kernel(DecOut o, DecParams p) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    NRLDPC_DYNAMIC_LDS(smem);
     __shared__ uint32_t crc_tab[256];
     __shared__ uint32_t slots[kMaxThreads / 64];
     T *lq = reinterpret_cast<T *>(smem);
@@ -555,6 +562,7 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
 
 }  // namespace nrldpc
 
+#ifndef NRLDPC_CPU_EMU
 // Host-callable launchers (C linkage inside the library only).
 extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
                                            const nrldpc::DecParams *p, int ncb, int threads,
@@ -605,3 +613,4 @@ extern "C" hipError_t nrldpc_configure_kernels() {
 }
 
 extern "C" int nrldpc_bp_kernel_enabled() { return NRLDPC_BP_KERNEL; }
+#endif  // NRLDPC_CPU_EMU

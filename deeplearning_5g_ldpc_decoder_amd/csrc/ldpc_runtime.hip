@@ -110,34 +110,13 @@ void report_bp_disabled() {
     fprintf(stderr, "libnrldpc_mi355: algo 0 (FP32 BP) GPU kernel is disabled in this build\n");
 }
 
-DecParams make_params(const ldpc_decoder_cfg_t &c) {
-    DecParams p;
-    p.max_it = c.max_iterations;
-    p.alpha = (int16_t)c.alpha;        // int16 view, as the reference's helpers
-    p.beta = (int16_t)c.beta;
-    p.max_llr = c.max_llr;
-    p.min_llr = c.min_llr;
-    p.filler = c.filling_bit_value;
-    p.falpha = (float)c.alpha / 32768.0f;
-    p.fbeta = (float)c.beta / 128.0f;
-    p.fmax = (float)c.max_llr / 128.0f;
-    p.fmin = (float)c.min_llr / 128.0f;
-    return p;
-}
+DecParams make_params(const ldpc_decoder_cfg_t &c) { return decoder_params(c); }
 
 CbDesc make_desc(const Params &p, const Runtime &r) {
-    CbDesc d;
-    memset(&d, 0, sizeof d);
-    d.bgn = (uint16_t)p.bgn; d.Zc = (uint16_t)p.Zc; d.B = (uint16_t)p.B;
-    d.K = (uint16_t)p.K; d.N = (uint16_t)p.N; d.E = (uint16_t)p.E;
-    d.k0 = (uint16_t)p.k0; d.Qm = (uint16_t)p.Qm;
-    d.crctype = (uint8_t)p.crctype; d.harq_on = (uint8_t)p.harq_on;
-    d.etab_off = r.etab_off[p.bgn - 1][p.Zc];
-    d.soft_off = kNone; d.lq_off = kNone;
-    return d;
+    return codeblock_desc(p, r.etab_off[p.bgn - 1][p.Zc]);
 }
 
-int threads_for(int zc) { return ((zc + 63) / 64) * 64; }
+int threads_for(int zc) { return workgroup_threads(zc); }
 
 }  // namespace
 

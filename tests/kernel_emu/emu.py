@@ -1,0 +1,62 @@
+"""ctypes front end of the CPU build of the decode kernels (test-only)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libldpc_emu.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        L.emu_decode_batch.argtypes = [ctypes.c_int] + [P] * 13 + [ctypes.c_uint64]
+        L.emu_decode_batch.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def dims(cfg):
+    from oracle import oracle as O
+    st, p = O.params(*cfg)
+    return p
+
+
+def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True):
+    """Run the kernel source on the CPU.  Returns per-codeblock lists."""
+    n = len(cfgs)
+    cf = np.array(cfgs, np.int32).reshape(n, 7)
+    ps = [dims(tuple(c)) for c in cfgs]
+    off = np.cumsum([0] + [len(x) for x in llrs[:-1]]).astype(np.uint32)
+    llr = np.ascontiguousarray(np.concatenate(llrs), np.int16)
+    hard = np.zeros(sum(p["B"] // 8 for p in ps), np.uint8)
+    st = np.zeros(n, np.int32)
+    it = np.zeros(n, np.int32)
+    nlq = sum(p["N"] + 2 * p["Zc"] for p in ps)
+    lq = np.zeros(nlq, np.int16 if row[0] == 1 else np.float32) if want_lq else None
+    hoff = np.zeros(n, np.uint32)
+    loff = np.zeros(n, np.uint32)
+    nwork = sum((316 if c[0] == 1 else 197) * p["Zc"] for c, p in zip(cfgs, ps))
+    work = np.zeros(max(nwork, 1), np.float32)
+    slots = None if soft_slots is None else np.ascontiguousarray(soft_slots, np.int32)
+    rc = lib().emu_decode_batch(n, _p(cf), _p(off), _p(np.array(row, np.int32)), _p(llr),
+                                _p(soft_pool), _p(slots), _p(hard), _p(st), _p(it), _p(lq),
+                                _p(hoff), _p(loff), _p(work), ctypes.c_uint64(len(work)))
+    if rc != 0:
+        raise RuntimeError(f"emu_decode_batch returned {rc}")
+    out = []
+    for i, p in enumerate(ps):
+        h = hard[hoff[i]:hoff[i] + p["B"] // 8]
+        q = lq[loff[i]:loff[i] + p["N"] + 2 * p["Zc"]] if want_lq else None
+        out.append(dict(status=int(st[i]), iters=int(it[i]), hard=h, lq=q))
+    return out
