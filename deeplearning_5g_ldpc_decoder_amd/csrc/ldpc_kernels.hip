@@ -210,76 +210,151 @@ struct StateI16 {
     uint32_t sgn[Graph<BG>::kRows];   // sign bit per edge | argmin << 24
 };
 
-__device__ inline int16_t scale_i16(int m, const DecParams &p) {
-    int off = m - p.beta;
-    if (off < 0) off = 0;
-    const int scaled = p.alpha * off;
-    return (int16_t)((scaled + (1 << 14)) >> 15);
+// Bit j of v as 0 / -1.
+__device__ __forceinline__ int sext_bit(uint32_t v, int j) {
+#ifdef NRLDPC_CPU_EMU
+    return -(int)((v >> j) & 1u);
+#else
+    return __builtin_amdgcn_sbfe((int)v, (unsigned)j, 1u);
+#endif
 }
 
-template <int BG, int R>
+__device__ __forceinline__ int wrap16(int v) { return (int)(int16_t)v; }
+
+// int16 wrap of a - b, sign-extended: one SDWA subtract on gfx950.
+__device__ __forceinline__ int sub_wrap16(int a, int b) {
+#ifdef NRLDPC_CPU_EMU
+    return wrap16(a - b);
+#else
+    int r;
+    asm("v_sub_u32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_SEXT "
+        "src0_sel:DWORD src1_sel:DWORD"
+        : "=v"(r) : "v"(a), "v"(b));
+    return r;
+#endif
+}
+
+// Median of three (a clamp of b into [a, c] when a <= c).
+__device__ __forceinline__ int med3_i32(int a, int b, int c) {
+#ifdef NRLDPC_CPU_EMU
+    return max(min(a, b), min(max(a, b), c));
+#else
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#endif
+}
+
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+#ifdef NRLDPC_CPU_EMU
+    return max(min(a, b), min(max(a, b), c));
+#else
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#endif
+}
+
+// (acc << 1) | sign bit of v, in one funnel shift.
+__device__ __forceinline__ uint32_t push_sign(uint32_t acc, int v) {
+#ifdef NRLDPC_CPU_EMU
+    return (acc << 1) | ((uint32_t)v >> 31);
+#else
+    return __builtin_amdgcn_alignbit(acc, (uint32_t)v, 31u);
+#endif
+}
+
+// (alpha * max(m - beta, 0) + 2^14) >> 15 as int16 (c_lib/fixedpoint_ldpc_decoder.c:38-46).
+// alpha is in [-32768, 32767] and the offset in [0, 65535], so a 24-bit
+// multiply is exact and the product fits in 32 bits.
+__device__ inline int scale_i16(int m, const DecParams &p) {
+    const int off = max(m - p.beta, 0);
+#ifdef NRLDPC_CPU_EMU
+    const int scaled = p.alpha * off;
+#else
+    const int scaled = __mul24(p.alpha, off);
+#endif
+    return wrap16((scaled + (1 << 14)) >> 15);
+}
+
+// Larger than every two-minimum key (magnitudes reach 32768, see below).
+constexpr uint32_t kKeyMax = (32768u << 5) | 31u;
+
+// One base-graph row, one lifted check per lane (c_lib/fixedpoint_ldpc_decoder.c:103-144).
+//
+// Magnitudes and edge indices travel together as key = |rel| << 5 | j, so the
+// two-minimum search is a min and a med3 per edge; unique keys make the
+// first-occurrence tie-break of the reference's strict '<' come out the same.
+// |rel| is taken without the reference's 32767 cap for rel = -32768: both
+// minima are clipped to max_llr <= 32767 before use, and whenever 32768 is
+// one of the two smallest magnitudes both clipped minima are equal, so the
+// messages are the same; likewise when every magnitude is 32767 (reference
+// argmin -1).  Sign bits are kept in reverse edge order (edge j at bit
+// deg-1-j).  kGeneral = false is the fast path for decoder rows with
+// alpha >= 0 (as int16) and min_llr <= max_llr: new messages then never reach
+// -32768 and the LQ clip is a plain clamp; true is exact for every row
+// (message wrapped to int16 before the add, clip as the reference orders its
+// two tests).
+template <int BG, int R, bool kGeneral>
 __device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &st,
                                         const uint32_t *__restrict__ etab, int zx,
                                         int Zc, const DecParams &p) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     const uint32_t mag = st.mag[R], sb = st.sgn[R];
-    const int16_t o1 = (int16_t)(mag & 0xFFFFu), o2 = (int16_t)(mag >> 16);
+    const int o1 = wrap16((int)mag), o2 = (int)mag >> 16;
     const int oidx = (int)(sb >> 24);
-    int addr[deg];
-    int16_t rel[deg];
+    int addr[deg], rel[deg];
+    uint32_t k1 = kKeyMax, k2 = kKeyMax, negs = 0;
 #pragma unroll
     for (int j = 0; j < deg; ++j) {
         const uint32_t t = etab[e0 + j];
-        int c = zx + (int)(t >> 16);
-        c = (c >= Zc) ? c - Zc : c;
-        addr[j] = (int)(t & 0xFFFFu) + c;
-        const int16_t ov = (j == oidx) ? o2 : o1;
-        const int16_t lr = ((sb >> j) & 1u) ? (int16_t)(-(int)ov) : ov;
-        rel[j] = (int16_t)((int)lq[addr[j]] - (int)lr);
+        const uint32_t c = (uint32_t)zx + (t >> 16);
+        addr[j] = (int)(t & 0xFFFFu) + (int)min(c, c - (uint32_t)Zc);
+        // previous message of this edge (the wrap of its int16 form is
+        // absorbed by the wrap of rel)
+        const int ov = (j == oidx) ? o2 : o1;
+        const int s = sext_bit(sb, deg - 1 - j);
+        const int v = sub_wrap16((int)lq[addr[j]], (ov ^ s) - s);
+        rel[j] = v;
+        negs = push_sign(negs, v);
+        const uint32_t key = ((uint32_t)max(v, -v) << 5) | (uint32_t)j;
+        k2 = med3_u32(k1, key, k2);
+        k1 = min(k1, key);
     }
-    int m1 = 32767, m2 = 32767, idx = 255;
-    uint32_t negs = 0;
-#pragma unroll
-    for (int j = 0; j < deg; ++j) {
-        const int v = rel[j];
-        const bool neg = v < 0;
-        negs |= (uint32_t)neg << j;
-        const int a = neg ? (v == -32768 ? 32767 : -v) : v;
-        const bool lt = a < m1;
-        m2 = lt ? m1 : (a < m2 ? a : m2);
-        idx = lt ? j : idx;
-        m1 = lt ? a : m1;
-    }
-    if (m1 > p.max_llr) m1 = p.max_llr;
-    if (m2 > p.max_llr) m2 = p.max_llr;
-    const int16_t n1 = scale_i16(m1, p), n2 = scale_i16(m2, p);
-    const uint32_t mask = (deg >= 32) ? 0xFFFFFFFFu : ((1u << deg) - 1u);
+    const int idx = (int)(k1 & 31u);
+    const int m1 = min((int)(k1 >> 5), p.max_llr), m2 = min((int)(k2 >> 5), p.max_llr);
+    const int n1 = scale_i16(m1, p), n2 = scale_i16(m2, p);
+    constexpr uint32_t mask = (1u << deg) - 1u;
     const uint32_t bits = ((__popc(negs) & 1) ? ~negs : negs) & mask;
 #pragma unroll
     for (int j = 0; j < deg; ++j) {
-        const int16_t nv = (j == idx) ? n2 : n1;
-        const int16_t lr = ((bits >> j) & 1u) ? (int16_t)(-(int)nv) : nv;
-        int v = (int)rel[j] + (int)lr;
-        if (v > p.max_llr) v = p.max_llr;
-        else if (v < p.min_llr) v = p.min_llr;
+        const int nv = (j == idx) ? n2 : n1;
+        const int s = sext_bit(bits, deg - 1 - j);
+        int v;
+        if constexpr (kGeneral) {
+            v = rel[j] + sub_wrap16(nv ^ s, s);
+            v = v > p.max_llr ? p.max_llr : max(v, p.min_llr);
+        } else {
+            v = med3_i32(p.min_llr, rel[j] + ((nv ^ s) - s), p.max_llr);
+        }
         lq[addr[j]] = (int16_t)v;
     }
     st.mag[R] = (uint32_t)(uint16_t)n1 | ((uint32_t)(uint16_t)n2 << 16);
     st.sgn[R] = bits | ((uint32_t)idx << 24);
 }
 
-template <int BG, int R>
+template <int BG, int R, bool kGeneral>
 __device__ __forceinline__ void rows_i16(int16_t *lq, StateI16<BG> &st, const uint32_t *etab,
                                          int zx, int Zc, const DecParams &p) {
     if constexpr (R < Graph<BG>::kRows) {
-        if (zx < Zc) row_i16<BG, R>(lq, st, etab, zx, Zc, p);
+        if (zx < Zc) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p);
         __syncthreads();
-        rows_i16<BG, R + 1>(lq, st, etab, zx, Zc, p);
+        rows_i16<BG, R + 1, kGeneral>(lq, st, etab, zx, Zc, p);
     }
 }
 
-template <int BG>
+template <int BG, bool kGeneral>
 __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
                        const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
     StateI16<BG> st;
@@ -292,7 +367,7 @@ __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        rows_i16<BG, 0>(lq, st, etab, zx, d.Zc, p);
+        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, d.Zc, p);
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -507,7 +582,9 @@ template <typename T> __device__ inline T from_soft(int16_t v);
 template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
 template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
 
-template <Algo A, typename T>
+// kGeneral selects the exact-for-every-row int16 path (see row_i16); the
+// other algorithms ignore it.
+template <Algo A, typename T, bool kGeneral = false>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_decode_kernel(DecOut o, DecParams p) {
     NRLDPC_DYNAMIC_LDS(smem);
@@ -534,8 +611,9 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     int status = 1, it = 0;
     if (p.max_it > 0) {
         if constexpr (A == Algo::kMinSumI16) {
-            it = (d.bgn == 1) ? run_i16<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
-                              : run_i16<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
+            it = (d.bgn == 1)
+                     ? run_i16<1, kGeneral>(lq, etab, d, p, crc, crc_tab, slots, &status)
+                     : run_i16<2, kGeneral>(lq, etab, d, p, crc, crc_tab, slots, &status);
         } else if constexpr (A == Algo::kMinSumF32) {
             it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
@@ -573,8 +651,12 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
     const dim3 grid(ncb), block(threads);
     switch (algo) {
     case 1:
-        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t>), grid, block, lds,
-                           stream, *o, *p);
+        if (p->alpha >= 0 && p->min_llr <= p->max_llr)
+            hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>), grid, block,
+                               lds, stream, *o, *p);
+        else
+            hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>), grid, block,
+                               lds, stream, *o, *p);
         break;
     case 4:
         hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumF32, float>), grid, block, lds,
@@ -597,7 +679,11 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     const int max_lds = 160 * 1024 - 2048;
     hipError_t e;
     e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t>),
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute(

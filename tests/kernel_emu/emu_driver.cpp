@@ -78,8 +78,12 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     const unsigned threads = (unsigned)workgroup_threads(max_zc);
     switch (row[0]) {
     case 1:
-        emu_launch(n, threads, (size_t)max_soft * 2,
-                   [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t>(o, p); });
+        if (p.alpha >= 0 && p.min_llr <= p.max_llr)
+            emu_launch(n, threads, (size_t)max_soft * 2,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>(o, p); });
+        else
+            emu_launch(n, threads, (size_t)max_soft * 2,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>(o, p); });
         return 0;
     case 4:
         emu_launch(n, threads, (size_t)max_soft * 4,

@@ -43,6 +43,12 @@ struct EmuBlock {
 };
 inline EmuBlock g_emu_block;
 
+// HIP's device min/max overloads for the integer types the kernels use.
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
+inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
 inline unsigned char *emu_dynamic_lds() { return g_emu_block.lds.data(); }
 inline void __syncthreads() { g_emu_block.bar->arrive_and_wait(); }
 inline int __popc(uint32_t v) { return __builtin_popcount(v); }
