@@ -18,7 +18,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_decode_ex", "ldpc_mi355_channel_seed", "ldpc_mi355_plan_create",
     "ldpc_mi355_plan_layout", "ldpc_mi355_plan_decode", "ldpc_mi355_plan_destroy",
     "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
-    "ldpc_mi355_algo_available",
+    "ldpc_mi355_algo_available", "ldpc_mi355_plan_tiles", "ldpc_mi355_set_tile_codeblocks",
     "ldpc_mi355_synth_batch",
 )
 EXPORTED_DATA = (
@@ -120,6 +120,10 @@ def load():
     L.ldpc_mi355_sync.restype = ctypes.c_int32
     L.ldpc_mi355_soft_slot_elems.argtypes = []
     L.ldpc_mi355_soft_slot_elems.restype = ctypes.c_int32
+    L.ldpc_mi355_plan_tiles.argtypes = [P]
+    L.ldpc_mi355_plan_tiles.restype = ctypes.c_int32
+    L.ldpc_mi355_set_tile_codeblocks.argtypes = [ctypes.c_int32] * 3
+    L.ldpc_mi355_set_tile_codeblocks.restype = ctypes.c_int32
     L.ldpc_mi355_algo_available.argtypes = [ctypes.c_int32]
     L.ldpc_mi355_algo_available.restype = ctypes.c_int32
     L.ldpc_mi355_version.argtypes = []

@@ -31,7 +31,11 @@ class BatchPlan:
     codeblock or -1 (default: none).  want_lq: also return posteriors.
     """
 
-    def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False):
+    PACK = 2        # plan flag: pack codeblocks of one (BG, Zc) per workgroup
+    NO_PACK = 4     # plan flag: one codeblock per workgroup
+
+    def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False,
+                 flags=0):
         L = _lib.load()
         n = len(cfgs)
         if n == 0:
@@ -54,9 +58,10 @@ class BatchPlan:
         bad = ctypes.c_int32(-1)
         dec = _decoder(self.row)
         st = L.ldpc_mi355_plan_create(ctypes.byref(self._handle), arr, n, ctypes.byref(dec),
-                                      1 if want_lq else 0, ctypes.byref(bad))
+                                      (1 if want_lq else 0) | int(flags), ctypes.byref(bad))
         if st != 0:
             raise ValueError(f"plan_create failed ({st}), codeblock {bad.value}")
+        self.tiles = int(L.ldpc_mi355_plan_tiles(self._handle))
         hb, le, es = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int32()
         hoff = np.zeros(n, np.uint32)
         loff = np.zeros(n, np.uint32)

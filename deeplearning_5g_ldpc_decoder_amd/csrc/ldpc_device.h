@@ -44,6 +44,18 @@ struct DecParams {
     float fmax, fmin;           // algo 4: max_llr/128, min_llr/128
 };
 
+// A packed tile: `count` codeblocks of one (BG, Zc) share one workgroup of
+// kMaxThreads lanes, codeblock i on lanes [i*Zc, (i+1)*Zc) with its beliefs in
+// its own LDS region.  order[first .. first+count) are their indices in the
+// codeblock descriptor array.
+constexpr int kMaxTileCodeblocks = kMaxThreads / 2;
+struct TileDesc {
+    uint32_t first;
+    uint32_t etab_off;
+    uint16_t count, bgn, Zc, lds_stride;   // lds_stride: LQ entries per codeblock
+};
+static_assert(sizeof(TileDesc) == 16, "TileDesc layout");
+
 // Per-codeblock result flags (status) follow c_ldpc_5g_decode: 0 CRC pass,
 // 1 not converged.
 struct DecOut {

@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <thread>
@@ -168,6 +169,43 @@ CbDesc codeblock_desc(const Params &p, uint32_t etab_off) {
     d.soft_off = kNone;
     d.lq_off = kNone;
     return d;
+}
+
+void build_tiles(const std::vector<CbDesc> &descs, const uint32_t (*etab_off)[kMaxZc + 1],
+                 int (*k_of)(int bgn, int zc), std::vector<TileDesc> &tiles,
+                 std::vector<uint32_t> &order, int &lds_entries) {
+    std::vector<uint32_t> idx(descs.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (uint32_t)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (descs[a].bgn != descs[b].bgn) return descs[a].bgn < descs[b].bgn;
+        return descs[a].Zc > descs[b].Zc;
+    });
+    tiles.clear();
+    lds_entries = 0;
+    size_t i = 0;
+    while (i < idx.size()) {
+        const CbDesc &d0 = descs[idx[i]];
+        const int k = k_of(d0.bgn, d0.Zc);
+        size_t j = i;
+        while (j < idx.size() && j - i < (size_t)k && descs[idx[j]].bgn == d0.bgn &&
+               descs[idx[j]].Zc == d0.Zc)
+            ++j;
+        TileDesc t;
+        t.first = (uint32_t)i;
+        t.etab_off = etab_off[d0.bgn - 1][d0.Zc];
+        t.count = (uint16_t)(j - i);
+        t.bgn = d0.bgn;
+        t.Zc = d0.Zc;
+        t.lds_stride = (uint16_t)(((d0.N + 2 * d0.Zc) + 7) & ~7);
+        lds_entries = std::max(lds_entries, (int)t.count * (int)t.lds_stride);
+        tiles.push_back(t);
+        i = j;
+    }
+    order = idx;
+    std::stable_sort(tiles.begin(), tiles.end(), [](const TileDesc &a, const TileDesc &b) {
+        if (a.bgn != b.bgn) return a.bgn < b.bgn;
+        return a.count * a.Zc > b.count * b.Zc;
+    });
 }
 
 void edge_table(int bgn, int zc, int ils, uint32_t *out) {

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/nr_ldpc_mi355.h"
 #include "ldpc_device.h"
 
@@ -32,6 +34,14 @@ void rate_match_bits(const Params &p, const uint8_t *cw, uint8_t *out_bits);
 // The descriptor's buffer offsets are set to "none"; callers fill them.
 DecParams decoder_params(const ldpc_decoder_cfg_t &c);
 CbDesc codeblock_desc(const Params &p, uint32_t etab_off);
+// Groups codeblock descriptors by (BG, Zc) into packed tiles of up to
+// k_of(bgn, zc) codeblocks; order receives descriptor indices tile by tile,
+// lds_entries the largest tile's LQ footprint.  Tiles of base graph 1 (more
+// rows) come first, then fuller tiles, so the longest work starts earliest.
+void build_tiles(const std::vector<CbDesc> &descs, const uint32_t (*etab_off)[kMaxZc + 1],
+                 int (*k_of)(int bgn, int zc), std::vector<TileDesc> &tiles,
+                 std::vector<uint32_t> &order, int &lds_entries);
+
 // Workgroup size for lifting size zc: one lane per lifted check, whole waves.
 inline int workgroup_threads(int zc) { return ((zc + 63) / 64) * 64; }
 

@@ -344,13 +344,24 @@ __device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &
     st.sgn[R] = bits | ((uint32_t)idx << 24);
 }
 
+// All rows of one iteration; lanes with run = false only take part in the
+// barriers.
 template <int BG, int R, bool kGeneral>
 __device__ __forceinline__ void rows_i16(int16_t *lq, StateI16<BG> &st, const uint32_t *etab,
-                                         int zx, int Zc, const DecParams &p) {
+                                         int zx, int Zc, bool run, const DecParams &p) {
     if constexpr (R < Graph<BG>::kRows) {
-        if (zx < Zc) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p);
+        if (run) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p);
         __syncthreads();
-        rows_i16<BG, R + 1, kGeneral>(lq, st, etab, zx, Zc, p);
+        rows_i16<BG, R + 1, kGeneral>(lq, st, etab, zx, Zc, run, p);
+    }
+}
+
+template <int BG>
+__device__ __forceinline__ void init_state(StateI16<BG> &st) {
+#pragma unroll
+    for (int r = 0; r < Graph<BG>::kRows; ++r) {
+        st.mag[r] = 0u;
+        st.sgn[r] = 255u << 24;
     }
 }
 
@@ -358,16 +369,12 @@ template <int BG, bool kGeneral>
 __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
                        const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
     StateI16<BG> st;
-#pragma unroll
-    for (int r = 0; r < Graph<BG>::kRows; ++r) {
-        st.mag[r] = 0u;
-        st.sgn[r] = 255u << 24;
-    }
+    init_state<BG>(st);
     const int zx = threadIdx.x;
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, d.Zc, p);
+        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, d.Zc, zx < d.Zc, p);
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -638,6 +645,117 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packed tiles (algo 1): several codeblocks of one (BG, Zc) per workgroup, for
+// batches of mixed sizes.  Each codeblock keeps its own LDS beliefs, CRC and
+// early stop; the tile iterates until all of its codeblocks have stopped.
+// Per-codeblock CRC remainders are XOR-accumulated with LDS atomics into one
+// of two banks (bank = iteration parity); the bank not in use is cleared
+// right after each CRC barrier, when every lane has read it.
+// ---------------------------------------------------------------------------
+template <int BG, bool kGeneral>
+__device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int cbl,
+                             bool member, const CrcCtx &crc, const uint32_t *tab,
+                             uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
+                             int *status, int *iters) {
+    StateI16<BG> st;
+    init_state<BG>(st);
+    bool done = !member;
+    bool tile_active = __syncthreads_or(member);
+    const Crc &c = crc.spec;
+    for (int it = 0; it < p.max_it && tile_active; ++it) {
+        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, Zc, !done, p);
+        const int b = it & 1;
+        if (!done) {
+            uint32_t r = 0;
+            for (int k = crc.b_lo; k < crc.b_hi; ++k) {
+                const uint32_t byte = hard_byte(lq, k);
+                r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
+            }
+            if (r) atomicXor(&bank[b][cbl], crc_mulmod(r, crc.mult, c));
+        }
+        __syncthreads();
+        if (!done) {
+            *iters = it + 1;
+            if (bank[b][cbl] == 0u) {
+                done = true;
+                *status = 0;
+            }
+        }
+        if (member && zx == 0) bank[b ^ 1][cbl] = 0u;
+        tile_active = __syncthreads_or(!done);
+    }
+}
+
+template <bool kGeneral>
+__global__ void __launch_bounds__(kMaxThreads)
+ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p) {
+    NRLDPC_DYNAMIC_LDS(smem);
+    __shared__ uint32_t crc_tab[4][256];
+    __shared__ uint32_t bank[2][kMaxTileCodeblocks];
+
+    const TileDesc td = tiles[blockIdx.x];
+    const int Zc = td.Zc, lane = threadIdx.x, nt = blockDim.x;
+    const int cbl = lane / Zc;
+    const bool member = cbl < td.count;
+    const int zx = lane - cbl * Zc;
+    const int cbi = member ? (int)order[td.first + cbl] : 0;
+    const CbDesc d = o.desc[cbi];
+    const int n = d.N + 2 * Zc;
+    int16_t *lq = reinterpret_cast<int16_t *>(smem) + (member ? cbl : 0) * td.lds_stride;
+
+    // CRC tables of all four polynomials (codeblocks may differ in crctype)
+    for (int i = lane; i < 4 * 256; i += nt) {
+        const Crc cs = crc_spec(i >> 8);
+        uint32_t r = (uint32_t)(i & 255) << (cs.w - 8);
+        for (int k = 0; k < 8; ++k) r = crc_xtimes(r, cs);
+        crc_tab[i >> 8][i & 255] = r;
+    }
+    for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
+        bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
+    CrcCtx crc;
+    crc.spec = crc_spec(d.crctype);
+    crc.b_lo = crc.b_hi = 0;
+    crc.mult = 0u;
+    if (member) {
+        const int16_t *llr = o.llr + d.llr_off;
+        int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+        for (int i = zx; i < n; i += Zc) {
+            const int16_t v = recover_one(i, d, llr, soft, p.filler);
+            if (soft) soft[i] = v;
+            lq[i] = v;
+        }
+        const int nbytes = d.B / 8, per = (nbytes + Zc - 1) / Zc;
+        crc.b_lo = min(zx * per, nbytes);
+        crc.b_hi = min((zx + 1) * per, nbytes);
+        if (crc.b_hi > crc.b_lo) crc.mult = crc_x8pow(nbytes - crc.b_hi, crc.spec);
+    }
+    __syncthreads();
+
+    int status = 1, iters = 0;
+    const uint32_t *etab = o.etab + td.etab_off;
+    const uint32_t *tab = crc_tab[member ? d.crctype : 0];
+    if (p.max_it > 0) {
+        if (td.bgn == 1)
+            run_tile_i16<1, kGeneral>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p, &status,
+                                      &iters);
+        else
+            run_tile_i16<2, kGeneral>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p, &status,
+                                      &iters);
+    }
+    if (!member) return;
+    if (p.max_it > 0)
+        for (int b = zx; b < d.B / 8; b += Zc) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
+    if (zx == 0) {
+        o.status[cbi] = status;
+        o.iters[cbi] = iters;
+    }
+    if (d.lq_off != kNone) {
+        int16_t *out = reinterpret_cast<int16_t *>(o.lq) + d.lq_off;
+        for (int i = zx; i < n; i += Zc) out[i] = lq[i];
+    }
+}
+
 }  // namespace nrldpc
 
 #ifndef NRLDPC_CPU_EMU
@@ -695,8 +813,32 @@ extern "C" hipError_t nrldpc_configure_kernels() {
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
 #endif
-    return e;
+    if (e != hipSuccess) return e;
+    // tiles hold at most 384 lanes' worth of beliefs (~56 KB) next to 5.6 KB
+    // of static LDS
+    const int tile_lds = 96 * 1024;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ldpc_tile_kernel<false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&ldpc_tile_kernel<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
 }
 
 extern "C" int nrldpc_bp_kernel_enabled() { return NRLDPC_BP_KERNEL; }
+
+// Packed-tile launch (algo 1): ntiles workgroups of kMaxThreads lanes.
+extern "C" hipError_t nrldpc_launch_tiles(const nrldpc::DecOut *o, const nrldpc::TileDesc *tiles,
+                                          const uint32_t *order, const nrldpc::DecParams *p,
+                                          int ntiles, int lds_entries, hipStream_t stream) {
+    using namespace nrldpc;
+    const size_t lds = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
+    const dim3 grid(ntiles), block(kMaxThreads);
+    if (p->alpha >= 0 && p->min_llr <= p->max_llr)
+        hipLaunchKernelGGL((ldpc_tile_kernel<false>), grid, block, lds, stream, *o, tiles, order,
+                           *p);
+    else
+        hipLaunchKernelGGL((ldpc_tile_kernel<true>), grid, block, lds, stream, *o, tiles, order,
+                           *p);
+    return hipGetLastError();
+}
 #endif  // NRLDPC_CPU_EMU

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -18,6 +19,9 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
 extern "C" int nrldpc_bp_kernel_enabled();
+extern "C" hipError_t nrldpc_launch_tiles(const DecOut *o, const TileDesc *tiles,
+                                          const uint32_t *order, const DecParams *p, int ntiles,
+                                          int lds_entries, hipStream_t stream);
 
 namespace {
 
@@ -212,7 +216,37 @@ struct ldpc_mi355_plan {
     uint64_t hard_bytes = 0, lq_elems = 0;
     std::vector<uint32_t> hard_off, lq_off;
     bool want_lq = false;
+    // packed tiles (algo 1): several codeblocks of one (BG, Zc) per workgroup
+    int ntiles = 0, tile_lds = 0;
+    TileDesc *d_tiles = nullptr;
+    uint32_t *d_order = nullptr;
 };
+
+namespace {
+// Codeblocks per packed tile, by (BG, Zc); 0 = default (as many as fit).
+int g_tile_k[2][kMaxZc + 1] = {};
+std::mutex g_tile_mu;
+
+int tile_codeblocks(int bgn, int zc) {
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    const int cap = std::min(kMaxThreads / zc, kMaxTileCodeblocks);
+    const int k = g_tile_k[bgn - 1][zc];
+    return k > 0 ? std::min(k, cap) : cap;
+}
+
+}  // namespace
+
+extern "C" int32_t ldpc_mi355_set_tile_codeblocks(int32_t bgn, int32_t zc, int32_t k) {
+    if ((bgn != 1 && bgn != 2) || zc < 2 || zc > kMaxZc || lifting_set(zc) < 0 || k < 0)
+        return -1;
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    g_tile_k[bgn - 1][zc] = k;
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_plan_tiles(const ldpc_mi355_plan_t *plan) {
+    return plan ? plan->ntiles : -1;
+}
 
 extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs,
                                           int32_t n, const ldpc_decoder_cfg_t *dec,
@@ -270,6 +304,37 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
         ldpc_mi355_plan_destroy(pl);
         return -1;
     }
+    // Packed tiles for algo 1 when asked (flags bit 1) or when some lifting
+    // size leaves room for two or more codeblocks per workgroup (bit 2
+    // forbids packing).
+    bool pack = false;
+    if (pl->algo == 1 && !(flags & 4u)) {
+        if (flags & 2u) {
+            pack = true;
+        } else {
+            for (const CbDesc &d : descs)
+                if (tile_codeblocks(d.bgn, d.Zc) > 1) {
+                    pack = true;
+                    break;
+                }
+        }
+    }
+    if (pack) {
+        std::vector<TileDesc> tiles;
+        std::vector<uint32_t> order;
+        build_tiles(descs, r.etab_off, tile_codeblocks, tiles, order, pl->tile_lds);
+        pl->ntiles = (int)tiles.size();
+        if (hipMalloc(&pl->d_tiles, sizeof(TileDesc) * tiles.size()) != hipSuccess ||
+            hipMemcpy(pl->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc(&pl->d_order, sizeof(uint32_t) * order.size()) != hipSuccess ||
+            hipMemcpy(pl->d_order, order.data(), sizeof(uint32_t) * order.size(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
+            ldpc_mi355_plan_destroy(pl);
+            return -1;
+        }
+    }
     *plan = pl;
     return 0;
 }
@@ -306,8 +371,12 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     // NULL is HIP's default (null) stream, as everywhere in HIP -- it is also
     // the handle PyTorch reports for its default stream.
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
-                                plan->max_soft, s));
+    if (plan->ntiles > 0)
+        HIP_OK(nrldpc_launch_tiles(&o, plan->d_tiles, plan->d_order, &plan->dp, plan->ntiles,
+                                   plan->tile_lds, s));
+    else
+        HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
+                                    plan->max_soft, s));
     return 0;
 }
 
@@ -315,6 +384,8 @@ extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
     if (!plan) return;
     if (plan->d_desc) (void)hipFree(plan->d_desc);
     if (plan->d_work) (void)hipFree(plan->d_work);
+    if (plan->d_tiles) (void)hipFree(plan->d_tiles);
+    if (plan->d_order) (void)hipFree(plan->d_order);
     delete plan;
 }
 

@@ -156,7 +156,10 @@ typedef struct {
 typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
 
 /* Validates n codeblock descriptors against one decoder row and uploads
- * them to the device.  flags: bit 0 = produce posteriors.  Returns 0, -2 if
+ * them to the device.  flags: bit 0 = produce posteriors; bit 1 = pack
+ * codeblocks of one (BG, Zc) into shared workgroups (algo 1); bit 2 = never
+ * pack.  Without bits 1/2, algo-1 plans pack whenever some lifting size
+ * leaves room for two or more codeblocks per workgroup.  Returns 0, -2 if
  * any codeblock config is invalid (index in *bad_index when non-NULL), -3
  * for an unsupported algo, -1 on a GPU error. */
 int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs, int32_t n,
@@ -180,6 +183,16 @@ int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const int16_t *llr
                                int32_t *iterations, void *lq, void *stream);
 
 void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan);
+
+/* Number of packed workgroups of a plan (0 = one codeblock per workgroup),
+ * -1 for NULL. */
+int32_t ldpc_mi355_plan_tiles(const ldpc_mi355_plan_t *plan);
+
+/* Codeblocks per packed workgroup for lifting size zc of base graph bgn,
+ * used by plans created afterwards (k = 0 restores the default, as many as
+ * fit in 384 lanes; larger k is capped there).  Set by the autotuner.
+ * 0, or -1 for an invalid (bgn, zc, k). */
+int32_t ldpc_mi355_set_tile_codeblocks(int32_t bgn, int32_t zc, int32_t k);
 
 /* 1 when this build runs decoder algo `algo` on the GPU, else 0 (algo 0,
  * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls

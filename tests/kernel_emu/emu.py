@@ -18,6 +18,8 @@ def lib():
         P = ctypes.c_void_p
         L.emu_decode_batch.argtypes = [ctypes.c_int] + [P] * 13 + [ctypes.c_uint64]
         L.emu_decode_batch.restype = ctypes.c_int
+        L.emu_decode_tiles.argtypes = [ctypes.c_int] + [P] * 12 + [ctypes.c_int]
+        L.emu_decode_tiles.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -32,8 +34,11 @@ def dims(cfg):
     return p
 
 
-def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True):
-    """Run the kernel source on the CPU.  Returns per-codeblock lists."""
+def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True,
+                 tile_k=None):
+    """Run the kernel source on the CPU (packed tiles of tile_k codeblocks
+    when tile_k is not None, 0 = as many as fit).  Returns per-codeblock
+    dicts."""
     n = len(cfgs)
     cf = np.array(cfgs, np.int32).reshape(n, 7)
     ps = [dims(tuple(c)) for c in cfgs]
@@ -49,11 +54,18 @@ def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True)
     nwork = sum((316 if c[0] == 1 else 197) * p["Zc"] for c, p in zip(cfgs, ps))
     work = np.zeros(max(nwork, 1), np.float32)
     slots = None if soft_slots is None else np.ascontiguousarray(soft_slots, np.int32)
-    rc = lib().emu_decode_batch(n, _p(cf), _p(off), _p(np.array(row, np.int32)), _p(llr),
-                                _p(soft_pool), _p(slots), _p(hard), _p(st), _p(it), _p(lq),
-                                _p(hoff), _p(loff), _p(work), ctypes.c_uint64(len(work)))
-    if rc != 0:
-        raise RuntimeError(f"emu_decode_batch returned {rc}")
+    if tile_k is None:
+        rc = lib().emu_decode_batch(n, _p(cf), _p(off), _p(np.array(row, np.int32)), _p(llr),
+                                    _p(soft_pool), _p(slots), _p(hard), _p(st), _p(it), _p(lq),
+                                    _p(hoff), _p(loff), _p(work), ctypes.c_uint64(len(work)))
+        if rc != 0:
+            raise RuntimeError(f"emu_decode_batch returned {rc}")
+    else:
+        rc = lib().emu_decode_tiles(n, _p(cf), _p(off), _p(np.array(row, np.int32)), _p(llr),
+                                    _p(soft_pool), _p(slots), _p(hard), _p(st), _p(it), _p(lq),
+                                    _p(hoff), _p(loff), int(tile_k))
+        if rc <= 0:
+            raise RuntimeError(f"emu_decode_tiles returned {rc}")
     out = []
     for i, p in enumerate(ps):
         h = hard[hoff[i]:hoff[i] + p["B"] // 8]

@@ -25,48 +25,56 @@ struct EdgeTables {
         }
     }
 };
-}  // namespace
 
-// n codeblocks, cfgs[7*i..] = ldpc_in_config_t fields, one decoder row.
-// Offsets of each codeblock's hard bytes / posteriors are returned.
-// Returns 0, -2 for an invalid config, -3 for an unknown algo.
-extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_off,
-                                const int32_t *row, const int16_t *llr, int16_t *soft_pool,
-                                const int32_t *soft_slot, uint8_t *hard, int32_t *status,
-                                int32_t *iters, void *lq, uint32_t *hard_off,
-                                uint32_t *lq_off, float *work, uint64_t work_elems) {
+EdgeTables &tables() {
     static EdgeTables et;
+    return et;
+}
+
+struct Batch {
+    std::vector<CbDesc> descs;
+    DecParams p;
+    int max_zc = 2, max_soft = 0;
+    uint64_t work = 0;
+};
+
+// Descriptors exactly as ldpc_mi355_plan_create lays them out.
+int make_batch(int n, const int32_t *cfgs, const uint32_t *llr_off, const int32_t *row,
+               const int32_t *soft_slot, bool want_lq, uint32_t *hard_off, uint32_t *lq_off,
+               Batch &b) {
     ldpc_decoder_cfg_t dc{(uint16_t)row[0], (uint16_t)row[1], (uint16_t)row[2], (uint16_t)row[3],
                           (int16_t)row[4], (int16_t)row[5], (int16_t)row[6]};
-    const DecParams p = decoder_params(dc);
-    std::vector<CbDesc> descs(n);
+    b.p = decoder_params(dc);
+    b.descs.resize(n);
     uint32_t h = 0, q = 0;
-    uint64_t w = 0;
-    int max_zc = 2, max_soft = 0;
     for (int i = 0; i < n; ++i) {
         const int32_t *c = cfgs + 7 * i;
         ldpc_in_config_t in{(uint16_t)c[0], (uint16_t)c[1], (uint16_t)c[2], (uint16_t)c[3],
                             (uint16_t)c[4], (uint16_t)c[5], (uint16_t)c[6]};
         Params pr;
         if (derive(in, pr)) return -2;
-        CbDesc d = codeblock_desc(pr, et.off[pr.bgn - 1][pr.Zc]);
+        CbDesc d = codeblock_desc(pr, tables().off[pr.bgn - 1][pr.Zc]);
         d.llr_off = llr_off[i];
         d.hard_off = h;
         d.soft_off = soft_slot && soft_slot[i] >= 0 ? (uint32_t)soft_slot[i] * kMaxSoft : kNone;
-        d.lq_off = lq ? q : kNone;
-        d.work_off = (uint32_t)w;
+        d.lq_off = want_lq ? q : kNone;
+        d.work_off = (uint32_t)b.work;
         hard_off[i] = h;
         lq_off[i] = q;
         h += pr.B / 8;
         q += pr.N + 2 * pr.Zc;
-        w += (uint64_t)pr.edges * pr.Zc;
-        max_zc = pr.Zc > max_zc ? pr.Zc : max_zc;
-        max_soft = pr.N + 2 * pr.Zc > max_soft ? pr.N + 2 * pr.Zc : max_soft;
-        descs[i] = d;
+        b.work += (uint64_t)pr.edges * pr.Zc;
+        b.max_zc = std::max(b.max_zc, pr.Zc);
+        b.max_soft = std::max(b.max_soft, pr.N + 2 * pr.Zc);
+        b.descs[i] = d;
     }
-    if (row[0] == 0 && w > work_elems) return -1;
+    return 0;
+}
+
+DecOut outputs(const Batch &b, const int16_t *llr, int16_t *soft_pool, uint8_t *hard,
+               int32_t *status, int32_t *iters, void *lq, float *work) {
     DecOut o;
-    o.desc = descs.data();
+    o.desc = b.descs.data();
     o.llr = llr;
     o.soft = soft_pool;
     o.hard = hard;
@@ -74,26 +82,80 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     o.iters = iters;
     o.lq = lq;
     o.work = work;
-    o.etab = et.tab.data();
-    const unsigned threads = (unsigned)workgroup_threads(max_zc);
+    o.etab = tables().tab.data();
+    return o;
+}
+
+int g_fixed_k = 0;
+int fixed_k(int bgn, int zc) {
+    (void)bgn;
+    const int cap = std::min(kMaxThreads / zc, kMaxTileCodeblocks);
+    return g_fixed_k > 0 ? std::min(g_fixed_k, cap) : cap;
+}
+}  // namespace
+
+// n codeblocks, cfgs[7*i..] = ldpc_in_config_t fields, one decoder row, one
+// codeblock per workgroup.  Returns 0, -2 for an invalid config, -3 for an
+// unknown algo, -1 when the BP message store is too small.
+extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_off,
+                                const int32_t *row, const int16_t *llr, int16_t *soft_pool,
+                                const int32_t *soft_slot, uint8_t *hard, int32_t *status,
+                                int32_t *iters, void *lq, uint32_t *hard_off,
+                                uint32_t *lq_off, float *work, uint64_t work_elems) {
+    Batch b;
+    if (int rc = make_batch(n, cfgs, llr_off, row, soft_slot, lq != nullptr, hard_off, lq_off, b))
+        return rc;
+    if (row[0] == 0 && b.work > work_elems) return -1;
+    const DecOut o = outputs(b, llr, soft_pool, hard, status, iters, lq, work);
+    const DecParams p = b.p;
+    const unsigned threads = (unsigned)workgroup_threads(b.max_zc);
     switch (row[0]) {
     case 1:
         if (p.alpha >= 0 && p.min_llr <= p.max_llr)
-            emu_launch(n, threads, (size_t)max_soft * 2,
+            emu_launch(n, threads, (size_t)b.max_soft * 2,
                        [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>(o, p); });
         else
-            emu_launch(n, threads, (size_t)max_soft * 2,
+            emu_launch(n, threads, (size_t)b.max_soft * 2,
                        [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>(o, p); });
         return 0;
     case 4:
-        emu_launch(n, threads, (size_t)max_soft * 4,
+        emu_launch(n, threads, (size_t)b.max_soft * 4,
                    [&] { ldpc_decode_kernel<Algo::kMinSumF32, float>(o, p); });
         return 0;
     case 0:
-        emu_launch(n, threads, (size_t)max_soft * 4,
+        emu_launch(n, threads, (size_t)b.max_soft * 4,
                    [&] { ldpc_decode_kernel<Algo::kBpF32, float>(o, p); });
         return 0;
     default:
         return -3;
     }
+}
+
+// Same batch through the packed-tile kernel (algo 1) with k codeblocks per
+// tile (0 = as many as fit).  Returns the number of tiles, or < 0.
+extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_off,
+                                const int32_t *row, const int16_t *llr, int16_t *soft_pool,
+                                const int32_t *soft_slot, uint8_t *hard, int32_t *status,
+                                int32_t *iters, void *lq, uint32_t *hard_off, uint32_t *lq_off,
+                                int k) {
+    if (row[0] != 1) return -3;
+    Batch b;
+    if (int rc = make_batch(n, cfgs, llr_off, row, soft_slot, lq != nullptr, hard_off, lq_off, b))
+        return rc;
+    std::vector<TileDesc> tiles;
+    std::vector<uint32_t> order;
+    int lds_entries = 0;
+    g_fixed_k = k;
+    build_tiles(b.descs, tables().off, fixed_k, tiles, order, lds_entries);
+    const DecOut o = outputs(b, llr, soft_pool, hard, status, iters, lq, nullptr);
+    const DecParams p = b.p;
+    const TileDesc *td = tiles.data();
+    const uint32_t *ord = order.data();
+    if (p.alpha >= 0 && p.min_llr <= p.max_llr)
+        emu_launch((unsigned)tiles.size(), kMaxThreads, (size_t)lds_entries * 2,
+                   [&] { ldpc_tile_kernel<false>(o, td, ord, p); });
+    else
+        emu_launch((unsigned)tiles.size(), kMaxThreads, (size_t)lds_entries * 2,
+                   [&] { ldpc_tile_kernel<true>(o, td, ord, p); });
+    return (int)tiles.size();
 }

@@ -8,6 +8,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <barrier>
 #include <cstring>
 #include <functional>
@@ -52,6 +53,23 @@ inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 inline unsigned char *emu_dynamic_lds() { return g_emu_block.lds.data(); }
 inline void __syncthreads() { g_emu_block.bar->arrive_and_wait(); }
 inline int __popc(uint32_t v) { return __builtin_popcount(v); }
+
+inline int __syncthreads_or(int pred) {
+    static std::atomic<int> acc[2];
+    static thread_local int phase = 0;
+    const int ph = phase;
+    phase ^= 1;
+    if (pred) acc[ph].fetch_or(1);
+    __syncthreads();
+    const int r = acc[ph].load();
+    __syncthreads();
+    if (threadIdx.x == 0) acc[ph].store(0);   // cleared before the next use of this phase
+    return r;
+}
+
+inline uint32_t atomicXor(uint32_t *addr, uint32_t v) {
+    return __atomic_fetch_xor(addr, v, __ATOMIC_SEQ_CST);
+}
 
 inline uint32_t __shfl_xor(uint32_t v, int mask, int width) {
     (void)width;

@@ -1,0 +1,108 @@
+"""GPU tests of batched decoding: packed tiles over every lifting size, the
+plain vs packed kernels, hipGraph capture/replay.  Bit-exact vs the oracle."""
+import numpy as np
+import pytest
+
+from test_kernel_emu import ALL_ZC, _all_zc_batch
+
+pytestmark = pytest.mark.gpu
+
+ROW1 = [1, 8, 22936, 0, 1280, 32767, -32767]
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    _lib.load()
+    return torch
+
+
+def _check(oracle_mod, plan, out, cfgs, llrs, row):
+    hard = plan.split_hard(out["hard"].cpu().numpy())
+    lq = plan.split_lq(out["lq"].cpu().numpy()) if "lq" in out else None
+    st = out["status"].cpu().numpy()
+    its = out["iters"].cpu().numpy()
+    for i, (cfg, llr) in enumerate(zip(cfgs, llrs)):
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+        assert (st[i], its[i]) == (s, k), (i, cfg)
+        assert np.array_equal(hard[i], h), (i, cfg)
+        if lq is not None:
+            assert np.array_equal(lq[i], q), (i, cfg)
+
+
+@pytest.mark.parametrize("flags_k", [(2, 0), (2, 4), (4, 0)])
+def test_all_lifting_sizes_packed_and_plain(torch_dev, oracle_mod, flags_k):
+    """Every reachable (BG, Zc), rates 0.25..0.85, Qm 2..8, rv 0..3 in one
+    plan: packed tiles (default and k = 4 per tile) and one codeblock per
+    workgroup, all bit-exact with the oracle."""
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    flags, k = flags_k
+    L = _lib.load()
+    rng = np.random.default_rng(77 + k)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=3, zcs=ALL_ZC)
+    assert len(cfgs) > 250
+    for zc in ALL_ZC:
+        for bgn in (1, 2):
+            L.ldpc_mi355_set_tile_codeblocks(bgn, zc, k)
+    try:
+        plan = BatchPlan(cfgs, ROW1, want_lq=True, flags=flags)
+        assert (plan.tiles > 0) == (flags == 2)
+        out = plan.decode(torch_dev.from_numpy(np.concatenate(llrs)).cuda())
+        _check(oracle_mod, plan, out, cfgs, llrs, ROW1)
+        plan.close()
+    finally:
+        for zc in ALL_ZC:
+            for bgn in (1, 2):
+                L.ldpc_mi355_set_tile_codeblocks(bgn, zc, 0)
+
+
+def test_packed_general_rows(torch_dev, oracle_mod):
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    rng = np.random.default_rng(8)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=ALL_ZC[::3])
+    d = torch_dev.from_numpy(np.concatenate(llrs)).cuda()
+    for row in ([1, 6, 40000, 16, 1280, 32767, -32767], [1, 6, 22936, 0, 1280, 300, 2000],
+                [1, 0, 22936, 0, 1280, 32767, -32767]):
+        plan = BatchPlan(cfgs, row, want_lq=True, flags=2)
+        out = plan.decode(d)
+        if row[1] == 0:
+            st = out["status"].cpu().numpy()
+            its = out["iters"].cpu().numpy()
+            assert np.all(st == 1) and np.all(its == 0)
+        else:
+            _check(oracle_mod, plan, out, cfgs, llrs, row)
+        plan.close()
+
+
+def test_hip_graph_capture_replay(torch_dev, oracle_mod):
+    """A packed plan captured in a HIP graph: every replay, with eager work
+    and synchronizations in between, equals the eager launch."""
+    torch = torch_dev
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    rng = np.random.default_rng(12)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=ALL_ZC[::2])
+    d_llr = torch.from_numpy(np.concatenate(llrs)).cuda()
+    plan = BatchPlan(cfgs, ROW1, want_lq=True)
+    eager = plan.decode(d_llr)
+    ref = {k: v.cpu().numpy() for k, v in eager.items()}
+    out = plan.alloc_outputs("cuda")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.launch(d_llr, out, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.launch(d_llr, out, stream=s.cuda_stream)
+    for rep in range(4):
+        for t in out.values():
+            t.fill_(7 + rep)            # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k, v in out.items():
+            assert np.array_equal(v.cpu().numpy(), ref[k]), (rep, k)
+    plan.close()

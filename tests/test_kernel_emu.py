@@ -89,3 +89,76 @@ def test_emu_mixed_batch_matches_oracle(oracle_mod, row):
         assert (out["status"], out["iters"]) == (s, k), cfg
         assert np.array_equal(out["hard"], h), cfg
         assert np.array_equal(out["lq"].view(np.uint8), q.view(np.uint8)), cfg
+
+
+ALL_ZC = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 32,
+          36, 40, 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128, 144, 160, 176,
+          192, 208, 224, 240, 256, 288, 320, 352, 384]
+# spread of small / odd / large lifting sizes for the CPU-emulated runs
+SOME_ZC = [2, 3, 7, 15, 26, 44, 64, 120, 208, 384]
+
+
+def _all_zc_batch(oracle_mod, rng, per_zc=2, zcs=ALL_ZC):
+    """Codeblocks covering the given lifting sizes (TS 38.212 Table 5.3.2-1)
+    for each base graph where reachable, rates 0.25..0.85, Qm 2..8."""
+    cfgs, llrs = [], []
+    for bgn in (1, 2):
+        for zc in zcs:
+            found = None
+            kb = 22 if bgn == 1 else 10
+            for crc, clen in ((2, 24), (0, 16)):
+                for B in range((kb * zc) // 8 * 8, clen + 7, -8):
+                    A = B - clen
+                    st, p = oracle_mod.params(bgn, A, crc, 1000, 0, 2)
+                    if st == 0 and p["Zc"] == zc:
+                        found = (A, crc, B)
+                        break
+                if found:
+                    break
+            if found is None:
+                continue
+            A, crc, B = found
+            for _ in range(per_zc):
+                Qm = int(rng.choice([2, 4, 6, 8]))
+                R = float(rng.uniform(0.25, 0.85))
+                E = int(B / R)
+                E -= E % Qm
+                rv = int(rng.integers(0, 4))
+                cb = oracle_mod.make_codeblock(bgn, A, crc, E, rv, Qm,
+                                               float(rng.uniform(-4, -1)) + 10 * np.log10(Qm) / 2,
+                                               rng)
+                cfgs.append((bgn, A, crc, E, rv, Qm, 0))
+                llrs.append(cb["llr"])
+    return cfgs, llrs
+
+
+@pytest.mark.parametrize("tile_k", [0, 3])
+def test_emu_packed_tiles_lifting_sizes(oracle_mod, tile_k):
+    """Packed tiles (several codeblocks of one BG/Zc per 384-lane workgroup,
+    per-codeblock CRC and early stop) over every reachable lifting size,
+    bit-exact with the oracle per codeblock."""
+    rng = np.random.default_rng(31 + tile_k)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=3, zcs=SOME_ZC)
+    assert len(cfgs) >= 50
+    row = [1, 8, 22936, 0, 1280, 32767, -32767]
+    outs = emu.decode_batch(cfgs, row, llrs, tile_k=tile_k)
+    for cfg, llr, out in zip(cfgs, llrs, outs):
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+        assert (out["status"], out["iters"]) == (s, k), cfg
+        assert np.array_equal(out["hard"], h), cfg
+        assert np.array_equal(out["lq"], q), cfg
+
+
+def test_emu_packed_tiles_general_row(oracle_mod):
+    """Packed tiles with a decoder row that needs the exact general path
+    (alpha negative as int16, min_llr > max_llr)."""
+    rng = np.random.default_rng(5)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=SOME_ZC)
+    for row in ([1, 6, 40000, 16, 1280, 32767, -32767], [1, 6, 22936, 0, 1280, 300, 2000]):
+        outs = emu.decode_batch(cfgs, row, llrs, tile_k=0)
+        for cfg, llr, out in zip(cfgs, llrs, outs):
+            c = oracle_mod.code(*cfg)
+            s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+            assert (out["status"], out["iters"]) == (s, k), (row, cfg)
+            assert np.array_equal(out["lq"], q), (row, cfg)
