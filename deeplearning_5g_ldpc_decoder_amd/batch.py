@@ -22,6 +22,32 @@ def _decoder(row):
     return DecoderConfig(*[int(v) for v in row])
 
 
+_tuning_applied = False
+
+
+def apply_tuning(path=None):
+    """Load the autotuned codeblocks-per-workgroup table (tile_table.json)
+    into the library; plans built afterwards use it.  Missing table: the
+    library's default (as many codeblocks as fit in 384 lanes)."""
+    global _tuning_applied
+    import json
+    import os
+    if path is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
+    _tuning_applied = True
+    if not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        table = json.load(f)
+    L = _lib.load()
+    n = 0
+    for key, v in table.get("classes", {}).items():
+        bgn, zc = (int(x) for x in key.split(":"))
+        if L.ldpc_mi355_set_tile_codeblocks(bgn, zc, int(v["k"])) == 0:
+            n += 1
+    return n
+
+
 class BatchPlan:
     """A fixed batch of codeblocks and one decoder row.
 
@@ -37,6 +63,8 @@ class BatchPlan:
     def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False,
                  flags=0):
         L = _lib.load()
+        if not _tuning_applied:
+            apply_tuning()
         n = len(cfgs)
         if n == 0:
             raise ValueError("empty batch")
