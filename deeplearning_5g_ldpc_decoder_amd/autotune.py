@@ -51,7 +51,7 @@ def tune_class(torch, bgn, zc, reps=4, distinct=16, cus=256):
     B = A + (16 if crc == 0 else 24)
     E = (2 * B) // 2 * 2                      # rate 1/2, QPSK
     cfg = (bgn, A, crc, E, 0, 2, 0)
-    llr, _ = synth_batch(cfg, distinct, -(waterfall_esn0_db(B / E, 2) + 0.5), seed=zc + 97 * bgn)
+    llr, _ = synth_batch(cfg, distinct, -(waterfall_esn0_db(B / E, 2, B) + 0.5), seed=zc + 97 * bgn)
     d_llr = torch.from_numpy(llr.reshape(-1)).cuda()
     L = _lib.load()
     cap = min(384 // zc, 192)
