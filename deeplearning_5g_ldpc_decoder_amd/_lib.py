@@ -19,7 +19,11 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_plan_layout", "ldpc_mi355_plan_decode", "ldpc_mi355_plan_destroy",
     "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
     "ldpc_mi355_algo_available", "ldpc_mi355_plan_tiles", "ldpc_mi355_set_tile_codeblocks",
-    "ldpc_mi355_synth_batch",
+    "ldpc_mi355_synth_batch", "ldpc_mi355_synth_float_batch",
+    "ldpc_mi355_unrolled_create", "ldpc_mi355_unrolled_destroy", "ldpc_mi355_unrolled_dims",
+    "ldpc_mi355_unrolled_set_codeblocks_per_workgroup", "ldpc_mi355_unrolled_record_bytes",
+    "ldpc_mi355_unrolled_scratch_bytes", "ldpc_mi355_unrolled_forward",
+    "ldpc_mi355_unrolled_backward",
 )
 EXPORTED_DATA = (
     "BG1_HBG_offset_array", "BG1_HBG_array", "BG1_HBG_Vij_array",
@@ -131,6 +135,26 @@ def load():
     L.ldpc_mi355_synth_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32,
                                          ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, P, P]
     L.ldpc_mi355_synth_batch.restype = ctypes.c_int32
+    L.ldpc_mi355_synth_float_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32, P,
+                                               ctypes.c_uint64, ctypes.c_int32, P, P]
+    L.ldpc_mi355_synth_float_batch.restype = ctypes.c_int32
+    L.ldpc_mi355_unrolled_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(LdpcInConfig),
+                                             ctypes.c_int32, ctypes.c_float, ctypes.c_int32]
+    L.ldpc_mi355_unrolled_create.restype = ctypes.c_int32
+    L.ldpc_mi355_unrolled_destroy.argtypes = [P]
+    L.ldpc_mi355_unrolled_destroy.restype = None
+    L.ldpc_mi355_unrolled_dims.argtypes = [P, P]
+    L.ldpc_mi355_unrolled_dims.restype = ctypes.c_int32
+    L.ldpc_mi355_unrolled_set_codeblocks_per_workgroup.argtypes = [P, ctypes.c_int32]
+    L.ldpc_mi355_unrolled_set_codeblocks_per_workgroup.restype = ctypes.c_int32
+    L.ldpc_mi355_unrolled_record_bytes.argtypes = [P, ctypes.c_int32]
+    L.ldpc_mi355_unrolled_record_bytes.restype = ctypes.c_uint64
+    L.ldpc_mi355_unrolled_scratch_bytes.argtypes = [P, ctypes.c_int32]
+    L.ldpc_mi355_unrolled_scratch_bytes.restype = ctypes.c_uint64
+    L.ldpc_mi355_unrolled_forward.argtypes = [P, ctypes.c_int32] + [P] * 7
+    L.ldpc_mi355_unrolled_forward.restype = ctypes.c_int32
+    L.ldpc_mi355_unrolled_backward.argtypes = [P, ctypes.c_int32] + [P] * 11
+    L.ldpc_mi355_unrolled_backward.restype = ctypes.c_int32
     _lib = L
     return L
 

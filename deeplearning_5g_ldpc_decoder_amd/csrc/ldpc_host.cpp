@@ -429,7 +429,9 @@ inline int16_t quantize(double llr) {
     return (int16_t)q;
 }
 
-void channel_symbols(const uint8_t *bits, size_t nsym, int Qm, double nv, Rng &rng, int16_t *out) {
+// Channel LLRs of nsym symbols; emit(i, llr) receives LLR i (double).
+template <typename Emit>
+void channel_llrs(const uint8_t *bits, size_t nsym, int Qm, double nv, Rng &rng, Emit emit) {
     const double sigma = sqrt(nv / 2.0);
     if (Qm == 1) {
         const double A = 1.0 / sqrt(2.0);
@@ -437,7 +439,7 @@ void channel_symbols(const uint8_t *bits, size_t nsym, int Qm, double nv, Rng &r
             const double s = (1 - 2 * bits[i]) * A;
             double na, nb;
             rng.normal2(na, nb);
-            out[i] = quantize(4.0 * ((s + sigma * na) + (s + sigma * nb)) * A / nv);
+            emit(i, 4.0 * ((s + sigma * na) + (s + sigma * nb)) * A / nv);
         }
         return;
     }
@@ -467,10 +469,28 @@ void channel_symbols(const uint8_t *bits, size_t nsym, int Qm, double nv, Rng &r
                     if (lbits[v][k]) d1 = d < d1 ? d : d1;
                     else d0 = d < d0 ? d : d0;
                 }
-                out[i * Qm + 2 * k + ax] = quantize((d1 - d0) / nv);
+                emit(i * Qm + 2 * k + ax, (d1 - d0) / nv);
             }
     }
 }
+
+void channel_symbols(const uint8_t *bits, size_t nsym, int Qm, double nv, Rng &rng, int16_t *out) {
+    channel_llrs(bits, nsym, Qm, nv, rng, [out](size_t i, double v) { out[i] = quantize(v); });
+}
+
+// Random payload + CRC (B bits, one per byte) from rng.
+void random_block(const Params &p, Rng &rng, uint8_t *b, std::vector<uint8_t> &bytes) {
+    for (int k = 0; k < p.A; k += 64) {
+        uint64_t w = rng.next();
+        for (int j = 0; j < 64 && k + j < p.A; ++j) b[k + j] = (w >> j) & 1u;
+    }
+    memset(bytes.data(), 0, bytes.size());
+    for (int k = 0; k < p.A; ++k) bytes[k >> 3] |= (uint8_t)(b[k] << (7 - (k & 7)));
+    const uint32_t crc = calculate_crc(bytes.data(), p.A / 8, p.crctype);
+    const int clen = p.B - p.A;
+    for (int k = 0; k < clen; ++k) b[p.A + k] = (crc >> (clen - 1 - k)) & 1u;
+}
+
 }  // namespace
 
 extern "C" void ldpc_mi355_channel_seed(uint64_t seed) {
@@ -512,20 +532,75 @@ extern "C" int32_t ldpc_mi355_synth_batch(const ldpc_in_config_t *cfg, int32_t n
         std::vector<uint8_t> bytes((size_t)p.A / 8 + 1);
         for (int i; (i = next.fetch_add(1)) < n;) {
             Rng rng(seed * 0x100000001B3ull + (uint64_t)i);
-            for (int k = 0; k < p.A; k += 64) {
-                uint64_t w = rng.next();
-                for (int j = 0; j < 64 && k + j < p.A; ++j) b[k + j] = (w >> j) & 1u;
-            }
-            memset(bytes.data(), 0, bytes.size());
-            for (int k = 0; k < p.A; ++k) bytes[k >> 3] |= (uint8_t)(b[k] << (7 - (k & 7)));
-            const uint32_t crc = calculate_crc(bytes.data(), p.A / 8, p.crctype);
-            const int clen = p.B - p.A;
-            for (int k = 0; k < clen; ++k) b[p.A + k] = (crc >> (clen - 1 - k)) & 1u;
+            random_block(p, rng, b.data(), bytes);
             encode_codeword(p, b.data(), cw.data());
             rate_match_bits(p, cw.data(), rm.data());
             channel_symbols(rm.data(), (size_t)p.E / p.Qm, p.Qm, nv, rng,
                             llr_out + (size_t)i * p.E);
             if (payload_out) memcpy(payload_out + (size_t)i * p.A, b.data(), (size_t)p.A);
+        }
+    };
+    if (threads < 1) threads = 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    return 0;
+}
+
+// Float inputs of the trainable decoder in its belief layout
+// (deep_learning/ldpc_training_data.py:24-82): random payload + CRC,
+// encoded, rate-matched, unquantized channel LLRs (py5gphy's demodulators
+// give the same max-log values as the test channel above, in float), then the
+// float rate recovery of py5gphy/ldpc/nr_ldpc_raterecover.py:15-68: Qm
+// deinterleave, circular buffer from k0 over Ncb = N skipping filler
+// positions with repeats summed, filler positions set to 10 * max|LLR| of the
+// codeblock, and 2Zc leading zeros for the punctured columns.  One RNG stream
+// per codeblock from (seed, index).  noise_power_db: n values; llr_out:
+// n*(N+2Zc) floats; bits_out (optional): n*B bits (payload + CRC), one per
+// byte.  0 or -2.
+extern "C" int32_t ldpc_mi355_synth_float_batch(const ldpc_in_config_t *cfg, int32_t n,
+                                                const float *noise_power_db, uint64_t seed,
+                                                int32_t threads, float *llr_out,
+                                                uint8_t *bits_out) {
+    Params p;
+    if (const char *why = derive(*cfg, p)) {
+        print_invalid(*cfg, why);
+        return -2;
+    }
+    if (n < 0 || (n > 0 && (!noise_power_db || !llr_out))) return -2;
+    const int nn = p.N + 2 * p.Zc, S = p.E / p.Qm;
+    const int f0 = p.B - 2 * p.Zc, f1 = p.K - 2 * p.Zc;   // filler span in the buffer
+    std::atomic<int> next{0};
+    auto worker = [&] {
+        std::vector<uint8_t> b(p.B), cw((size_t)nn), rm((size_t)p.E);
+        std::vector<uint8_t> bytes((size_t)p.A / 8 + 1);
+        std::vector<double> fe((size_t)p.E), buf((size_t)p.N);
+        for (int i; (i = next.fetch_add(1)) < n;) {
+            Rng rng(seed * 0x100000001B3ull + (uint64_t)i);
+            random_block(p, rng, b.data(), bytes);
+            encode_codeword(p, b.data(), cw.data());
+            rate_match_bits(p, cw.data(), rm.data());
+            const double nv = pow(10.0, (double)noise_power_db[i] / 10.0);
+            double amax = 0.0;
+            channel_llrs(rm.data(), (size_t)S, p.Qm, nv, rng, [&](size_t k, double v) {
+                const float f = (float)v;   // the demodulators return float32
+                fe[k] = f;
+                amax = fabs((double)f) > amax ? fabs((double)f) : amax;
+            });
+            std::fill(buf.begin(), buf.end(), 0.0);
+            int pos = p.k0;
+            for (int k = 0; k < p.E; ++k) {
+                while (pos >= f0 && pos < f1) pos = (pos + 1) % p.N;
+                buf[pos] += fe[(size_t)(k % S) * p.Qm + k / S];
+                pos = (pos + 1) % p.N;
+            }
+            const double fill = (double)(float)amax * 10.0;
+            for (int k = f0; k < f1; ++k) buf[k] = fill;
+            float *out = llr_out + (size_t)i * nn;
+            for (int k = 0; k < 2 * p.Zc; ++k) out[k] = 0.0f;
+            for (int k = 0; k < p.N; ++k) out[2 * p.Zc + k] = (float)buf[k];
+            if (bits_out) memcpy(bits_out + (size_t)i * p.B, b.data(), (size_t)p.B);
         }
     };
     if (threads < 1) threads = 1;

@@ -34,7 +34,7 @@
 #endif
 
 #include "ldpc_device.h"
-#include "nr_bg_tables.h"
+#include "ldpc_graph.h"
 
 // The FP32 BP kernel (algo 0) did not return on its first MI355X run and no
 // defect has been found in its source, so it is compiled out until one is:
@@ -44,16 +44,6 @@
 #endif
 
 namespace nrldpc {
-
-template <int BG> struct Graph;
-template <> struct Graph<1> {
-    static constexpr int kRows = NR_BG1_ROWS;
-    __host__ __device__ static constexpr int start(int r) { return kBG1RowStart[r]; }
-};
-template <> struct Graph<2> {
-    static constexpr int kRows = NR_BG2_ROWS;
-    __host__ __device__ static constexpr int start(int r) { return kBG2RowStart[r]; }
-};
 
 // ---------------------------------------------------------------------------
 // CRC (TS 38.212 5.1; c_lib/nr_crc.c:129-179): MSB-first, init 0.

@@ -145,6 +145,18 @@ int32_t ldpc_mi355_synth_batch(const ldpc_in_config_t *cfg, int32_t n, float noi
                                uint64_t seed, int32_t threads, int16_t *llr_out,
                                uint8_t *payload_out);
 
+/* Float inputs of the trainable decoder, in its N+2Zc belief layout
+ * (deep_learning/ldpc_training_data.py:24-82): random payload and CRC,
+ * encoding, rate matching, unquantized channel LLRs at noise_power_db[i]
+ * for codeblock i, then the float rate recovery of
+ * py5gphy/ldpc/nr_ldpc_raterecover.py:15-68 (repeats summed, filler
+ * positions 10 * max|LLR|, 2Zc leading zeros).  llr_out: n*(N+2Zc) floats;
+ * bits_out (may be NULL): n*B target bits (payload + CRC), one per byte.
+ * Noise streams are derived from (seed, index).  0, or -2. */
+int32_t ldpc_mi355_synth_float_batch(const ldpc_in_config_t *cfg, int32_t n,
+                                     const float *noise_power_db, uint64_t seed, int32_t threads,
+                                     float *llr_out, uint8_t *bits_out);
+
 /* One codeblock of a batch. */
 typedef struct {
     ldpc_in_config_t cfg;  /* HARQ_on: combine into the soft buffer slot */
@@ -207,6 +219,66 @@ int32_t ldpc_mi355_soft_slot_elems(void);
 
 /* Version string of this library. */
 const char *ldpc_mi355_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Unrolled trainable FP32 min-sum.  Replaces the forward of           */
+/* CTableNeuralLayeredMinSumDecoder (deep_learning/                    */
+/* deeplearning_ldpc_model.py:315-410) and the gradient torch autograd */
+/* takes through it, for one LDPC config and a batch of codeblocks.    */
+/* alpha/beta are the per-(iteration, base-graph row) values the row   */
+/* update uses (the reference's alpha()/beta() mapped through          */
+/* _alpha_for_row, :285-313); callers map their parameter sharing mode */
+/* onto this [iterations][rows] table.  All data pointers are device   */
+/* pointers; calls are asynchronous on `stream` (NULL = default        */
+/* stream) and allocate nothing.                                       */
+/* ------------------------------------------------------------------ */
+typedef struct ldpc_mi355_unrolled ldpc_mi355_unrolled_t;
+
+/* flags bit: the check update treats its inputs as constants
+ * (detach_check_update_inputs=True, deeplearning_ldpc_model.py:365). */
+#define LDPC_MI355_UNROLLED_DETACH 1
+
+/* Model for config `cfg` (gen_ldpc_parameters layout: n = N + 2Zc beliefs)
+ * unrolled over `iterations`; clip_llr > 0 clamps beliefs to
+ * [-clip_llr, clip_llr] like the reference's clip_llr (<= 0: no clamp).
+ * 0, -2 invalid config, -4 invalid argument, -1 GPU error. */
+int32_t ldpc_mi355_unrolled_create(ldpc_mi355_unrolled_t **handle, const ldpc_in_config_t *cfg,
+                                   int32_t iterations, float clip_llr, int32_t flags);
+void ldpc_mi355_unrolled_destroy(ldpc_mi355_unrolled_t *handle);
+
+/* dims[8] = n, rows, edges, Zc, B, iterations, codeblocks per workgroup,
+ * lanes per codeblock. */
+int32_t ldpc_mi355_unrolled_dims(const ldpc_mi355_unrolled_t *handle, int32_t *dims);
+
+/* Codeblocks per workgroup (0 = default, as many as fit 384 lanes and
+ * 64 KB of LDS).  -4 if k does not fit. */
+int32_t ldpc_mi355_unrolled_set_codeblocks_per_workgroup(ldpc_mi355_unrolled_t *handle,
+                                                         int32_t k);
+
+/* Device bytes of the forward record a backward needs, and of the backward
+ * scratch, for `batch` codeblocks. */
+uint64_t ldpc_mi355_unrolled_record_bytes(const ldpc_mi355_unrolled_t *handle, int32_t batch);
+uint64_t ldpc_mi355_unrolled_scratch_bytes(const ldpc_mi355_unrolled_t *handle, int32_t batch);
+
+/* llr: [batch][n] float beliefs in; lq_out: [batch][n] final beliefs;
+ * iter_lq_out (may be NULL): [iterations][batch][n] beliefs after each
+ * iteration; record (may be NULL for inference): record_bytes(batch),
+ * written for ldpc_mi355_unrolled_backward.  alpha/beta: [iterations][rows]. */
+int32_t ldpc_mi355_unrolled_forward(const ldpc_mi355_unrolled_t *handle, int32_t batch,
+                                    const float *llr, const float *alpha, const float *beta,
+                                    float *lq_out, float *iter_lq_out, void *record,
+                                    void *stream);
+
+/* Gradients of a loss L given dL/d lq_out (g_lq, [batch][n]) and, when the
+ * forward produced per-iteration beliefs, dL/d iter_lq_out (g_iter_lq, may be
+ * NULL).  Writes dL/d llr (g_llr, may be NULL) and dL/d alpha, dL/d beta
+ * ([iterations][rows] each, summed over the batch in a fixed order).  record
+ * is the forward's, for the same llr/alpha/beta; scratch: scratch_bytes(batch). */
+int32_t ldpc_mi355_unrolled_backward(const ldpc_mi355_unrolled_t *handle, int32_t batch,
+                                     const float *llr, const float *alpha, const float *beta,
+                                     const void *record, const float *g_lq,
+                                     const float *g_iter_lq, float *g_llr, float *g_alpha,
+                                     float *g_beta, void *scratch, void *stream);
 
 #ifdef __cplusplus
 }

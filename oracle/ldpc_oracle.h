@@ -121,6 +121,40 @@ double orc_decode_batch_i16(const int16_t *llr, uint8_t *hard, int32_t *status,
                             int E, int rv, int Qm, const orc_dec_t *d,
                             int threads);
 
+/* Unrolled float layered min-sum of the trainable decoder
+ * (deep_learning/deeplearning_ldpc_model.py:315-410) on one codeblock
+ * (unrolled_oracle.c).  llr: N+2Zc beliefs; alpha/beta: [iters][rows]
+ * values as used by the row update; clip > 0 clamps beliefs to
+ * [-clip, clip] (the reference's clip_llr), clip <= 0 disables it.
+ * lq_out: N+2Zc final beliefs; iter_out (may be NULL): [iters][N+2Zc]
+ * beliefs after each iteration; over (may be NULL, int[2]) counts input
+ * beliefs beyond +-clip and check updates with a minimum above clip (the
+ * cases where c_lib's FP32 min-sum, which clips minima and not inputs,
+ * computes differently).  The backward adds d loss / d alpha and
+ * d loss / d beta into g_alpha/g_beta ([iters][rows]) for the loss whose
+ * gradient is g_lq (final beliefs) plus g_iter (per-iteration beliefs, may
+ * be NULL), and writes d loss / d llr into g_llr (may be NULL).  detach = 1
+ * follows detach_check_update_inputs=True. */
+void orc_unrolled_forward_f32(const orc_code_t *c, int iters, const float *llr,
+                              const float *alpha, const float *beta,
+                              float clip, float *lq_out, float *iter_out,
+                              int *over);
+void orc_unrolled_forward_f64(const orc_code_t *c, int iters,
+                              const double *llr, const double *alpha,
+                              const double *beta, double clip, double *lq_out,
+                              double *iter_out, int *over);
+void orc_unrolled_backward_f32(const orc_code_t *c, int iters,
+                               const float *llr, const float *alpha,
+                               const float *beta, float clip, int detach,
+                               const float *g_lq, const float *g_iter,
+                               float *g_llr, double *g_alpha, double *g_beta);
+void orc_unrolled_backward_f64(const orc_code_t *c, int iters,
+                               const double *llr, const double *alpha,
+                               const double *beta, double clip, int detach,
+                               const double *g_lq, const double *g_iter,
+                               double *g_llr, double *g_alpha,
+                               double *g_beta);
+
 #ifdef __cplusplus
 }
 #endif

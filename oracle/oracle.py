@@ -70,6 +70,12 @@ def lib():
         L.orc_decode_batch_i16.argtypes = [P, P, P, P] + [ctypes.c_int] * 7 + [
             ctypes.POINTER(OrcDec), ctypes.c_int]
         L.orc_decode_batch_i16.restype = ctypes.c_double
+        for sfx, ft in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
+            getattr(L, "orc_unrolled_forward_" + sfx).argtypes = [
+                ctypes.POINTER(OrcCode), ctypes.c_int, P, P, P, ft, P, P, P]
+            getattr(L, "orc_unrolled_backward_" + sfx).argtypes = [
+                ctypes.POINTER(OrcCode), ctypes.c_int, P, P, P, ft, ctypes.c_int,
+                P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -265,3 +271,55 @@ def make_codeblock(bgn, A, crctype, E, rv, Qm, noise_power_db, rng):
     rm = rate_match(c, cw)
     llr = channel_llr(rm, Qm, noise_power_db, rng)
     return dict(code=c, payload=payload, bits=b, cw=cw, rm=rm, llr=llr)
+
+
+# --------------------------------------------------------------------------
+# Unrolled float min-sum of the trainable decoder
+# (deep_learning/deeplearning_ldpc_model.py:315-410) and its gradient.
+# --------------------------------------------------------------------------
+_UNROLLED_DT = {"f32": np.float32, "f64": np.float64}
+
+
+def unrolled_forward(c, iters, llr, alpha, beta, clip=None, precision="f32",
+                     want_iters=False, want_over=False):
+    """One codeblock.  llr (N+2Zc,), alpha/beta (iters, rows).
+    Returns (lq, iter_lq or None) [+ over counts when want_over]."""
+    dt = _UNROLLED_DT[precision]
+    n = c.N + 2 * c.Zc
+    llr = np.ascontiguousarray(llr, dtype=dt)
+    a = np.ascontiguousarray(alpha, dtype=dt).reshape(iters, c.rows)
+    b = np.ascontiguousarray(beta, dtype=dt).reshape(iters, c.rows)
+    lq = np.zeros(n, dt)
+    it = np.zeros((iters, n), dt) if want_iters else None
+    over = np.zeros(2, np.int32)
+    getattr(lib(), "orc_unrolled_forward_" + precision)(
+        ctypes.byref(c), iters, _p(llr), _p(a), _p(b), -1.0 if clip is None else clip,
+        _p(lq), _p(it) if it is not None else None, _p(over))
+    return (lq, it, over) if want_over else (lq, it)
+
+
+def unrolled_backward(c, iters, llr, alpha, beta, g_lq, g_iter=None, clip=None,
+                      detach=False, precision="f32"):
+    """One codeblock.  Returns (g_llr, g_alpha, g_beta); the alpha/beta
+    gradients are float64 (iters, rows)."""
+    dt = _UNROLLED_DT[precision]
+    n = c.N + 2 * c.Zc
+    llr = np.ascontiguousarray(llr, dtype=dt)
+    a = np.ascontiguousarray(alpha, dtype=dt).reshape(iters, c.rows)
+    b = np.ascontiguousarray(beta, dtype=dt).reshape(iters, c.rows)
+    g_lq = np.ascontiguousarray(g_lq, dtype=dt)
+    gi = None if g_iter is None else np.ascontiguousarray(g_iter, dtype=dt).reshape(iters, n)
+    g_llr = np.zeros(n, dt)
+    ga = np.zeros((iters, c.rows), np.float64)
+    gb = np.zeros((iters, c.rows), np.float64)
+    getattr(lib(), "orc_unrolled_backward_" + precision)(
+        ctypes.byref(c), iters, _p(llr), _p(a), _p(b), -1.0 if clip is None else clip,
+        1 if detach else 0, _p(g_lq), _p(gi) if gi is not None else None, _p(g_llr),
+        _p(ga), _p(gb))
+    return g_llr, ga, gb
+
+
+def row_degrees(c):
+    ro = np.ctypeslib.as_array(ctypes.cast(c.row_off, ctypes.POINTER(ctypes.c_uint16)),
+                               shape=(c.rows + 1,))
+    return np.diff(ro.astype(np.int64))

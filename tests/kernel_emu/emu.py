@@ -20,6 +20,13 @@ def lib():
         L.emu_decode_batch.restype = ctypes.c_int
         L.emu_decode_tiles.argtypes = [ctypes.c_int] + [P] * 12 + [ctypes.c_int]
         L.emu_decode_tiles.restype = ctypes.c_int
+        I, F = ctypes.c_int, ctypes.c_float
+        L.emu_unrolled_forward.argtypes = [I, I, I, I, I, F, I, I, I] + [P] * 6
+        L.emu_unrolled_forward.restype = I
+        L.emu_unrolled_record_bytes.argtypes = [I, I, I, I, I, F, I, I]
+        L.emu_unrolled_record_bytes.restype = ctypes.c_long
+        L.emu_unrolled_backward.argtypes = [I, I, I, I, I, F, I, I, I] + [P] * 9
+        L.emu_unrolled_backward.restype = I
         _lib = L
     return _lib
 
@@ -71,4 +78,39 @@ def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True,
         h = hard[hoff[i]:hoff[i] + p["B"] // 8]
         q = lq[loff[i]:loff[i] + p["N"] + 2 * p["Zc"]] if want_lq else None
         out.append(dict(status=int(st[i]), iters=int(it[i]), hard=h, lq=q))
+    return out
+
+
+def unrolled(cfg, T, llr, alpha, beta, clip=None, detach=False, cbs=0, g_lq=None, g_iter=None,
+             want_iters=False):
+    """Unrolled trainable min-sum kernels on the CPU.  cfg = (bgn, A, crctype, E);
+    llr (batch, n) float32; alpha/beta (T, rows).  Returns dict(lq, iter_lq,
+    and with g_lq: g_llr, g_alpha, g_beta)."""
+    bgn, A, crctype, E = cfg[:4]
+    llr = np.ascontiguousarray(llr, np.float32)
+    batch, n = llr.shape
+    a = np.ascontiguousarray(alpha, np.float32)
+    b = np.ascontiguousarray(beta, np.float32)
+    c = -1.0 if clip is None else float(clip)
+    L = lib()
+    nb = L.emu_unrolled_record_bytes(bgn, A, crctype, E, T, c, int(detach), batch)
+    assert nb >= 0
+    rec = np.zeros(max(nb // 4, 1), np.uint32)
+    lq = np.zeros((batch, n), np.float32)
+    it = np.zeros((T, batch, n), np.float32) if (want_iters or g_iter is not None) else None
+    rc = L.emu_unrolled_forward(bgn, A, crctype, E, T, c, int(detach), cbs, batch, _p(llr),
+                                _p(a), _p(b), _p(lq), _p(it), _p(rec))
+    assert rc == 0, rc
+    out = dict(lq=lq, iter_lq=it)
+    if g_lq is not None:
+        g = np.ascontiguousarray(g_lq, np.float32)
+        gi = None if g_iter is None else np.ascontiguousarray(g_iter, np.float32)
+        g_llr = np.zeros((batch, n), np.float32)
+        ga = np.zeros(a.shape, np.float32)
+        gb = np.zeros(b.shape, np.float32)
+        rc = L.emu_unrolled_backward(bgn, A, crctype, E, T, c, int(detach), cbs, batch, _p(llr),
+                                     _p(a), _p(b), _p(rec), _p(g), _p(gi), _p(g_llr), _p(ga),
+                                     _p(gb))
+        assert rc == 0, rc
+        out.update(g_llr=g_llr, g_alpha=ga, g_beta=gb)
     return out

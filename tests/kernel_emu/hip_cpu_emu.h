@@ -81,6 +81,15 @@ inline uint32_t __shfl_xor(uint32_t v, int mask, int width) {
     return r;
 }
 
+inline float __shfl_xor(float v, int mask, int width) {
+    uint32_t u;
+    std::memcpy(&u, &v, sizeof u);
+    u = __shfl_xor(u, mask, width);
+    float r;
+    std::memcpy(&r, &u, sizeof r);
+    return r;
+}
+
 // Runs fn once per lane of each workgroup in [0, grid).
 inline void emu_launch(unsigned grid, unsigned block, size_t lds_bytes,
                        const std::function<void()> &fn) {

@@ -162,3 +162,49 @@ def test_emu_packed_tiles_general_row(oracle_mod):
             s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
             assert (out["status"], out["iters"]) == (s, k), (row, cfg)
             assert np.array_equal(out["lq"], q), (row, cfg)
+
+
+# --------------------------------------------------------------------------
+# Unrolled trainable min-sum kernels (ldpc_unrolled.hip) against the oracle:
+# forward beliefs, per-iteration beliefs and dL/dllr bit for bit, the alpha
+# and beta sums (float partials, double reduction) to 1e-6 relative.
+# --------------------------------------------------------------------------
+UNROLLED_CASES = [
+    # cfg (bgn, A, crctype, E), iterations, codeblocks per workgroup (0 = default), batch,
+    # clip, detach
+    ((2, 40, 0, 200), 2, 0, 5, None, False),   # BG2 Zc=10: 16 lanes per codeblock, 24 packed
+    ((2, 40, 0, 200), 3, 1, 2, 5.0, False),    # one codeblock per workgroup
+    ((2, 40, 0, 200), 2, 2, 3, 5.0, True),     # detach
+    ((1, 104, 0, 390), 2, 3, 4, 5.0, False),   # BG1 Zc=6, last workgroup partly empty
+    ((1, 104, 0, 390), 2, 3, 4, None, True),
+    ((1, 1496, 2, 4560), 1, 0, 1, 5.0, False),  # BG1 Zc=72: two 64-lane segments
+]
+
+
+@pytest.mark.parametrize("cfg,T,cbs,batch,clip,detach", UNROLLED_CASES)
+def test_emu_unrolled_matches_oracle(oracle_mod, cfg, T, cbs, batch, clip, detach):
+    O = oracle_mod
+    c = O.code(*cfg)
+    n = c.N + 2 * c.Zc
+    rng = np.random.default_rng(T * 7 + batch)
+    llr = rng.normal(0.0, 3.0, (batch, n)).astype(np.float32)
+    llr[:, :2 * c.Zc] = 0.0
+    alpha = rng.uniform(0.55, 0.95, (T, c.rows)).astype(np.float32)
+    beta = rng.uniform(0.0, 0.3, (T, c.rows)).astype(np.float32)
+    g = rng.normal(size=(batch, n)).astype(np.float32)
+    gi = (rng.normal(size=(T, batch, n)) * 0.3).astype(np.float32)
+    r = emu.unrolled(cfg, T, llr, alpha, beta, clip=clip, detach=detach, cbs=cbs, g_lq=g,
+                     g_iter=gi)
+    ga = np.zeros((T, c.rows))
+    gb = np.zeros((T, c.rows))
+    for k in range(batch):
+        q, it = O.unrolled_forward(c, T, llr[k], alpha, beta, clip=clip, want_iters=True)
+        assert np.array_equal(q, r["lq"][k])
+        assert np.array_equal(it, r["iter_lq"][:, k])
+        gl, a_k, b_k = O.unrolled_backward(c, T, llr[k], alpha, beta, g[k], gi[:, k], clip=clip,
+                                           detach=detach)
+        assert np.array_equal(gl, r["g_llr"][k])
+        ga += a_k
+        gb += b_k
+    for mine, ref in ((r["g_alpha"], ga), (r["g_beta"], gb)):
+        assert np.abs(mine - ref).max() <= 1e-6 * max(np.abs(ref).max(), 1e-3)
