@@ -118,6 +118,124 @@ def bench_mixed(args):
     print(json.dumps(line))
 
 
+def bench_train(args):
+    """Training steps of the trainable decoder: BG1 Zc=96 (A=2088 + CRC24B,
+    E=6336, R=1/3, QPSK), 10 unrolled iterations, alpha and beta per
+    (iteration, row degree), clip_llr = 32767/128, batch 2048 with Es/N0 uniform
+    in [0, 2] dB per codeblock, BCE on -LLR of the B bits, float64 grad-norm
+    clip, Adam.  A step = forward + loss + backward + optimizer step over one
+    resident batch (a pool of 4 synthesized batches is cycled).  Beside it:
+    the same step through torch autograd of a torch restatement of the
+    reference model (tests/unrolled_torch_ref.py) on the same GPU."""
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.trainer import BG1_ZC96_WORKLOAD, _clip_grad_norm_float64
+    from deeplearning_5g_ldpc_decoder_amd.unrolled import (CTableNeuralLayeredMinSumDecoder,
+                                                           load_ldpc_table_config,
+                                                           synth_float_batch)
+    w = BG1_ZC96_WORKLOAD
+    n = args.batch if args.batch != BATCH else w["batch_size"]
+    tc = load_ldpc_table_config(w["bgn"], w["A"], w["crctype"], w["E"], w["rv"], w["Qm"])
+    clip = 32767.0 / 128.0
+    T = w["num_iterations"]
+    log(f"synthesizing 4 x {n} training codeblocks")
+    rng = np.random.default_rng(0)
+    pool = []
+    for k in range(4):
+        llr, bits = synth_float_batch(tc, rng.uniform(-2.0, 0.0, n), seed=k,
+                                      threads=min(16, os.cpu_count() or 8))
+        pool.append((torch.from_numpy(llr).cuda(), torch.from_numpy(bits.astype(np.float32)).cuda()))
+    model = CTableNeuralLayeredMinSumDecoder(
+        tc, num_iterations=T, alpha_mode=w["alpha_mode"], beta_mode=w["beta_mode"],
+        init_alpha=w["init_alpha"], init_beta=w["init_beta"], learnable_beta=True,
+        clip_llr=clip).cuda()
+    opt = torch.optim.Adam(model.parameters(), lr=w["lr"])
+    crit = torch.nn.BCEWithLogitsLoss()
+    B = tc.B
+    stream = torch.cuda.current_stream()
+
+    def step(k, ev=None):
+        x, tgt = pool[k % len(pool)]
+        opt.zero_grad(set_to_none=True)
+        if ev:
+            ev[0].record(stream)
+        _, q = model(x, return_llr=True)
+        loss = crit(-q[:, :B], tgt)
+        if ev:
+            ev[1].record(stream)
+        loss.backward()
+        if ev:
+            ev[2].record(stream)
+        _clip_grad_norm_float64(model.parameters(), 1.0)
+        opt.step()
+        return loss
+
+    log("warmup")
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    losses = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        losses.append(step(k, evs[k]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    bwd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    losses = [float(v) for v in losses]
+    ms = wall * 1e3 / args.steps
+    log(f"kernels: {ms:.3f} ms/step (forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms)")
+
+    ref = None
+    if not args.no_reference:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from unrolled_torch_ref import unrolled_forward
+        rmodel = CTableNeuralLayeredMinSumDecoder(
+            tc, num_iterations=T, alpha_mode=w["alpha_mode"], beta_mode=w["beta_mode"],
+            init_alpha=w["init_alpha"], init_beta=w["init_beta"], learnable_beta=True,
+            clip_llr=clip).cuda()
+        ropt = torch.optim.Adam(rmodel.parameters(), lr=w["lr"])
+
+        def rstep(k):
+            x, tgt = pool[k % len(pool)]
+            ropt.zero_grad(set_to_none=True)
+            q, _ = unrolled_forward(tc, x, rmodel.alpha_table(), rmodel.beta_table(), clip=clip)
+            loss = crit(-q[:, :B], tgt)
+            loss.backward()
+            _clip_grad_norm_float64(rmodel.parameters(), 1.0)
+            ropt.step()
+            return loss
+
+        log("reference-style torch autograd step (warmup 1, timed 2)")
+        rstep(0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(2):
+            rstep(k + 1)
+        torch.cuda.synchronize()
+        rms = (time.perf_counter() - t) * 1e3 / 2
+        ref = {"ms_per_step": rms, "codeblocks_per_s": n / rms * 1e3,
+               "peak_memory_GB": torch.cuda.max_memory_allocated() / 2 ** 30,
+               "what": "torch autograd through a torch restatement of "
+                       "deep_learning/deeplearning_ldpc_model.py:315-410 on the same GPU"}
+        log(f"reference-style: {rms:.1f} ms/step")
+    k = model.kernel()
+    line = {"metric": "training codeblocks/s (BG1 Zc=96, 10 unrolled iterations, batch 2048)",
+            "value": n / ms * 1e3, "unit": "codeblocks/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "train_bg1_zc96_minsum_f32_10it_batch2048",
+                       "global_batch": n, "iterations": T, "alpha_mode": w["alpha_mode"],
+                       "beta_mode": w["beta_mode"], "clip_llr": clip,
+                       "parallelism": "single-gpu"},
+            "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+            "record_MB": k.record_bytes(n) / 2 ** 20, "scratch_MB": k.scratch_bytes(n) / 2 ** 20,
+            "loss_first": losses[0], "loss_last": losses[-1],
+            "reference_autograd": ref,
+            "speedup_vs_reference_autograd": (ref["ms_per_step"] / ms) if ref else None}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,9 +247,12 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="batch launches only (for rocprofv3): no latency loop, no "
                          "per-codeblock path, no CPU leg")
-    ap.add_argument("--workload", default="bg1_zc384", choices=["bg1_zc384", "mixed"],
+    ap.add_argument("--workload", default="bg1_zc384", choices=["bg1_zc384", "mixed", "train"],
                     help="bg1_zc384: the headline workload (default); mixed: 8192 mixed-size "
-                         "codeblocks in one launch (use --batch to change)")
+                         "codeblocks in one launch (use --batch to change); train: training "
+                         "steps of the trainable decoder (BG1 Zc=96, batch 2048)")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="train workload: skip the torch-autograd reference-style step")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
@@ -145,6 +266,8 @@ def main():
         if args.batch == BATCH:
             args.batch = 8192
         return bench_mixed(args)
+    if args.workload == "train":
+        return bench_train(args)
 
     import torch
     from deeplearning_5g_ldpc_decoder_amd import _lib
