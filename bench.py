@@ -236,6 +236,61 @@ def bench_train(args):
     print(json.dumps(line))
 
 
+def bench_serve(args):
+    """Served decode loop: requests of 1-64 mixed-size codeblocks (base graph,
+    lifting size, rate 0.2-0.9, Qm 2-8, Es/N0 within 1 dB of each rate's
+    waterfall) arrive as a Poisson process from 16 UEs x 4 HARQ processes,
+    each with a 0.5 ms deadline; failed transmissions come back after 1 ms with
+    the next redundancy version (0, 2, 3, 1) and soft-combine into device
+    HARQ buffers.  Continuous batching over 3 launch contexts, admission by a
+    256 MB device LLR pool.  First the capacity (requests/s with every request
+    queued at once), then offered loads of 0.2-1.0 x capacity."""
+    from deeplearning_5g_ldpc_decoder_amd.serving import (DevicePool, GpuExecutor, HarqSlots,
+                                                          RequestPool, Server, poisson_arrivals)
+    log("synthesizing 128 request templates (4 redundancy versions each)")
+    pool = RequestPool(128, seed=11, max_cbs=64, threads=min(16, os.cpu_count() or 8))
+    harq = HarqSlots(16, 4, 64)
+    llr_pool = DevicePool(256 << 20)
+    ex = GpuExecutor(pool, harq, llr_pool, n_contexts=3, max_batch_codeblocks=args.serve_batch)
+    srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=1e-3)
+    rng = np.random.default_rng(5)
+
+    def run(arr):
+        n = len(arr)
+        return srv.run(arr, rng.integers(0, len(pool), n), ues=rng.integers(0, 16, n))
+
+    run(np.zeros(64))          # warmup
+    nsat = 600
+    st = run(np.zeros(nsat))
+    sat = st.summary(0.5e-3)
+    cap = sum(1 for r in st.requests if r.rv_index == 0) / st.wall_s
+    log(f"capacity {cap:.0f} requests/s, {sat['info_gbps']:.2f} Gbps saturated")
+    sweep = []
+    for f in (0.2, 0.4, 0.6, 0.8, 1.0):
+        rate = f * cap
+        dur = min(1.5, 1500.0 / rate)
+        s = run(poisson_arrivals(rate, dur, seed=int(f * 10))).summary(0.5e-3)
+        s.update(load=f, offered_requests_per_s=rate, duration_s=dur)
+        log(f"load {f:.1f}: p50 {s['latency_ms']['p50']:.3f} ms p99 {s['latency_ms']['p99']:.3f} "
+            f"ms, miss {s['deadline_miss_rate']:.3f}, {s['info_gbps']:.2f} Gbps")
+        sweep.append(s)
+    ex.close()
+    mid = sweep[2]
+    line = {"metric": "served decode p99 latency at 0.6x capacity (mixed sizes, HARQ, 0.5 ms "
+                      "deadline)",
+            "value": mid["latency_ms"]["p99"], "unit": "ms", "n_gpus": 1,
+            "steps": sum(s["requests"] for s in sweep), "warmup": 64,
+            "ms_per_step": mid["latency_ms"]["mean"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+            "config": {"workload": "serve_mixed_1to64cb_harq16x4_poisson",
+                       "decoder_row": ROW, "contexts": 3,
+                       "max_batch_codeblocks": args.serve_batch,
+                       "llr_pool_MB": 256, "harq_slots": 16 * 4 * 64,
+                       "parallelism": "single-gpu"},
+            "capacity_requests_per_s": cap, "saturated": sat, "sweep": sweep}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,10 +302,14 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="batch launches only (for rocprofv3): no latency loop, no "
                          "per-codeblock path, no CPU leg")
-    ap.add_argument("--workload", default="bg1_zc384", choices=["bg1_zc384", "mixed", "train"],
+    ap.add_argument("--workload", default="bg1_zc384",
+                    choices=["bg1_zc384", "mixed", "train", "serve"],
                     help="bg1_zc384: the headline workload (default); mixed: 8192 mixed-size "
                          "codeblocks in one launch (use --batch to change); train: training "
-                         "steps of the trainable decoder (BG1 Zc=96, batch 2048)")
+                         "steps of the trainable decoder (BG1 Zc=96, batch 2048); serve: the "
+                         "served loop (Poisson arrivals, HARQ, continuous batching)")
+    ap.add_argument("--serve-batch", type=int, default=256,
+                    help="serve workload: codeblocks per batch at most")
     ap.add_argument("--no-reference", action="store_true",
                     help="train workload: skip the torch-autograd reference-style step")
     args = ap.parse_args()
@@ -268,6 +327,8 @@ def main():
         return bench_mixed(args)
     if args.workload == "train":
         return bench_train(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
     from deeplearning_5g_ldpc_decoder_amd import _lib

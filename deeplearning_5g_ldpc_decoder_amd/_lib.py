@@ -24,6 +24,8 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_unrolled_set_codeblocks_per_workgroup", "ldpc_mi355_unrolled_record_bytes",
     "ldpc_mi355_unrolled_scratch_bytes", "ldpc_mi355_unrolled_forward",
     "ldpc_mi355_unrolled_backward",
+    "ldpc_mi355_batch_create", "ldpc_mi355_batch_decode", "ldpc_mi355_batch_layout",
+    "ldpc_mi355_batch_query", "ldpc_mi355_batch_wait", "ldpc_mi355_batch_destroy",
 )
 EXPORTED_DATA = (
     "BG1_HBG_offset_array", "BG1_HBG_array", "BG1_HBG_Vij_array",
@@ -155,6 +157,20 @@ def load():
     L.ldpc_mi355_unrolled_forward.restype = ctypes.c_int32
     L.ldpc_mi355_unrolled_backward.argtypes = [P, ctypes.c_int32] + [P] * 11
     L.ldpc_mi355_unrolled_backward.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int32,
+                                          ctypes.POINTER(DecoderConfig), ctypes.c_uint32]
+    L.ldpc_mi355_batch_create.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_decode.argtypes = [P, ctypes.POINTER(CbDescriptor), ctypes.c_int32] + \
+        [P] * 8
+    L.ldpc_mi355_batch_decode.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_layout.argtypes = [P] * 5
+    L.ldpc_mi355_batch_layout.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_query.argtypes = [P]
+    L.ldpc_mi355_batch_query.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_wait.argtypes = [P]
+    L.ldpc_mi355_batch_wait.restype = ctypes.c_int32
+    L.ldpc_mi355_batch_destroy.argtypes = [P]
+    L.ldpc_mi355_batch_destroy.restype = None
     _lib = L
     return L
 

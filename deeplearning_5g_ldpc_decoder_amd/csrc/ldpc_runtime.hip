@@ -248,6 +248,64 @@ extern "C" int32_t ldpc_mi355_plan_tiles(const ldpc_mi355_plan_t *plan) {
     return plan ? plan->ntiles : -1;
 }
 
+namespace {
+
+// Descriptors of a batch as the kernels read them, laid out back to back in
+// the hard / posterior outputs.  Returns 0 or -2 (config of codeblock
+// *bad_index is invalid).
+struct BatchLayout {
+    std::vector<CbDesc> descs;
+    std::vector<uint32_t> hard_off, lq_off;
+    uint64_t hard_bytes = 0, lq_elems = 0, work = 0;
+    int max_zc = 2, max_soft = 0;
+};
+
+int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, const Runtime &r,
+                 BatchLayout &L, int32_t *bad_index) {
+    L.descs.resize((size_t)n);
+    L.hard_off.resize((size_t)n);
+    L.lq_off.resize((size_t)n);
+    L.hard_bytes = L.lq_elems = L.work = 0;
+    L.max_zc = 2;
+    L.max_soft = 0;
+    for (int i = 0; i < n; ++i) {
+        Params p;
+        if (const char *why = derive(cbs[i].cfg, p)) {
+            print_invalid(cbs[i].cfg, why);
+            if (bad_index) *bad_index = i;
+            return -2;
+        }
+        CbDesc d = make_desc(p, r);
+        d.llr_off = cbs[i].llr_offset;
+        d.hard_off = (uint32_t)L.hard_bytes;
+        d.soft_off = cbs[i].soft_slot >= 0 ? (uint32_t)cbs[i].soft_slot * (uint32_t)kMaxSoft
+                                           : kNone;
+        d.lq_off = want_lq ? (uint32_t)L.lq_elems : kNone;
+        d.work_off = (uint32_t)L.work;
+        L.hard_off[i] = d.hard_off;
+        L.lq_off[i] = want_lq ? (uint32_t)L.lq_elems : 0u;
+        L.hard_bytes += (uint64_t)p.B / 8;
+        L.lq_elems += (uint64_t)(p.N + 2 * p.Zc);
+        if (algo == 0) L.work += (uint64_t)p.edges * p.Zc;
+        if (p.Zc > L.max_zc) L.max_zc = p.Zc;
+        if (p.N + 2 * p.Zc > L.max_soft) L.max_soft = p.N + 2 * p.Zc;
+        L.descs[i] = d;
+    }
+    return 0;
+}
+
+// Packed tiles for algo 1 when asked (flags bit 1) or when some lifting size
+// leaves room for two or more codeblocks per workgroup (bit 2 forbids it).
+bool want_packing(int algo, uint32_t flags, const std::vector<CbDesc> &descs) {
+    if (algo != 1 || (flags & 4u)) return false;
+    if (flags & 2u) return true;
+    for (const CbDesc &d : descs)
+        if (tile_codeblocks(d.bgn, d.Zc) > 1) return true;
+    return false;
+}
+
+}  // namespace
+
 extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs,
                                           int32_t n, const ldpc_decoder_cfg_t *dec,
                                           uint32_t flags, int32_t *bad_index) {
@@ -266,63 +324,29 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     pl->algo = dec->algo;
     pl->dp = make_params(*dec);
     pl->want_lq = (flags & 1u) != 0;
-    std::vector<CbDesc> descs((size_t)n);
-    pl->hard_off.resize(n);
-    pl->lq_off.resize(n);
-    uint64_t work = 0;
-    int max_zc = 2;
-    for (int i = 0; i < n; ++i) {
-        Params p;
-        if (const char *why = derive(cbs[i].cfg, p)) {
-            print_invalid(cbs[i].cfg, why);
-            if (bad_index) *bad_index = i;
-            delete pl;
-            return -2;
-        }
-        CbDesc d = make_desc(p, r);
-        d.llr_off = cbs[i].llr_offset;
-        d.hard_off = (uint32_t)pl->hard_bytes;
-        d.soft_off = cbs[i].soft_slot >= 0 ? (uint32_t)cbs[i].soft_slot * (uint32_t)kMaxSoft
-                                           : kNone;
-        d.lq_off = pl->want_lq ? (uint32_t)pl->lq_elems : kNone;
-        d.work_off = (uint32_t)work;
-        pl->hard_off[i] = d.hard_off;
-        pl->lq_off[i] = pl->want_lq ? (uint32_t)pl->lq_elems : 0u;
-        pl->hard_bytes += (uint64_t)p.B / 8;
-        pl->lq_elems += (uint64_t)(p.N + 2 * p.Zc);
-        if (pl->algo == 0) work += (uint64_t)p.edges * p.Zc;
-        if (p.Zc > max_zc) max_zc = p.Zc;
-        if (p.N + 2 * p.Zc > pl->max_soft) pl->max_soft = p.N + 2 * p.Zc;
-        descs[i] = d;
+    BatchLayout L;
+    if (int st = build_layout(cbs, n, pl->algo, pl->want_lq, r, L, bad_index)) {
+        delete pl;
+        return st;
     }
-    pl->threads = threads_for(max_zc);
+    pl->hard_off = L.hard_off;
+    pl->lq_off = L.lq_off;
+    pl->hard_bytes = L.hard_bytes;
+    pl->lq_elems = L.lq_elems;
+    pl->max_soft = L.max_soft;
+    pl->threads = threads_for(L.max_zc);
     if (hipMalloc(&pl->d_desc, sizeof(CbDesc) * (size_t)n) != hipSuccess ||
-        hipMemcpy(pl->d_desc, descs.data(), sizeof(CbDesc) * (size_t)n,
+        hipMemcpy(pl->d_desc, L.descs.data(), sizeof(CbDesc) * (size_t)n,
                   hipMemcpyHostToDevice) != hipSuccess ||
-        (work && hipMalloc(&pl->d_work, work * sizeof(float)) != hipSuccess)) {
+        (L.work && hipMalloc(&pl->d_work, L.work * sizeof(float)) != hipSuccess)) {
         fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
         ldpc_mi355_plan_destroy(pl);
         return -1;
     }
-    // Packed tiles for algo 1 when asked (flags bit 1) or when some lifting
-    // size leaves room for two or more codeblocks per workgroup (bit 2
-    // forbids packing).
-    bool pack = false;
-    if (pl->algo == 1 && !(flags & 4u)) {
-        if (flags & 2u) {
-            pack = true;
-        } else {
-            for (const CbDesc &d : descs)
-                if (tile_codeblocks(d.bgn, d.Zc) > 1) {
-                    pack = true;
-                    break;
-                }
-        }
-    }
-    if (pack) {
+    if (want_packing(pl->algo, flags, L.descs)) {
         std::vector<TileDesc> tiles;
         std::vector<uint32_t> order;
-        build_tiles(descs, r.etab_off, tile_codeblocks, tiles, order, pl->tile_lds);
+        build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, pl->tile_lds);
         pl->ntiles = (int)tiles.size();
         if (hipMalloc(&pl->d_tiles, sizeof(TileDesc) * tiles.size()) != hipSuccess ||
             hipMemcpy(pl->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
@@ -387,6 +411,163 @@ extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
     if (plan->d_tiles) (void)hipFree(plan->d_tiles);
     if (plan->d_order) (void)hipFree(plan->d_order);
     delete plan;
+}
+
+// ---------------------------------------------------------------------------
+// Reusable batch contexts (continuous batching): descriptors and tiles are
+// built on the host into the context's pinned staging buffer, copied to its
+// device buffer with one asynchronous copy on the caller's stream, and the
+// decode is launched behind it -- no allocation and no synchronization per
+// call.  An event recorded after the launch tells when the context (and its
+// staging buffer) may be reused; a decode on a context whose previous decode
+// is still running first waits for it.
+// ---------------------------------------------------------------------------
+struct ldpc_mi355_batch {
+    int capacity = 0, algo = 1;
+    uint32_t flags = 0;
+    bool want_lq = false;
+    DecParams dp;
+    size_t stage_bytes = 0;
+    unsigned char *h_stage = nullptr, *d_stage = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    BatchLayout layout;
+    int n = 0, ntiles = 0;
+};
+
+namespace {
+size_t batch_stage_bytes(int capacity) {
+    const size_t d = ((sizeof(CbDesc) * (size_t)capacity) + 255) & ~(size_t)255;
+    const size_t t = ((sizeof(TileDesc) * (size_t)capacity) + 255) & ~(size_t)255;
+    return d + t + sizeof(uint32_t) * (size_t)capacity;
+}
+}  // namespace
+
+extern "C" int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t capacity,
+                                           const ldpc_decoder_cfg_t *dec, uint32_t flags) {
+    if (!batch || !dec) return -4;
+    *batch = nullptr;
+    if (capacity <= 0) return -4;
+    if (!algo_supported(dec->algo)) return -3;
+    if (dec->algo == 0) return -3;   // BP keeps per-codeblock message buffers: plans only
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    auto *b = new ldpc_mi355_batch;
+    b->capacity = capacity;
+    b->algo = dec->algo;
+    b->flags = flags;
+    b->want_lq = (flags & 1u) != 0;
+    b->dp = make_params(*dec);
+    b->stage_bytes = batch_stage_bytes(capacity);
+    if (hipHostMalloc(&b->h_stage, b->stage_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&b->d_stage, b->stage_bytes) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        fprintf(stderr, "libnrldpc_mi355: batch context allocation failed\n");
+        ldpc_mi355_batch_destroy(b);
+        return -1;
+    }
+    *batch = b;
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi355_cb_t *cbs,
+                                           int32_t n, const int16_t *llr, int16_t *soft_pool,
+                                           uint8_t *hard, int32_t *status, int32_t *iterations,
+                                           void *lq, void *stream, int32_t *bad_index) {
+    if (bad_index) *bad_index = -1;
+    if (!b || !cbs || n <= 0) return -4;
+    if (n > b->capacity) return -4;
+    if (b->want_lq && !lq) return -4;
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    if (b->pending) {
+        HIP_OK(hipEventSynchronize(b->done));
+        b->pending = false;
+    }
+    BatchLayout &L = b->layout;
+    b->n = 0;
+    b->ntiles = 0;
+    if (int st = build_layout(cbs, n, b->algo, b->want_lq, r, L, bad_index)) return st;
+    const size_t doff = 0;
+    const size_t toff = ((sizeof(CbDesc) * (size_t)b->capacity) + 255) & ~(size_t)255;
+    const size_t ooff = toff + (((sizeof(TileDesc) * (size_t)b->capacity) + 255) & ~(size_t)255);
+    memcpy(b->h_stage + doff, L.descs.data(), sizeof(CbDesc) * (size_t)n);
+    size_t bytes = sizeof(CbDesc) * (size_t)n;
+    int tile_lds = 0;
+    if (want_packing(b->algo, b->flags, L.descs)) {
+        std::vector<TileDesc> tiles;
+        std::vector<uint32_t> order;
+        build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, tile_lds);
+        b->ntiles = (int)tiles.size();
+        memcpy(b->h_stage + toff, tiles.data(), sizeof(TileDesc) * tiles.size());
+        memcpy(b->h_stage + ooff, order.data(), sizeof(uint32_t) * order.size());
+        bytes = ooff + sizeof(uint32_t) * order.size();
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIP_OK(hipMemcpyAsync(b->d_stage, b->h_stage, bytes, hipMemcpyHostToDevice, s));
+    DecOut o;
+    o.desc = reinterpret_cast<const CbDesc *>(b->d_stage + doff);
+    o.llr = llr;
+    o.soft = soft_pool;
+    o.hard = hard;
+    o.status = status;
+    o.iters = iterations;
+    o.lq = lq;
+    o.work = nullptr;
+    o.etab = r.d_etab;
+    if (b->ntiles > 0)
+        HIP_OK(nrldpc_launch_tiles(&o, reinterpret_cast<const TileDesc *>(b->d_stage + toff),
+                                   reinterpret_cast<const uint32_t *>(b->d_stage + ooff),
+                                   &b->dp, b->ntiles, tile_lds, s));
+    else
+        HIP_OK(nrldpc_launch_decode(b->algo, &o, &b->dp, n, threads_for(L.max_zc), L.max_soft,
+                                    s));
+    HIP_OK(hipEventRecord(b->done, s));
+    b->pending = true;
+    b->n = n;
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_batch_layout(const ldpc_mi355_batch_t *b, uint64_t *hard_bytes,
+                                           uint64_t *lq_elems, uint32_t *hard_offsets,
+                                           uint32_t *lq_offsets) {
+    if (!b) return -4;
+    if (hard_bytes) *hard_bytes = b->layout.hard_bytes;
+    if (lq_elems) *lq_elems = b->want_lq ? b->layout.lq_elems : 0;
+    if (hard_offsets) memcpy(hard_offsets, b->layout.hard_off.data(), 4 * (size_t)b->n);
+    if (lq_offsets) memcpy(lq_offsets, b->layout.lq_off.data(), 4 * (size_t)b->n);
+    return b->n;
+}
+
+extern "C" int32_t ldpc_mi355_batch_query(ldpc_mi355_batch_t *b) {
+    if (!b) return -4;
+    if (!b->pending) return 1;
+    const hipError_t e = hipEventQuery(b->done);
+    if (e == hipSuccess) {
+        b->pending = false;
+        return 1;
+    }
+    if (e == hipErrorNotReady) return 0;
+    fprintf(stderr, "libnrldpc_mi355: batch query failed: %s\n", hipGetErrorString(e));
+    return -1;
+}
+
+extern "C" int32_t ldpc_mi355_batch_wait(ldpc_mi355_batch_t *b) {
+    if (!b) return -4;
+    if (b->pending) {
+        HIP_OK(hipEventSynchronize(b->done));
+        b->pending = false;
+    }
+    return 0;
+}
+
+extern "C" void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *b) {
+    if (!b) return;
+    if (b->pending) (void)hipEventSynchronize(b->done);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->d_stage) (void)hipFree(b->d_stage);
+    delete b;
 }
 
 extern "C" int32_t ldpc_mi355_sync(void) {

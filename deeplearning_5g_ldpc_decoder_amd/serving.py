@@ -1,0 +1,503 @@
+"""Served decode loop on the MI355X: continuous batching, admission by free
+device memory, HARQ soft buffers resident on the device.
+
+Requests are slots of 1-64 codeblocks of mixed base graph / lifting size /
+rate / modulation, each from one UE's HARQ process and carrying a deadline.
+The loop (``Server.run``) runs in real time:
+
+* arrivals join a FIFO queue at their arrival instant (Poisson process);
+* whenever a launch context is free, the queue head is admitted as one batch
+  for as long as the batch stays within ``max_batch_codeblocks`` and its
+  LLRs fit in the device LLR pool (``DevicePool``, first-fit over one HBM
+  allocation): admission is by free device memory, and requests that do not
+  fit wait until a running batch frees its region;
+* each batch gets its own launch context (``ldpc_mi355_batch_*``) and HIP
+  stream: its LLRs are copied host->device, decoded (int16 layered
+  min-sum, CRC early stop) and its results copied back, all asynchronously,
+  so batches admitted while others run overlap on the device;
+* completions are polled; a codeblock that fails CRC is retransmitted with
+  the next redundancy version of the sequence 0, 2, 3, 1 after
+  ``harq_rtt_s``, soft-combined on the device into the same HARQ slot
+  (HARQ_on = 1); slot = (UE, process, codeblock index), ``ldpc_mi355``
+  soft-buffer slots of 68*384 int16 values.
+
+Results of a request do not depend on what it was batched with: every
+codeblock's decode reads only its own LLRs and HARQ slot (tests check this
+against each request decoded alone).
+"""
+import bisect
+import ctypes
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import CbDescriptor, DecoderConfig, LdpcInConfig, ptr
+
+RV_SEQUENCE = (0, 2, 3, 1)
+DEFAULT_ROW = (1, 8, 22936, 0, 1280, 32767, -32767)
+
+
+# --------------------------------------------------------------------------
+# Request material
+# --------------------------------------------------------------------------
+class RequestPool:
+    """Pre-synthesized request templates (host side).  Template k holds
+    n_k codeblocks (1..max_cbs) drawn like the mixed-size workload (base graph,
+    lifting size, rate, Qm, per-codeblock Es/N0 around the rate's waterfall
+    + ``snr_offset_db``), their payloads, and the channel LLRs of each
+    redundancy version in RV_SEQUENCE (same payload and noise seed per rv)."""
+
+    def __init__(self, n_templates, seed=0, max_cbs=64, snr_offset_db=0.0, threads=16,
+                 pinned=True):
+        from concurrent.futures import ThreadPoolExecutor
+
+        from .batch import synth_batch
+        from .workloads import draw_config
+        rng = np.random.default_rng(seed)
+        self.cfgs, self.esn0, self.sizes = [], [], []
+        for _ in range(n_templates):
+            k = int(rng.integers(1, max_cbs + 1))
+            draws = [draw_config(rng) for _ in range(k)]
+            self.cfgs.append([d[0] for d in draws])
+            self.esn0.append([d[1] + snr_offset_db for d in draws])
+            self.sizes.append(k)
+        jobs = [(t, c, rvi) for t in range(n_templates) for c in range(self.sizes[t])
+                for rvi in range(len(RV_SEQUENCE))]
+
+        def one(job):
+            t, c, rvi = job
+            cfg = list(self.cfgs[t][c])
+            cfg[4] = RV_SEQUENCE[rvi]
+            llr, pay = synth_batch(tuple(cfg), 1, -self.esn0[t][c],
+                                   seed=(seed * 7919 + t) * 131 + c, threads=1)
+            return llr[0], pay[0]
+
+        with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+            out = list(ex.map(one, jobs))
+        # one flat int16 array: template t, rv index r -> its codeblocks' LLRs
+        self.offsets = {}
+        total = sum(len(o[0]) for o in out)
+        flat = np.empty(total, np.int16)
+        self.payloads = [[None] * self.sizes[t] for t in range(n_templates)]
+        pos = 0
+        it = iter(zip(jobs, out))
+        for (t, c, rvi), (llr, pay) in it:
+            self.offsets.setdefault((t, rvi), []).append((pos, len(llr)))
+            flat[pos:pos + len(llr)] = llr
+            pos += len(llr)
+            if rvi == 0:
+                self.payloads[t][c] = pay
+        self.flat = flat
+        self.pinned = None
+        if pinned:
+            import torch
+            self.pinned = torch.from_numpy(flat).pin_memory()
+            self.flat = self.pinned.numpy()
+        self.info_bits = [sum(c[1] for c in cfgs) for cfgs in self.cfgs]
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def llr(self, t, rvi):
+        """Concatenated LLRs of template t's codeblocks for rv index rvi."""
+        parts = self.offsets[(t, rvi)]
+        a, b = parts[0][0], parts[-1][0] + parts[-1][1]
+        return self.flat[a:b], [p[1] for p in parts]
+
+
+@dataclass
+class Request:
+    rid: int
+    template: int
+    ue: int
+    proc: int
+    rv_index: int
+    arrival: float
+    deadline: float
+    n_cbs: int = 0
+    info_bits: int = 0
+    admitted: float = -1.0
+    done: float = -1.0
+    batch_id: int = -1
+    status: Optional[np.ndarray] = None
+    iters: Optional[np.ndarray] = None
+    hard: Optional[List[np.ndarray]] = None
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+    @property
+    def ok(self):
+        return self.status is not None and bool(np.all(self.status == 0))
+
+
+# --------------------------------------------------------------------------
+# Device memory: LLR pool (admission) and HARQ slots
+# --------------------------------------------------------------------------
+class DevicePool:
+    """First-fit allocator over one device int16 tensor of ``nbytes``: the
+    admission budget.  Offsets are in int16 elements, 128-element aligned."""
+
+    ALIGN = 128
+
+    def __init__(self, nbytes, device="cuda", tensor=None):
+        self.elems = int(nbytes) // 2 // self.ALIGN * self.ALIGN
+        if tensor is None:
+            import torch
+            tensor = torch.empty(self.elems, dtype=torch.int16, device=device)
+        self.tensor = tensor
+        self.free = [(0, self.elems)]   # sorted (offset, length)
+        self.used = 0
+        self.peak = 0
+
+    def alloc(self, elems):
+        need = (int(elems) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= need:
+                if ln == need:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + need, ln - need)
+                self.used += need
+                self.peak = max(self.peak, self.used)
+                return off, need
+        return None
+
+    def release(self, off, need):
+        i = bisect.bisect_left(self.free, (off, 0))
+        self.free.insert(i, (off, need))
+        # coalesce with neighbours
+        if i + 1 < len(self.free) and self.free[i][0] + self.free[i][1] == self.free[i + 1][0]:
+            self.free[i] = (self.free[i][0], self.free[i][1] + self.free[i + 1][1])
+            self.free.pop(i + 1)
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == self.free[i][0]:
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            self.free.pop(i)
+        self.used -= need
+
+    @property
+    def free_bytes(self):
+        return 2 * (self.elems - self.used)
+
+
+class HarqSlots:
+    """Device int16 soft buffers, one slot per (UE, HARQ process, codeblock)."""
+
+    def __init__(self, n_ues=16, n_procs=4, max_cbs=64, device="cuda"):
+        import torch
+        self.n_ues, self.n_procs, self.max_cbs = n_ues, n_procs, max_cbs
+        self.slot_elems = int(_lib.load().ldpc_mi355_soft_slot_elems())
+        self.tensor = torch.zeros(n_ues * n_procs * max_cbs * self.slot_elems, dtype=torch.int16,
+                                  device=device)
+
+    def slot(self, ue, proc, cb):
+        return (ue * self.n_procs + proc) * self.max_cbs + cb
+
+    @property
+    def nbytes(self):
+        return self.tensor.numel() * 2
+
+
+# --------------------------------------------------------------------------
+# GPU executor: launch contexts with their own streams
+# --------------------------------------------------------------------------
+class _Context:
+    def __init__(self, idx, cap_cbs, row, device):
+        import torch
+        L = _lib.load()
+        self.idx = idx
+        self.handle = ctypes.c_void_p()
+        st = L.ldpc_mi355_batch_create(ctypes.byref(self.handle), cap_cbs,
+                                       ctypes.byref(DecoderConfig(*row)), 0)
+        if st != 0:
+            raise RuntimeError(f"ldpc_mi355_batch_create failed ({st})")
+        self.stream = torch.cuda.Stream(device=device)    # default priority
+        self.cap = cap_cbs
+        self.status = torch.empty(cap_cbs, dtype=torch.int32, device=device)
+        self.iters = torch.empty(cap_cbs, dtype=torch.int32, device=device)
+        self.hard = torch.empty(cap_cbs * 1056, dtype=torch.uint8, device=device)
+        self.h_status = torch.empty(cap_cbs, dtype=torch.int32).pin_memory()
+        self.h_iters = torch.empty(cap_cbs, dtype=torch.int32).pin_memory()
+        self.h_hard = torch.empty(cap_cbs * 1056, dtype=torch.uint8).pin_memory()
+        self.descs = (CbDescriptor * cap_cbs)()
+        self.done_event = torch.cuda.Event()
+        self.busy = False
+        self.requests = []
+        self.region = None
+        self.hard_off = np.zeros(cap_cbs, np.uint32)
+        self.n = 0
+
+    def close(self):
+        if self.handle.value:
+            _lib.load().ldpc_mi355_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class GpuExecutor:
+    """Runs batches on ``n_contexts`` launch contexts (one HIP stream each)."""
+
+    def __init__(self, pool: RequestPool, harq: HarqSlots, llr_pool: DevicePool,
+                 row=DEFAULT_ROW, n_contexts=3, max_batch_codeblocks=256, device="cuda"):
+        import torch
+        self.torch = torch
+        self.pool = pool
+        self.harq = harq
+        self.llr_pool = llr_pool
+        self.row = tuple(int(v) for v in row)
+        self.ctxs = [_Context(i, max_batch_codeblocks, self.row, device)
+                     for i in range(n_contexts)]
+        self.max_cbs = max_batch_codeblocks
+        self._stage = torch.empty(llr_pool.elems, dtype=torch.int16).pin_memory()
+        self._stage_np = self._stage.numpy()
+        self.batches = 0
+
+    def free_context(self):
+        for c in self.ctxs:
+            if not c.busy:
+                return c
+        return None
+
+    def launch(self, ctx, reqs, region):
+        """Copy the batch's LLRs into its pool region, decode, copy results
+        back -- all queued on the context's stream."""
+        torch = self.torch
+        off, need = region
+        n = 0
+        pos = 0
+        for r in reqs:
+            llr, lens = self.pool.llr(r.template, r.rv_index)
+            self._stage_np[off + pos:off + pos + len(llr)] = llr
+            cfgs = self.pool.cfgs[r.template]
+            for c, (cfg, ln) in enumerate(zip(cfgs, lens)):
+                d = ctx.descs[n]
+                d.cfg = LdpcInConfig(cfg[0], cfg[1], cfg[2], cfg[3], RV_SEQUENCE[r.rv_index],
+                                     cfg[5], 1 if r.rv_index > 0 else 0)
+                d.llr_offset = off + pos
+                d.soft_slot = self.harq.slot(r.ue, r.proc, c)
+                pos += ln
+                n += 1
+        assert n <= ctx.cap and pos <= need
+        stream = ctx.stream
+        with torch.cuda.stream(stream):
+            self.llr_pool.tensor[off:off + pos].copy_(self._stage[off:off + pos],
+                                                      non_blocking=True)
+        bad = ctypes.c_int32(-1)
+        st = _lib.load().ldpc_mi355_batch_decode(
+            ctx.handle, ctx.descs, n, ptr(self.llr_pool.tensor), ptr(self.harq.tensor),
+            ptr(ctx.hard), ptr(ctx.status), ptr(ctx.iters), None,
+            ctypes.c_void_p(stream.cuda_stream), ctypes.byref(bad))
+        if st != 0:
+            raise RuntimeError(f"ldpc_mi355_batch_decode failed ({st}), codeblock {bad.value}")
+        hb = ctypes.c_uint64()
+        _lib.load().ldpc_mi355_batch_layout(ctx.handle, ctypes.byref(hb), None,
+                                            ptr(ctx.hard_off), None)
+        with torch.cuda.stream(stream):
+            ctx.h_status[:n].copy_(ctx.status[:n], non_blocking=True)
+            ctx.h_iters[:n].copy_(ctx.iters[:n], non_blocking=True)
+            ctx.h_hard[:hb.value].copy_(ctx.hard[:hb.value], non_blocking=True)
+            ctx.done_event.record(stream)
+        ctx.busy = True
+        ctx.requests = reqs
+        ctx.region = region
+        ctx.n = n
+        self.batches += 1
+
+    def poll(self):
+        """Contexts whose batch (and result copies) finished."""
+        out = []
+        for c in self.ctxs:
+            if c.busy and c.done_event.query():
+                out.append(c)
+        return out
+
+    def collect(self, ctx):
+        """Per-request results of a finished context; frees it."""
+        st = ctx.h_status.numpy()[:ctx.n].copy()
+        its = ctx.h_iters.numpy()[:ctx.n].copy()
+        hard = ctx.h_hard.numpy()
+        k = 0
+        for r in ctx.requests:
+            cfgs = self.pool.cfgs[r.template]
+            r.status = st[k:k + len(cfgs)]
+            r.iters = its[k:k + len(cfgs)]
+            r.hard = []
+            for c, cfg in enumerate(cfgs):
+                B = cfg[1] + (16 if cfg[2] == 0 else 24)
+                o = int(ctx.hard_off[k + c])
+                r.hard.append(hard[o:o + B // 8].copy())
+            k += len(cfgs)
+        reqs, region = ctx.requests, ctx.region
+        ctx.busy = False
+        ctx.requests = []
+        ctx.region = None
+        return reqs, region
+
+    def close(self):
+        for c in self.ctxs:
+            c.close()
+
+
+# --------------------------------------------------------------------------
+# The loop
+# --------------------------------------------------------------------------
+@dataclass
+class ServeStats:
+    requests: List[Request] = field(default_factory=list)
+    wall_s: float = 0.0
+    batches: int = 0
+    admission_waits: int = 0
+    max_blocked_arrivals: int = 0
+    max_inflight: int = 0
+    peak_pool_bytes: int = 0
+
+    def summary(self, deadline_s):
+        done = [r for r in self.requests if r.done >= 0]
+        lat = np.array([r.latency for r in done]) if done else np.array([np.nan])
+        info = sum(r.info_bits for r in done if r.ok)
+        return {
+            "requests": len(done), "batches": self.batches,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                           "p99": float(np.percentile(lat, 99) * 1e3),
+                           "mean": float(np.mean(lat) * 1e3)},
+            "deadline_miss_rate": float(np.mean(lat > deadline_s)),
+            "info_gbps": info / self.wall_s / 1e9 if self.wall_s > 0 else 0.0,
+            "codeblocks": int(sum(r.n_cbs for r in done)),
+            "crc_fail_rate": float(np.mean([not r.ok for r in done])) if done else 0.0,
+            "retransmissions": int(sum(1 for r in done if r.rv_index > 0)),
+            "admission_waits": self.admission_waits,
+            "max_blocked_arrivals": self.max_blocked_arrivals,
+            "max_batches_in_flight": self.max_inflight,
+            "peak_llr_pool_MB": self.peak_pool_bytes / 2 ** 20,
+            "wall_s": self.wall_s,
+        }
+
+
+class Server:
+    """Continuous-batching request loop over an executor (GpuExecutor)."""
+
+    def __init__(self, executor, deadline_s=0.5e-3, harq_rtt_s=1.0e-3, n_ues=16, n_procs=4,
+                 clock=time.perf_counter):
+        self.ex = executor
+        self.deadline_s = deadline_s
+        self.harq_rtt_s = harq_rtt_s
+        self.n_ues, self.n_procs = n_ues, n_procs
+        self.clock = clock
+
+    def run(self, arrivals, templates, ues=None, until_drained=True, max_wall_s=60.0):
+        """arrivals: sorted new-data arrival instants (s, relative to start);
+        templates: request template per arrival; ues: UE per arrival (default
+        round robin).  Returns ServeStats."""
+        pool = self.ex.pool
+        stats = ServeStats()
+        proc_busy = np.zeros((self.n_ues, self.n_procs), bool)
+        pending = []        # (time, seq, Request) not yet arrived, heap-ordered by time
+        import heapq
+        seq = 0
+        for i, (t, tpl) in enumerate(zip(arrivals, templates)):
+            ue = int(ues[i]) if ues is not None else i % self.n_ues
+            heapq.heappush(pending, (float(t), seq, ("new", ue, int(tpl))))
+            seq += 1
+        queue = []          # admitted-in-order FIFO of arrived Requests
+        blocked = []        # new-data arrivals waiting for a free HARQ process of their UE
+        rid = 0
+        t0 = self.clock()
+        inflight = 0
+
+        def make(kind_tpl, ue, proc, rvi, t_arr):
+            nonlocal rid
+            r = Request(rid=rid, template=kind_tpl, ue=ue, proc=proc, rv_index=rvi,
+                        arrival=t_arr, deadline=t_arr + self.deadline_s,
+                        n_cbs=len(pool.cfgs[kind_tpl]), info_bits=pool.info_bits[kind_tpl])
+            rid += 1
+            stats.requests.append(r)
+            return r
+
+        while True:
+            now = self.clock() - t0
+            # completions
+            for ctx in self.ex.poll():
+                reqs, region = self.ex.collect(ctx)
+                self.ex.llr_pool.release(*region)
+                inflight -= 1
+                for r in reqs:
+                    r.done = now
+                    if not r.ok and r.rv_index + 1 < len(RV_SEQUENCE):
+                        heapq.heappush(pending, (now + self.harq_rtt_s, seq,
+                                                 ("retx", r.ue, r.template, r.proc,
+                                                  r.rv_index + 1)))
+                        seq += 1
+                    else:
+                        proc_busy[r.ue, r.proc] = False
+            # arrivals
+            while pending and pending[0][0] <= now:
+                t_arr, _, ev = heapq.heappop(pending)
+                if ev[0] == "new":
+                    blocked.append((t_arr, ev[1], ev[2]))
+                else:
+                    _, ue, tpl, proc, rvi = ev
+                    queue.append(make(tpl, ue, proc, rvi, t_arr))
+            still = []
+            for t_arr, ue, tpl in blocked:
+                free = np.flatnonzero(~proc_busy[ue])
+                if len(free):
+                    proc_busy[ue, free[0]] = True
+                    queue.append(make(tpl, ue, int(free[0]), 0, t_arr))
+                else:
+                    still.append((t_arr, ue, tpl))
+            stats.max_blocked_arrivals = max(stats.max_blocked_arrivals, len(still))
+            blocked = still
+            # admission: continuous batching into free contexts
+            while queue:
+                ctx = self.ex.free_context()
+                if ctx is None:
+                    break
+                batch, cbs, elems = [], 0, 0
+                for r in queue:
+                    e = sum(c[3] for c in pool.cfgs[r.template])
+                    if cbs + r.n_cbs > self.ex.max_cbs:
+                        break
+                    batch.append(r)
+                    cbs += r.n_cbs
+                    elems += e
+                region = self.ex.llr_pool.alloc(elems) if batch else None
+                while batch and region is None:
+                    # admission by free device memory: shrink the batch to fit
+                    r = batch.pop()
+                    elems -= sum(c[3] for c in pool.cfgs[r.template])
+                    region = self.ex.llr_pool.alloc(elems) if batch else None
+                if not batch:
+                    stats.admission_waits += 1
+                    if inflight == 0:
+                        raise RuntimeError("request does not fit in the device LLR pool")
+                    break
+                del queue[:len(batch)]
+                now = self.clock() - t0
+                for r in batch:
+                    r.admitted = now
+                self.ex.launch(ctx, batch, region)
+                inflight += 1
+                stats.max_inflight = max(stats.max_inflight, inflight)
+                stats.peak_pool_bytes = max(stats.peak_pool_bytes, 2 * self.ex.llr_pool.used)
+            if not pending and not queue and not blocked and inflight == 0:
+                break
+            if not until_drained and not pending:
+                break
+            if now > max_wall_s:
+                break
+        stats.wall_s = self.clock() - t0
+        stats.batches = self.ex.batches
+        return stats
+
+
+def poisson_arrivals(rate_per_s, duration_s, seed=0):
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_per_s)
+        if t > duration_s:
+            return np.array(out)
+        out.append(t)

@@ -206,6 +206,31 @@ int32_t ldpc_mi355_plan_tiles(const ldpc_mi355_plan_t *plan);
  * 0, or -1 for an invalid (bgn, zc, k). */
 int32_t ldpc_mi355_set_tile_codeblocks(int32_t bgn, int32_t zc, int32_t k);
 
+/* Reusable batch contexts, for continuous batching: every decode takes a new
+ * set of up to `capacity` codeblocks (configs may differ from call to call),
+ * builds descriptors on the host into the context's pinned staging buffer,
+ * and issues one asynchronous copy plus the decode launch on `stream` -- no
+ * allocation, no synchronization.  A context runs one decode at a time: a
+ * decode on a context whose previous decode is still in flight first waits
+ * for it; ldpc_mi355_batch_query polls (1 done, 0 running), batch_wait
+ * blocks.  flags as for plans (bit 0 posteriors, bit 1 pack, bit 2 never
+ * pack).  Algos 1 and 4.  Returns 0, -2 invalid codeblock config
+ * (*bad_index), -3 unsupported algo, -4 invalid argument, -1 GPU error. */
+typedef struct ldpc_mi355_batch ldpc_mi355_batch_t;
+int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t capacity,
+                                const ldpc_decoder_cfg_t *decoder_config, uint32_t flags);
+int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *batch, const ldpc_mi355_cb_t *cbs, int32_t n,
+                                const int16_t *llr, int16_t *soft_pool, uint8_t *hard,
+                                int32_t *status, int32_t *iterations, void *lq, void *stream,
+                                int32_t *bad_index);
+/* Output layout of the context's last decode (like plan_layout); returns
+ * its number of codeblocks. */
+int32_t ldpc_mi355_batch_layout(const ldpc_mi355_batch_t *batch, uint64_t *hard_bytes,
+                                uint64_t *lq_elems, uint32_t *hard_offsets, uint32_t *lq_offsets);
+int32_t ldpc_mi355_batch_query(ldpc_mi355_batch_t *batch);
+int32_t ldpc_mi355_batch_wait(ldpc_mi355_batch_t *batch);
+void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *batch);
+
 /* 1 when this build runs decoder algo `algo` on the GPU, else 0 (algo 0,
  * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls
  * then return -1 with a message instead of decoding). */

@@ -1,0 +1,76 @@
+"""Served decode loop on the GPU (deeplearning_5g_ldpc_decoder_amd/serving.py):
+Poisson arrivals of mixed-size requests with HARQ retransmissions are batched
+continuously over several launch contexts; afterwards every request is decoded
+again alone, in the same per-process order, with fresh device HARQ buffers, and
+must give the same status, iteration counts and hard bits bit for bit.  First
+transmissions are also checked against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ROW = [1, 8, 22936, 0, 1280, 32767, -32767]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_serving_results_same_alone_and_in_the_mix(oracle_mod):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from deeplearning_5g_ldpc_decoder_amd.serving import (RV_SEQUENCE, DevicePool, GpuExecutor,
+                                                          HarqSlots, RequestPool, Server,
+                                                          poisson_arrivals)
+    _say("serving: synthesizing 24 request templates")
+    pool = RequestPool(24, seed=2, max_cbs=16, snr_offset_db=-1.0, threads=16)
+    harq = HarqSlots(16, 4, 16)
+    llr_pool = DevicePool(64 << 20)
+    ex = GpuExecutor(pool, harq, llr_pool, n_contexts=3, max_batch_codeblocks=96)
+    srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=0.3e-3)
+    rng = np.random.default_rng(4)
+    arr = poisson_arrivals(20000.0, 0.02, seed=4)
+    _say(f"serving: {len(arr)} arrivals")
+    stats = srv.run(arr, rng.integers(0, len(pool), len(arr)), ues=rng.integers(0, 16, len(arr)))
+    s = stats.summary(0.5e-3)
+    _say(f"serving: {s}")
+    reqs = stats.requests
+    assert s["requests"] == len(reqs) and all(r.done >= 0 for r in reqs)
+    assert s["max_batches_in_flight"] >= 2
+    assert any(r.rv_index > 0 for r in reqs)
+    ex.close()
+
+    _say("serving: replaying every request alone")
+    harq2 = HarqSlots(16, 4, 16)
+    ex2 = GpuExecutor(pool, harq2, llr_pool, n_contexts=1, max_batch_codeblocks=96)
+    for r in sorted(reqs, key=lambda q: q.admitted):
+        q = type(r)(rid=r.rid, template=r.template, ue=r.ue, proc=r.proc, rv_index=r.rv_index,
+                    arrival=0.0, deadline=0.0)
+        region = llr_pool.alloc(sum(c[3] for c in pool.cfgs[q.template]))
+        ctx = ex2.free_context()
+        ex2.launch(ctx, [q], region)
+        torch.cuda.synchronize()
+        assert ex2.poll() == [ctx]
+        ex2.collect(ctx)
+        llr_pool.release(*region)
+        assert np.array_equal(q.status, r.status), r.rid
+        assert np.array_equal(q.iters, r.iters), r.rid
+        assert all(np.array_equal(a, b) for a, b in zip(q.hard, r.hard)), r.rid
+    ex2.close()
+
+    _say("serving: first transmissions against the oracle")
+    O = oracle_mod
+    for r in [q for q in reqs if q.rv_index == 0][:6]:
+        llr, lens = pool.llr(r.template, 0)
+        pos = 0
+        for c, (cfg, ln) in enumerate(zip(pool.cfgs[r.template], lens)):
+            out = O.decode_path(llr[pos:pos + ln], tuple(cfg[:4]) + (RV_SEQUENCE[0], cfg[5], 0),
+                                ROW)
+            assert out["status"] == r.status[c] and out["iters"] == r.iters[c]
+            if out["iters"]:
+                assert np.array_equal(out["hard"], r.hard[c])
+            pos += ln

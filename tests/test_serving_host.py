@@ -1,0 +1,143 @@
+"""Host logic of the serving loop (deeplearning_5g_ldpc_decoder_amd/serving.py)
+on the CPU: admission by free device memory, HARQ retransmission with the
+redundancy-version sequence 0, 2, 3, 1 and soft combining, and result
+independence from batching.  A test executor decodes with the oracle
+(c_ldpc_5g_decode restatement with the HARQ buffer) and completes each batch
+after a fixed service time on a simulated clock."""
+import numpy as np
+import pytest
+
+from deeplearning_5g_ldpc_decoder_amd.serving import (RV_SEQUENCE, DevicePool, RequestPool,
+                                                      Server)
+
+ROW = [1, 8, 22936, 0, 1280, 32767, -32767]
+
+
+class FakeClock:
+    def __init__(self, dt=2e-6):
+        self.t = 0.0
+        self.dt = dt
+
+    def __call__(self):
+        self.t += self.dt
+        return self.t
+
+
+class OracleExecutor:
+    def __init__(self, pool, llr_pool, clock, n_contexts=2, max_batch_codeblocks=64,
+                 service_s=300e-6):
+        from oracle import oracle as O
+        self.O = O
+        self.pool, self.llr_pool, self.clock = pool, llr_pool, clock
+        self.max_cbs = max_batch_codeblocks
+        self.ctxs = [dict(busy=False, t_done=0.0, reqs=None, region=None)
+                     for _ in range(n_contexts)]
+        self.service_s = service_s
+        self.soft = {}
+        self.batches = 0
+        self.batch_sizes = []
+
+    def free_context(self):
+        for c in self.ctxs:
+            if not c["busy"]:
+                return c
+        return None
+
+    def decode_request(self, r):
+        llr, lens = self.pool.llr(r.template, r.rv_index)
+        pos = 0
+        st, its, hard = [], [], []
+        for c, (cfg, ln) in enumerate(zip(self.pool.cfgs[r.template], lens)):
+            key = (r.ue, r.proc, c)
+            harq = 1 if r.rv_index > 0 else 0
+            full = (cfg[0], cfg[1], cfg[2], cfg[3], RV_SEQUENCE[r.rv_index], cfg[5], harq)
+            out = self.O.decode_path(llr[pos:pos + ln], full, ROW,
+                                     harq=self.soft.get(key) if harq else None)
+            self.soft[key] = out["soft"].copy()
+            st.append(out["status"])
+            its.append(out["iters"])
+            hard.append(out["hard"])
+            pos += ln
+        r.status, r.iters, r.hard = np.array(st), np.array(its), hard
+
+    def launch(self, ctx, reqs, region):
+        for r in reqs:
+            self.decode_request(r)
+        ctx.update(busy=True, reqs=reqs, region=region, t_done=self.clock.t + self.service_s)
+        self.batches += 1
+        self.batch_sizes.append(len(reqs))
+
+    def poll(self):
+        return [c for c in self.ctxs if c["busy"] and self.clock.t >= c["t_done"]]
+
+    def collect(self, ctx):
+        ctx["busy"] = False
+        return ctx["reqs"], ctx["region"]
+
+
+@pytest.fixture(scope="module")
+def pool():
+    # low SNR so that first transmissions fail and HARQ retransmissions run
+    return RequestPool(8, seed=5, max_cbs=6, snr_offset_db=-1.5, threads=8, pinned=False)
+
+
+def test_device_pool_first_fit_and_coalescing():
+    p = DevicePool(1 << 20, tensor=np.zeros(1, np.int16))
+    a = p.alloc(1000)
+    b = p.alloc(5000)
+    c = p.alloc(300)
+    assert a[0] == 0 and b[0] == a[1] and c[0] == a[1] + b[1]
+    p.release(*b)
+    d = p.alloc(4000)                 # first fit reuses b's hole
+    assert d[0] == b[0]
+    p.release(*a)
+    p.release(*c)
+    p.release(*d)
+    assert p.used == 0 and p.free == [(0, p.elems)]
+    assert p.alloc(p.elems + 1) is None
+
+
+def test_serving_harq_chain_and_batching_independence(pool):
+    clock = FakeClock()
+    llr_pool = DevicePool(64 << 20, tensor=np.zeros(1, np.int16))
+    ex = OracleExecutor(pool, llr_pool, clock)
+    srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=200e-6, n_ues=4, n_procs=2, clock=clock)
+    rng = np.random.default_rng(1)
+    arrivals = np.sort(rng.uniform(0, 3e-3, 40))
+    stats = srv.run(arrivals, rng.integers(0, len(pool), 40), ues=rng.integers(0, 4, 40))
+    reqs = stats.requests
+    assert all(r.done >= r.arrival for r in reqs)
+    assert max(ex.batch_sizes) > 1                        # requests were batched together
+    retx = [r for r in reqs if r.rv_index > 0]
+    assert retx                                           # HARQ retransmissions happened
+    # every retransmission follows a failed transmission of the same process,
+    # with the next redundancy version
+    for r in retx:
+        prev = [q for q in reqs if q.ue == r.ue and q.proc == r.proc and q.rid < r.rid]
+        assert prev and not prev[-1].ok and prev[-1].rv_index == r.rv_index - 1
+        assert prev[-1].template == r.template
+    # HARQ combining helps: some chains that failed at rv0 end decoded
+    assert any(r.ok for r in retx)
+    # replay every request alone, in order, with fresh soft buffers: identical
+    alone = OracleExecutor(pool, llr_pool, clock)
+    for r in sorted(reqs, key=lambda q: q.admitted):
+        q = type(r)(rid=r.rid, template=r.template, ue=r.ue, proc=r.proc, rv_index=r.rv_index,
+                    arrival=0.0, deadline=0.0)
+        alone.decode_request(q)
+        assert np.array_equal(q.status, r.status) and np.array_equal(q.iters, r.iters)
+        assert all(np.array_equal(a, b) for a, b in zip(q.hard, r.hard))
+    s = stats.summary(0.5e-3)
+    assert s["requests"] == len(reqs) and s["retransmissions"] == len(retx)
+
+
+def test_admission_waits_for_device_memory(pool):
+    clock = FakeClock()
+    need = max(sum(c[3] for c in cfgs) for cfgs in pool.cfgs)
+    llr_pool = DevicePool(2 * need + 4 * DevicePool.ALIGN, tensor=np.zeros(1, np.int16))
+    ex = OracleExecutor(pool, llr_pool, clock, n_contexts=4)
+    srv = Server(ex, harq_rtt_s=100e-6, n_ues=16, n_procs=4, clock=clock)
+    stats = srv.run(np.zeros(20), [i % len(pool) for i in range(20)], ues=np.arange(20) % 16)
+    assert stats.admission_waits > 0
+    assert stats.peak_pool_bytes <= 2 * llr_pool.elems
+    assert len([r for r in stats.requests if r.rv_index == 0]) == 20
+    assert llr_pool.used == 0
