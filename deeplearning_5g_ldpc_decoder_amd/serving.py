@@ -39,6 +39,12 @@ from ._lib import CbDescriptor, DecoderConfig, LdpcInConfig, ptr
 RV_SEQUENCE = (0, 2, 3, 1)
 DEFAULT_ROW = (1, 8, 22936, 0, 1280, 32767, -32767)
 
+# ldpc_mi355_cb_t as a numpy record (24 bytes): descriptors of a batch are
+# built by concatenating per-template records instead of field by field.
+CB_DTYPE = np.dtype([("cfg", "<u2", (7,)), ("reserved", "<i2"), ("llr_offset", "<u4"),
+                     ("soft_slot", "<i4")])
+assert CB_DTYPE.itemsize == 24
+
 
 # --------------------------------------------------------------------------
 # Request material
@@ -64,8 +70,10 @@ class RequestPool:
             self.cfgs.append([d[0] for d in draws])
             self.esn0.append([d[1] + snr_offset_db for d in draws])
             self.sizes.append(k)
-        jobs = [(t, c, rvi) for t in range(n_templates) for c in range(self.sizes[t])
-                for rvi in range(len(RV_SEQUENCE))]
+        # ordered so that each (template, rv index) is one contiguous run of
+        # its codeblocks' LLRs
+        jobs = [(t, c, rvi) for t in range(n_templates) for rvi in range(len(RV_SEQUENCE))
+                for c in range(self.sizes[t])]
 
         def one(job):
             t, c, rvi = job
@@ -97,15 +105,37 @@ class RequestPool:
             self.pinned = torch.from_numpy(flat).pin_memory()
             self.flat = self.pinned.numpy()
         self.info_bits = [sum(c[1] for c in cfgs) for cfgs in self.cfgs]
+        self.cb_info = [np.array([c[1] for c in cfgs], np.int64) for cfgs in self.cfgs]
+        self.cb_hard = [np.array([(c[1] + (16 if c[2] == 0 else 24)) // 8 for c in cfgs])
+                        for cfgs in self.cfgs]
+        self.elems = [sum(c[3] for c in cfgs) for cfgs in self.cfgs]
+        # descriptor records per (template, rv index): llr_offset relative to
+        # the request's first LLR, soft_slot = codeblock index (the caller adds
+        # the base of the UE/process slots)
+        self.desc = {}
+        for t, cfgs in enumerate(self.cfgs):
+            lens = np.array([c[3] for c in cfgs], np.int64)
+            rel = np.concatenate([[0], np.cumsum(lens)[:-1]])
+            for rvi, rv in enumerate(RV_SEQUENCE):
+                d = np.zeros(len(cfgs), CB_DTYPE)
+                d["cfg"] = [(c[0], c[1], c[2], c[3], rv, c[5], 1 if rvi > 0 else 0)
+                            for c in cfgs]
+                d["llr_offset"] = rel
+                d["soft_slot"] = np.arange(len(cfgs))
+                self.desc[(t, rvi)] = d
 
     def __len__(self):
         return len(self.cfgs)
 
+    def span(self, t, rvi):
+        """[a, b) of template t's LLRs for rv index rvi in the flat array."""
+        parts = self.offsets[(t, rvi)]
+        return parts[0][0], parts[-1][0] + parts[-1][1]
+
     def llr(self, t, rvi):
         """Concatenated LLRs of template t's codeblocks for rv index rvi."""
-        parts = self.offsets[(t, rvi)]
-        a, b = parts[0][0], parts[-1][0] + parts[-1][1]
-        return self.flat[a:b], [p[1] for p in parts]
+        a, b = self.span(t, rvi)
+        return self.flat[a:b], [p[1] for p in self.offsets[(t, rvi)]]
 
 
 @dataclass
@@ -124,11 +154,26 @@ class Request:
     batch_id: int = -1
     status: Optional[np.ndarray] = None
     iters: Optional[np.ndarray] = None
-    hard: Optional[List[np.ndarray]] = None
+    hard_bytes: Optional[np.ndarray] = None   # the request's hard decisions, back to back
+    hard_len: Optional[np.ndarray] = None     # B/8 per codeblock
+    final: bool = False                       # last transmission of its HARQ chain
+    _hard: Optional[List[np.ndarray]] = None
 
     @property
     def latency(self):
         return self.done - self.arrival
+
+    @property
+    def hard(self):
+        """Per-codeblock B/8-byte hard decisions."""
+        if self._hard is None and self.hard_bytes is not None:
+            ends = np.cumsum(self.hard_len)
+            self._hard = np.split(self.hard_bytes, ends[:-1])
+        return self._hard
+
+    @hard.setter
+    def hard(self, v):
+        self._hard = v
 
     @property
     def ok(self):
@@ -251,8 +296,8 @@ class GpuExecutor:
         self.ctxs = [_Context(i, max_batch_codeblocks, self.row, device)
                      for i in range(n_contexts)]
         self.max_cbs = max_batch_codeblocks
-        self._stage = torch.empty(llr_pool.elems, dtype=torch.int16).pin_memory()
-        self._stage_np = self._stage.numpy()
+        if pool.pinned is None:
+            raise ValueError("the GPU executor needs the request LLRs in pinned memory")
         self.batches = 0
 
     def free_context(self):
@@ -266,39 +311,38 @@ class GpuExecutor:
         back -- all queued on the context's stream."""
         torch = self.torch
         off, need = region
-        n = 0
+        parts = []
         pos = 0
-        for r in reqs:
-            llr, lens = self.pool.llr(r.template, r.rv_index)
-            self._stage_np[off + pos:off + pos + len(llr)] = llr
-            cfgs = self.pool.cfgs[r.template]
-            for c, (cfg, ln) in enumerate(zip(cfgs, lens)):
-                d = ctx.descs[n]
-                d.cfg = LdpcInConfig(cfg[0], cfg[1], cfg[2], cfg[3], RV_SEQUENCE[r.rv_index],
-                                     cfg[5], 1 if r.rv_index > 0 else 0)
-                d.llr_offset = off + pos
-                d.soft_slot = self.harq.slot(r.ue, r.proc, c)
-                pos += ln
-                n += 1
-        assert n <= ctx.cap and pos <= need
         stream = ctx.stream
+        dev = self.llr_pool.tensor
         with torch.cuda.stream(stream):
-            self.llr_pool.tensor[off:off + pos].copy_(self._stage[off:off + pos],
-                                                      non_blocking=True)
+            for r in reqs:
+                # one host->device copy per request, straight from its pinned
+                # LLRs into the batch's pool region
+                a, b = self.pool.span(r.template, r.rv_index)
+                dev[off + pos:off + pos + b - a].copy_(self.pool.pinned[a:b], non_blocking=True)
+                d = self.pool.desc[(r.template, r.rv_index)].copy()
+                d["llr_offset"] += off + pos
+                d["soft_slot"] += self.harq.slot(r.ue, r.proc, 0)
+                parts.append(d)
+                pos += b - a
+        descs = np.concatenate(parts) if len(parts) > 1 else parts[0]
+        n = len(descs)
+        assert n <= ctx.cap and pos <= need
+        ctx.desc_buf = descs
         bad = ctypes.c_int32(-1)
         st = _lib.load().ldpc_mi355_batch_decode(
-            ctx.handle, ctx.descs, n, ptr(self.llr_pool.tensor), ptr(self.harq.tensor),
-            ptr(ctx.hard), ptr(ctx.status), ptr(ctx.iters), None,
-            ctypes.c_void_p(stream.cuda_stream), ctypes.byref(bad))
+            ctx.handle, descs.ctypes.data_as(ctypes.POINTER(CbDescriptor)), n,
+            ptr(self.llr_pool.tensor), ptr(self.harq.tensor), ptr(ctx.hard), ptr(ctx.status),
+            ptr(ctx.iters), None, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(bad))
         if st != 0:
             raise RuntimeError(f"ldpc_mi355_batch_decode failed ({st}), codeblock {bad.value}")
-        hb = ctypes.c_uint64()
-        _lib.load().ldpc_mi355_batch_layout(ctx.handle, ctypes.byref(hb), None,
-                                            ptr(ctx.hard_off), None)
+        # codeblocks' hard bytes are laid out back to back in batch order
+        hb = int(sum(int(self.pool.cb_hard[r.template].sum()) for r in reqs))
         with torch.cuda.stream(stream):
             ctx.h_status[:n].copy_(ctx.status[:n], non_blocking=True)
             ctx.h_iters[:n].copy_(ctx.iters[:n], non_blocking=True)
-            ctx.h_hard[:hb.value].copy_(ctx.hard[:hb.value], non_blocking=True)
+            ctx.h_hard[:hb].copy_(ctx.hard[:hb], non_blocking=True)
             ctx.done_event.record(stream)
         ctx.busy = True
         ctx.requests = reqs
@@ -316,20 +360,22 @@ class GpuExecutor:
 
     def collect(self, ctx):
         """Per-request results of a finished context; frees it."""
-        st = ctx.h_status.numpy()[:ctx.n].copy()
-        its = ctx.h_iters.numpy()[:ctx.n].copy()
+        st = ctx.h_status.numpy()
+        its = ctx.h_iters.numpy()
         hard = ctx.h_hard.numpy()
         k = 0
+        h = 0
         for r in ctx.requests:
-            cfgs = self.pool.cfgs[r.template]
-            r.status = st[k:k + len(cfgs)]
-            r.iters = its[k:k + len(cfgs)]
-            r.hard = []
-            for c, cfg in enumerate(cfgs):
-                B = cfg[1] + (16 if cfg[2] == 0 else 24)
-                o = int(ctx.hard_off[k + c])
-                r.hard.append(hard[o:o + B // 8].copy())
-            k += len(cfgs)
+            m = r.n_cbs
+            hl = self.pool.cb_hard[r.template]
+            nb = int(hl.sum())
+            r.status = st[k:k + m].copy()
+            r.iters = its[k:k + m].copy()
+            r.hard_bytes = hard[h:h + nb].copy()
+            r.hard_len = hl
+            r._hard = None
+            k += m
+            h += nb
         reqs, region = ctx.requests, ctx.region
         ctx.busy = False
         ctx.requests = []
@@ -347,6 +393,7 @@ class GpuExecutor:
 @dataclass
 class ServeStats:
     requests: List[Request] = field(default_factory=list)
+    cb_info: list = field(default_factory=list)
     wall_s: float = 0.0
     batches: int = 0
     admission_waits: int = 0
@@ -357,7 +404,11 @@ class ServeStats:
     def summary(self, deadline_s):
         done = [r for r in self.requests if r.done >= 0]
         lat = np.array([r.latency for r in done]) if done else np.array([np.nan])
-        info = sum(r.info_bits for r in done if r.ok)
+        # goodput: payload bits of the codeblocks that passed CRC on the last
+        # transmission of their HARQ chain
+        info = sum(int(self.cb_info[r.template][r.status == 0].sum())
+                   for r in done if r.final)
+        first = [r for r in done if r.rv_index == 0]
         return {
             "requests": len(done), "batches": self.batches,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
@@ -366,7 +417,10 @@ class ServeStats:
             "deadline_miss_rate": float(np.mean(lat > deadline_s)),
             "info_gbps": info / self.wall_s / 1e9 if self.wall_s > 0 else 0.0,
             "codeblocks": int(sum(r.n_cbs for r in done)),
-            "crc_fail_rate": float(np.mean([not r.ok for r in done])) if done else 0.0,
+            "transmission_fail_rate": float(np.mean([not r.ok for r in done])) if done else 0.0,
+            "codeblock_bler_first_tx": float(np.mean(np.concatenate(
+                [r.status != 0 for r in first]))) if first else 0.0,
+            "chains_failed": int(sum(1 for r in done if r.final and not r.ok)),
             "retransmissions": int(sum(1 for r in done if r.rv_index > 0)),
             "admission_waits": self.admission_waits,
             "max_blocked_arrivals": self.max_blocked_arrivals,
@@ -392,7 +446,7 @@ class Server:
         templates: request template per arrival; ues: UE per arrival (default
         round robin).  Returns ServeStats."""
         pool = self.ex.pool
-        stats = ServeStats()
+        stats = ServeStats(cb_info=pool.cb_info)
         proc_busy = np.zeros((self.n_ues, self.n_procs), bool)
         pending = []        # (time, seq, Request) not yet arrived, heap-ordered by time
         import heapq
@@ -404,6 +458,7 @@ class Server:
         queue = []          # admitted-in-order FIFO of arrived Requests
         blocked = []        # new-data arrivals waiting for a free HARQ process of their UE
         rid = 0
+        batches0 = self.ex.batches
         t0 = self.clock()
         inflight = 0
 
@@ -425,7 +480,8 @@ class Server:
                 inflight -= 1
                 for r in reqs:
                     r.done = now
-                    if not r.ok and r.rv_index + 1 < len(RV_SEQUENCE):
+                    r.final = r.ok or r.rv_index + 1 == len(RV_SEQUENCE)
+                    if not r.final:
                         heapq.heappush(pending, (now + self.harq_rtt_s, seq,
                                                  ("retx", r.ue, r.template, r.proc,
                                                   r.rv_index + 1)))
@@ -457,7 +513,7 @@ class Server:
                     break
                 batch, cbs, elems = [], 0, 0
                 for r in queue:
-                    e = sum(c[3] for c in pool.cfgs[r.template])
+                    e = pool.elems[r.template]
                     if cbs + r.n_cbs > self.ex.max_cbs:
                         break
                     batch.append(r)
@@ -467,7 +523,7 @@ class Server:
                 while batch and region is None:
                     # admission by free device memory: shrink the batch to fit
                     r = batch.pop()
-                    elems -= sum(c[3] for c in pool.cfgs[r.template])
+                    elems -= pool.elems[r.template]
                     region = self.ex.llr_pool.alloc(elems) if batch else None
                 if not batch:
                     stats.admission_waits += 1
@@ -489,7 +545,7 @@ class Server:
             if now > max_wall_s:
                 break
         stats.wall_s = self.clock() - t0
-        stats.batches = self.ex.batches
+        stats.batches = self.ex.batches - batches0
         return stats
 
 

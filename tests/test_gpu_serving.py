@@ -49,8 +49,8 @@ def test_serving_results_same_alone_and_in_the_mix(oracle_mod):
     ex2 = GpuExecutor(pool, harq2, llr_pool, n_contexts=1, max_batch_codeblocks=96)
     for r in sorted(reqs, key=lambda q: q.admitted):
         q = type(r)(rid=r.rid, template=r.template, ue=r.ue, proc=r.proc, rv_index=r.rv_index,
-                    arrival=0.0, deadline=0.0)
-        region = llr_pool.alloc(sum(c[3] for c in pool.cfgs[q.template]))
+                    arrival=0.0, deadline=0.0, n_cbs=r.n_cbs)
+        region = llr_pool.alloc(pool.elems[q.template])
         ctx = ex2.free_context()
         ex2.launch(ctx, [q], region)
         torch.cuda.synchronize()
