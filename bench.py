@@ -291,6 +291,61 @@ def bench_serve(args):
     print(json.dumps(line))
 
 
+def bench_bler(args):
+    """FP32 min-sum (algo 4, 20 iterations, CRC early stop) BLER sweep of the
+    BG2 workload: Zc 104 / 208 (B = 1040 / 2080), R 0.2 / 0.5, Qm 2 / 4 (Gray
+    QAM, max-log LLRs), 8 SNR points over [-2, 3] dB (the harness's
+    snr_power_db: noise_power_db = -snr - 10 log10 Qm), args.batch codeblocks
+    per point (default 4096), each point one batched launch; BLER with its
+    99 % Wilson interval, and decode throughput."""
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, synth_batch
+    from deeplearning_5g_ldpc_decoder_amd.ldpc_sim import bler_confidence_interval
+    row = [4, 20, 22936, 0, 1280, 32767, -32767]
+    n = args.batch if args.batch != BATCH else 4096
+    snrs = np.linspace(-2.0, 3.0, 8)
+    table = []
+    gpu_s, info = 0.0, 0
+    for A in (1016, 2056):
+        for R in (0.2, 0.5):
+            for qm in (2, 4):
+                B = A + 24
+                E = int(round(B / R / qm)) * qm
+                cfg = (2, A, 2, E, 0, qm, 0)
+                plan = BatchPlan([cfg] * n, row)
+                out = plan.alloc_outputs("cuda")
+                pts = []
+                for si, snr in enumerate(snrs):
+                    llr, pay = synth_batch(cfg, n, float(-snr - 10 * np.log10(qm)), seed=si,
+                                           threads=min(16, os.cpu_count() or 8))
+                    d = torch.from_numpy(llr.reshape(-1)).cuda()
+                    plan.launch(d, out)
+                    torch.cuda.synchronize()
+                    t = time.perf_counter()
+                    plan.launch(d, out)
+                    torch.cuda.synchronize()
+                    gpu_s += time.perf_counter() - t
+                    info += n * A
+                    st = out["status"].cpu().numpy()
+                    e = int(np.count_nonzero(st != 0))
+                    lo, hi = bler_confidence_interval(e, n)
+                    pts.append({"snr_db": float(snr), "bler": e / n, "ci99": [lo, hi],
+                                "mean_iterations": float(out["iters"].float().mean())})
+                plan.close()
+                table.append({"cfg": cfg, "rate": R, "points": pts})
+                log(f"BG2 A={A} R={R} Qm={qm}: " +
+                    " ".join(f"{p['bler']:.3f}" for p in pts))
+    line = {"metric": "FP32 min-sum decoded info Gbps over the BG2 BLER sweep",
+            "value": info / gpu_s / 1e9, "unit": "Gbps", "n_gpus": 1,
+            "steps": len(table) * len(snrs), "warmup": 1,
+            "ms_per_step": gpu_s * 1e3 / (len(table) * len(snrs)), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"bler_bg2_zc104_208_fp32ms_20it_batch{n}",
+                       "decoder_row": row, "parallelism": "single-gpu"},
+            "sweep": table}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,11 +358,12 @@ def main():
                     help="batch launches only (for rocprofv3): no latency loop, no "
                          "per-codeblock path, no CPU leg")
     ap.add_argument("--workload", default="bg1_zc384",
-                    choices=["bg1_zc384", "mixed", "train", "serve"],
+                    choices=["bg1_zc384", "mixed", "train", "serve", "bler"],
                     help="bg1_zc384: the headline workload (default); mixed: 8192 mixed-size "
                          "codeblocks in one launch (use --batch to change); train: training "
                          "steps of the trainable decoder (BG1 Zc=96, batch 2048); serve: the "
-                         "served loop (Poisson arrivals, HARQ, continuous batching)")
+                         "served loop (Poisson arrivals, HARQ, continuous batching); bler: the "
+                         "FP32 BG2 BLER sweep")
     ap.add_argument("--serve-batch", type=int, default=256,
                     help="serve workload: codeblocks per batch at most")
     ap.add_argument("--no-reference", action="store_true",
@@ -329,6 +385,8 @@ def main():
         return bench_train(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "bler":
+        return bench_bler(args)
 
     import torch
     from deeplearning_5g_ldpc_decoder_amd import _lib
