@@ -260,6 +260,26 @@ def bench_serve(args):
         return srv.run(arr, rng.integers(0, len(pool), n), ues=rng.integers(0, 16, n))
 
     run(np.zeros(64))          # warmup
+    # unloaded latency: each request alone (copy in, decode, copy out, detect)
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.serving import Request
+    alone = []
+    for t in range(len(pool)):
+        r = Request(rid=t, template=t, ue=0, proc=0, rv_index=0, arrival=0.0, deadline=0.0,
+                    n_cbs=len(pool.cfgs[t]))
+        region = llr_pool.alloc(pool.elems[t])
+        ctx = ex.free_context()
+        t0 = time.perf_counter()
+        ex.launch(ctx, [r], region)
+        while not ex.poll():
+            pass
+        ex.collect(ctx)
+        alone.append(time.perf_counter() - t0)
+        llr_pool.release(*region)
+    alone = np.array(alone) * 1e3
+    unloaded = {"p50_ms": float(np.percentile(alone, 50)),
+                "p99_ms": float(np.percentile(alone, 99)), "requests": len(alone)}
+    log(f"unloaded latency p50 {unloaded['p50_ms']:.3f} ms p99 {unloaded['p99_ms']:.3f} ms")
     nsat = 600
     st = run(np.zeros(nsat))
     sat = st.summary(0.5e-3)
@@ -287,7 +307,8 @@ def bench_serve(args):
                        "max_batch_codeblocks": args.serve_batch,
                        "llr_pool_MB": 256, "harq_slots": 16 * 4 * 64,
                        "parallelism": "single-gpu"},
-            "capacity_requests_per_s": cap, "saturated": sat, "sweep": sweep}
+            "capacity_requests_per_s": cap, "unloaded": unloaded, "saturated": sat,
+            "sweep": sweep}
     print(json.dumps(line))
 
 

@@ -399,6 +399,8 @@ class ServeStats:
     admission_waits: int = 0
     max_blocked_arrivals: int = 0
     max_inflight: int = 0
+    host_launch_s: float = 0.0      # host time inside executor.launch
+    host_collect_s: float = 0.0     # host time inside executor.collect
     peak_pool_bytes: int = 0
 
     def summary(self, deadline_s):
@@ -425,6 +427,9 @@ class ServeStats:
             "admission_waits": self.admission_waits,
             "max_blocked_arrivals": self.max_blocked_arrivals,
             "max_batches_in_flight": self.max_inflight,
+            "host_us_per_batch": {
+                "launch": 1e6 * self.host_launch_s / max(self.batches, 1),
+                "collect": 1e6 * self.host_collect_s / max(self.batches, 1)},
             "peak_llr_pool_MB": self.peak_pool_bytes / 2 ** 20,
             "wall_s": self.wall_s,
         }
@@ -475,7 +480,9 @@ class Server:
             now = self.clock() - t0
             # completions
             for ctx in self.ex.poll():
+                tc = self.clock()
                 reqs, region = self.ex.collect(ctx)
+                stats.host_collect_s += self.clock() - tc
                 self.ex.llr_pool.release(*region)
                 inflight -= 1
                 for r in reqs:
@@ -534,7 +541,9 @@ class Server:
                 now = self.clock() - t0
                 for r in batch:
                     r.admitted = now
+                tl = self.clock()
                 self.ex.launch(ctx, batch, region)
+                stats.host_launch_s += self.clock() - tl
                 inflight += 1
                 stats.max_inflight = max(stats.max_inflight, inflight)
                 stats.peak_pool_bytes = max(stats.peak_pool_bytes, 2 * self.ex.llr_pool.used)
