@@ -124,7 +124,8 @@ def bench_train(args):
     (iteration, row degree), clip_llr = 32767/128, batch 2048 with Es/N0 uniform
     in [0, 2] dB per codeblock, BCE on -LLR of the B bits, float64 grad-norm
     clip, Adam.  A step = forward + loss + backward + optimizer step over one
-    resident batch (a pool of 4 synthesized batches is cycled).  Beside it:
+    fresh batch synthesized on the device inside the step (encode, rate
+    match, channel, float rate recovery).  Beside it:
     the same step through torch autograd of a torch restatement of the
     reference model (tests/unrolled_torch_ref.py) on the same GPU."""
     import torch
@@ -153,8 +154,14 @@ def bench_train(args):
     B = tc.B
     stream = torch.cuda.current_stream()
 
+    from deeplearning_5g_ldpc_decoder_amd.unrolled import synth_float_batch_device
+    nz_rng = np.random.default_rng(1)
+
     def step(k, ev=None):
-        x, tgt = pool[k % len(pool)]
+        if ev:
+            ev[3].record(stream)
+        # fresh inputs every step, synthesized on the device
+        x, tgt = synth_float_batch_device(tc, nz_rng.uniform(-2.0, 0.0, n), seed=1000 + k)
         opt.zero_grad(set_to_none=True)
         if ev:
             ev[0].record(stream)
@@ -173,18 +180,20 @@ def bench_train(args):
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     losses = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         losses.append(step(k, evs[k]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    synth_ms = float(np.mean([e[3].elapsed_time(e[0]) for e in evs]))
     fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     bwd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
     losses = [float(v) for v in losses]
     ms = wall * 1e3 / args.steps
-    log(f"kernels: {ms:.3f} ms/step (forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms)")
+    log(f"kernels: {ms:.3f} ms/step (synthesis {synth_ms:.3f} ms, forward {fwd_ms:.3f} ms, "
+        f"backward {bwd_ms:.3f} ms)")
 
     ref = None
     if not args.no_reference:
@@ -228,7 +237,7 @@ def bench_train(args):
                        "global_batch": n, "iterations": T, "alpha_mode": w["alpha_mode"],
                        "beta_mode": w["beta_mode"], "clip_llr": clip,
                        "parallelism": "single-gpu"},
-            "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+            "input_synthesis_ms": synth_ms, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
             "record_MB": k.record_bytes(n) / 2 ** 20, "scratch_MB": k.scratch_bytes(n) / 2 ** 20,
             "loss_first": losses[0], "loss_last": losses[-1],
             "reference_autograd": ref,

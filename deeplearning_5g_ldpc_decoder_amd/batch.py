@@ -192,3 +192,41 @@ def synth_batch(cfg, n, noise_power_db, seed=0, threads=8, want_payload=True):
     if st != 0:
         raise ValueError(f"invalid config {cfg}")
     return llr, pay
+
+
+def synth_device(cfg, n, noise_power_db, seed=0, payload=None, outputs=("llr",), stream=None):
+    """Device synthesis of n codeblocks of config cfg (ldpc_mi355_synth_device).
+    noise_power_db: scalar or n values.  outputs: any of "bits", "cw", "rm",
+    "llr" (int16 E per codeblock), "llr_f32", "beliefs".  Returns a dict of
+    torch device tensors shaped [n, ...]."""
+    import torch
+    L = _lib.load()
+    ic = LdpcInConfig(*[int(v) for v in cfg])
+    pc = _lib.LdpcConfig()
+    if L.gen_ldpc_parameters(ctypes.byref(pc), ctypes.byref(ic)) != 0:
+        raise ValueError(f"invalid config {cfg}")
+    dev = torch.device("cuda")
+    nz = torch.as_tensor(np.broadcast_to(np.asarray(noise_power_db, np.float32), (n,)).copy(),
+                         device=dev)
+    nt = pc.N + 2 * pc.Zc
+    shapes = dict(bits=(pc.B, torch.uint8), cw=(nt, torch.uint8), rm=(ic.E, torch.uint8),
+                  llr=(ic.E, torch.int16), llr_f32=(ic.E, torch.float32),
+                  beliefs=(nt, torch.float32))
+    want = set(outputs)
+    if "beliefs" in want:
+        want.add("llr_f32")
+    out = {k: torch.empty((n, shapes[k][0]), dtype=shapes[k][1], device=dev) for k in want}
+    pay = None
+    if payload is not None:
+        pay = torch.as_tensor(np.ascontiguousarray(payload, np.uint8), device=dev)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    g = lambda k: ptr(out[k]) if k in out else None  # noqa: E731
+    st = L.ldpc_mi355_synth_device(ctypes.byref(ic), n, ptr(nz), ctypes.c_uint64(seed),
+                                   ptr(pay) if pay is not None else None, g("bits"), g("cw"),
+                                   g("rm"), g("llr"), g("llr_f32"), g("beliefs"),
+                                   ctypes.c_void_p(stream))
+    if st != 0:
+        raise RuntimeError(f"ldpc_mi355_synth_device failed ({st})")
+    out["_keep"] = (nz, pay)    # inputs stay alive until the caller drops the dict
+    return out

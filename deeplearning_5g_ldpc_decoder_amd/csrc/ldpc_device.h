@@ -70,4 +70,19 @@ struct DecOut {
     const uint32_t *etab;
 };
 
+// Device input synthesis (ldpc_synth.hip): n codeblocks of one config.
+struct SynthArgs {
+    const float *noise_db;      // n values: noise power (dB) per codeblock
+    const uint8_t *payload;     // n*A bits (one per byte) or null: generated
+    uint8_t *bits_out;          // n*B payload + CRC bits, or null
+    uint8_t *cw_out;            // n*(N+2Zc) codeword bits, or null
+    uint8_t *rm_out;            // n*E rate-matched bits, or null
+    int16_t *llr_out;           // n*E quantized LLRs, or null
+    float *llr_f32;             // n*E float LLRs, or null (needed for beliefs)
+    float *beliefs;             // n*(N+2Zc) float beliefs, or null
+    const uint32_t *etab;       // edge table of (BG, Zc)
+    uint64_t seed;
+    int n, A, B, K, N, E, Zc, Qm, k0, crctype;
+};
+
 }  // namespace nrldpc

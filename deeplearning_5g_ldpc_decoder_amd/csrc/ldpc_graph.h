@@ -11,12 +11,27 @@ template <int BG> struct Graph;
 template <> struct Graph<1> {
     static constexpr int kRows = NR_BG1_ROWS;
     static constexpr int kEdges = NR_BG1_EDGES;
+    static constexpr int kInfoCols = 22;   // systematic columns (kb)
+    static constexpr int kCols = 68;
     __host__ __device__ static constexpr int start(int r) { return kBG1RowStart[r]; }
+    // base-graph column of edge e (compile-time use only)
+    __host__ __device__ static constexpr int col(int e) { return kBG1EdgeCol[e]; }
 };
 template <> struct Graph<2> {
     static constexpr int kRows = NR_BG2_ROWS;
     static constexpr int kEdges = NR_BG2_EDGES;
+    static constexpr int kInfoCols = 10;
+    static constexpr int kCols = 52;
     __host__ __device__ static constexpr int start(int r) { return kBG2RowStart[r]; }
+    __host__ __device__ static constexpr int col(int e) { return kBG2EdgeCol[e]; }
 };
+
+// Edge of row r on base-graph column c, or -1 (compile-time).
+template <int BG>
+__host__ __device__ constexpr int find_edge_ct(int r, int c) {
+    for (int e = Graph<BG>::start(r); e < Graph<BG>::start(r + 1); ++e)
+        if (Graph<BG>::col(e) == c) return e;
+    return -1;
+}
 
 }  // namespace nrldpc

@@ -19,6 +19,7 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
 extern "C" int nrldpc_bp_kernel_enabled();
+extern "C" hipError_t nrldpc_launch_synth(int bgn, const SynthArgs *a, hipStream_t s);
 extern "C" hipError_t nrldpc_launch_tiles(const DecOut *o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams *p, int ntiles,
                                           int lds_entries, hipStream_t stream);
@@ -568,6 +569,50 @@ extern "C" void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *b) {
     if (b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->d_stage) (void)hipFree(b->d_stage);
     delete b;
+}
+
+// ---------------------------------------------------------------------------
+// Device input synthesis (one config, n codeblocks, one launch)
+// ---------------------------------------------------------------------------
+extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t n,
+                                           const float *noise_power_db, uint64_t seed,
+                                           const uint8_t *payload_bits, uint8_t *bits_out,
+                                           uint8_t *codeword_out, uint8_t *rm_bits_out,
+                                           int16_t *llr_out, float *llr_f32_out,
+                                           float *beliefs_out, void *stream) {
+    if (!cfg || n < 0) return -4;
+    Params p;
+    if (const char *why = derive(*cfg, p)) {
+        print_invalid(*cfg, why);
+        return -2;
+    }
+    if (n == 0) return 0;
+    if (!noise_power_db || (beliefs_out && !llr_f32_out)) return -4;
+    Runtime &r = rt();
+    if (r.ensure() != 0) return -1;
+    SynthArgs a{};
+    a.noise_db = noise_power_db;
+    a.payload = payload_bits;
+    a.bits_out = bits_out;
+    a.cw_out = codeword_out;
+    a.rm_out = rm_bits_out;
+    a.llr_out = llr_out;
+    a.llr_f32 = llr_f32_out;
+    a.beliefs = beliefs_out;
+    a.etab = r.d_etab + r.etab_off[p.bgn - 1][p.Zc];
+    a.seed = seed;
+    a.n = n;
+    a.A = p.A;
+    a.B = p.B;
+    a.K = p.K;
+    a.N = p.N;
+    a.E = p.E;
+    a.Zc = p.Zc;
+    a.Qm = p.Qm;
+    a.k0 = p.k0;
+    a.crctype = p.crctype;
+    HIP_OK(nrldpc_launch_synth(p.bgn, &a, reinterpret_cast<hipStream_t>(stream)));
+    return 0;
 }
 
 extern "C" int32_t ldpc_mi355_sync(void) {

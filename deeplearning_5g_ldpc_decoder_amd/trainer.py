@@ -8,7 +8,9 @@ supervision every ``intermediate_loss_stride`` iterations, float64 global
 gradient-norm clipping to 1.0, Adam with cosine annealing to 0.05 * lr, and
 the best-loss state kept on the host.  The unrolled decode and its gradient
 are the library's HIP kernels (see ``unrolled.py``); training inputs are
-synthesized on host threads by the library (``ldpc_mi355_synth_float_batch``).
+synthesized on the GPU (``ldpc_mi355_synth_device``: payload, CRC, encoding,
+rate matching, channel and the float rate recovery of the reference's
+training data), or on host threads with ``on_device=False``.
 
 New here: ``per_codeblock_noise`` draws one noise power per codeblock from
 ``training_noise_power_db_range`` (the reference draws one per batch).
@@ -23,17 +25,21 @@ from pathlib import Path
 
 import numpy as np
 
-from .unrolled import CTableNeuralLayeredMinSumDecoder, load_ldpc_table_config, synth_float_batch
+from .unrolled import (CTableNeuralLayeredMinSumDecoder, load_ldpc_table_config,
+                       synth_float_batch, synth_float_batch_device)
 
 CRC_TYPES = (0, 1, 2, 3)
 
 
 def make_c_table_training_batch(model, batch_size, noise_power_db, device, seed=0,
-                                threads=16):
+                                threads=16, on_device=True):
     """(llr [batch, n] float32, target [batch, B] float32) on ``device``.
-    noise_power_db: a scalar or one value per codeblock."""
+    noise_power_db: a scalar or one value per codeblock.  on_device: synthesize
+    on the GPU (ldpc_mi355_synth_device); otherwise on host threads."""
     import torch
     nz = np.broadcast_to(np.asarray(noise_power_db, np.float32), (batch_size,))
+    if on_device:
+        return synth_float_batch_device(model.config, nz, seed=seed)
     llr, bits = synth_float_batch(model.config, nz, seed=seed, threads=threads)
     return (torch.from_numpy(llr).to(device, non_blocking=True),
             torch.from_numpy(bits.astype(np.float32)).to(device, non_blocking=True))

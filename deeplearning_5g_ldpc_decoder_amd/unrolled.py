@@ -589,3 +589,16 @@ def synth_float_batch(table_config, noise_power_db, seed=0, threads=8):
     if st != 0:
         raise ValueError("invalid config")
     return llr, bits
+
+
+def synth_float_batch_device(table_config, noise_power_db, seed=0):
+    """Device synthesis of training inputs (ldpc_mi355_synth_device): float
+    beliefs [n, N+2Zc] and target bits [n, B] as float32 device tensors, for
+    per-codeblock noise powers (a sequence of n values)."""
+    import torch
+    from .batch import synth_device
+    c = table_config
+    nz = np.ascontiguousarray(noise_power_db, np.float32).reshape(-1)
+    out = synth_device((c.bgn, c.A, c.crctype, c.E, c.rv, c.Qm, c.HARQ_on), nz.size, nz,
+                       seed=seed, outputs=("bits", "beliefs"))
+    return out["beliefs"], out["bits"].to(torch.float32)

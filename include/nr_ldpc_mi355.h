@@ -157,6 +157,23 @@ int32_t ldpc_mi355_synth_float_batch(const ldpc_in_config_t *cfg, int32_t n,
                                      const float *noise_power_db, uint64_t seed, int32_t threads,
                                      float *llr_out, uint8_t *bits_out);
 
+/* The same inputs generated on the device (no host round trip): n codeblocks
+ * of one config, one workgroup each -- payload (the caller's payload_bits,
+ * n*A bits one per byte, or a counter-based generator keyed by seed), CRC,
+ * LDPC encoding, rate matching, Gray-QAM AWGN channel at noise_power_db[i]
+ * (device array of n floats), max-log LLRs.  Every output is optional
+ * (device pointers, may be NULL): bits_out n*B payload+CRC bits,
+ * codeword_out n*(N+2Zc) bits, rm_bits_out n*E rate-matched bits, llr_out
+ * n*E int16 LLRs (x128, rounded, clipped to +-10000, the c_ldpc_5g_decode
+ * input), llr_f32_out n*E float LLRs, beliefs_out n*(N+2Zc) float beliefs in
+ * the trainable decoder's layout (needs llr_f32_out).  Asynchronous on
+ * `stream`.  0, -2 invalid config, -4 invalid argument, -1 GPU error. */
+int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t n,
+                                const float *noise_power_db, uint64_t seed,
+                                const uint8_t *payload_bits, uint8_t *bits_out,
+                                uint8_t *codeword_out, uint8_t *rm_bits_out, int16_t *llr_out,
+                                float *llr_f32_out, float *beliefs_out, void *stream);
+
 /* One codeblock of a batch. */
 typedef struct {
     ldpc_in_config_t cfg;  /* HARQ_on: combine into the soft buffer slot */

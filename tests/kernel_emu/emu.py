@@ -27,6 +27,8 @@ def lib():
         L.emu_unrolled_record_bytes.restype = ctypes.c_long
         L.emu_unrolled_backward.argtypes = [I, I, I, I, I, F, I, I, I] + [P] * 9
         L.emu_unrolled_backward.restype = I
+        L.emu_synth.argtypes = [I] * 7 + [P, ctypes.c_uint64] + [P] * 7
+        L.emu_synth.restype = I
         _lib = L
     return _lib
 
@@ -113,4 +115,27 @@ def unrolled(cfg, T, llr, alpha, beta, clip=None, detach=False, cbs=0, g_lq=None
                                      _p(gb))
         assert rc == 0, rc
         out.update(g_llr=g_llr, g_alpha=ga, g_beta=gb)
+    return out
+
+
+def synth(cfg, n, noise_db, seed=0, payload=None, beliefs=False):
+    """Input-synthesis kernel on the CPU.  cfg = (bgn, A, crctype, E, rv, Qm).
+    Returns dict(bits, cw, rm, llr, llr_f32[, beliefs])."""
+    from oracle import oracle as O
+    bgn, A, crctype, E, rv, Qm = cfg[:6]
+    st, p = O.params(bgn, A, crctype, E, rv, Qm, 0)
+    assert st == 0
+    B, nt = p["B"], p["N"] + 2 * p["Zc"]
+    nz = np.ascontiguousarray(np.broadcast_to(np.float32(noise_db), (n,)), np.float32)
+    pay = None if payload is None else np.ascontiguousarray(payload, np.uint8)
+    out = dict(bits=np.zeros((n, B), np.uint8), cw=np.zeros((n, nt), np.uint8),
+               rm=np.zeros((n, E), np.uint8), llr=np.zeros((n, E), np.int16),
+               llr_f32=np.zeros((n, E), np.float32))
+    bel = np.zeros((n, nt), np.float32) if beliefs else None
+    rc = lib().emu_synth(bgn, A, crctype, E, rv, Qm, n, _p(nz), ctypes.c_uint64(seed), _p(pay),
+                         _p(out["bits"]), _p(out["cw"]), _p(out["rm"]), _p(out["llr"]),
+                         _p(out["llr_f32"]), _p(bel))
+    assert rc == 0, rc
+    if beliefs:
+        out["beliefs"] = bel
     return out
