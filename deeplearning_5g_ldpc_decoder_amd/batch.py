@@ -11,7 +11,7 @@ import numpy as np
 from . import _lib
 from ._lib import CbDescriptor, DecoderConfig, LdpcInConfig, ptr
 
-ALGO_LQ_DTYPE = {1: "int16", 0: "float32", 4: "float32"}
+ALGO_LQ_DTYPE = {1: "int16", 2: "int16", 3: "int16", 0: "float32", 4: "float32"}
 
 
 def soft_slot_elems():

@@ -42,6 +42,14 @@ struct DecParams {
     int32_t filler;             // filler-bit LLR (int16)
     float falpha, fbeta;        // algo 4: alpha/32768, beta/128
     float fmax, fmin;           // algo 4: max_llr/128, min_llr/128
+    // int16 algos: 1 when some row's scaled minimum can wrap to int16 -32768
+    // (or alpha < 0, or min_llr > max_llr), which needs the exact row form
+    int32_t general;
+    // algos 2/3: per (iteration, row) scale words of base graph 1 / 2,
+    // alpha (int16) | beta (int16) << 16, iteration t at t * rows; iterations
+    // at or past scale_iters use the last one
+    int32_t scale_iters;
+    const uint32_t *scale[2];
 };
 
 // A packed tile: `count` codeblocks of one (BG, Zc) share one workgroup of

@@ -143,8 +143,15 @@ void print_invalid(const ldpc_in_config_t &in, const char *why) {
             in.crctype, in.E, in.rv, in.Qm, in.HARQ_on);
 }
 
+bool i16_fast_ok(int alpha, int beta, int max_llr, int min_llr) {
+    if (alpha < 0 || min_llr > max_llr) return false;
+    const int64_t off = std::max(max_llr - beta, 0);
+    return (((int64_t)alpha * off + (1 << 14)) >> 15) <= 32767;
+}
+
 DecParams decoder_params(const ldpc_decoder_cfg_t &c) {
     DecParams p;
+    memset(&p, 0, sizeof p);
     p.max_it = c.max_iterations;
     p.alpha = (int16_t)c.alpha;   // the int16 view the reference's helpers take
     p.beta = (int16_t)c.beta;
@@ -155,7 +162,146 @@ DecParams decoder_params(const ldpc_decoder_cfg_t &c) {
     p.fbeta = (float)c.beta / 128.0f;
     p.fmax = (float)c.max_llr / 128.0f;
     p.fmin = (float)c.min_llr / 128.0f;
+    p.general = i16_fast_ok(p.alpha, p.beta, p.max_llr, p.min_llr) ? 0 : 1;
     return p;
+}
+
+// ---------------------------------------------------------------------------
+// Algo-2 / algo-3 scale tables.  The reference compiles its trained tables
+// (c_lib/BG1_trained_ldpc_weights.c, BG2_...) and its degree alpha defaults
+// (c_lib/degree_fixedpoint_ldpc_decoder.c:19-43) into the library; here they
+// are process-wide state with the same defaults, replaceable at run time.
+// The defaults are generated from the rule both reference files state: Q15
+// alpha 26623 for check degree <= 4, 24575 for 5..8, 22527 above 8; trained
+// tables repeat it over 32 iterations with beta 0.
+// ---------------------------------------------------------------------------
+namespace {
+int16_t rule_alpha(int degree) {
+    return degree <= 4 ? 26623 : (degree <= 8 ? 24575 : 22527);
+}
+
+int row_degree(int bgn, int r) {
+    const uint16_t *rs = bgn == 1 ? kBG1RowStart : kBG2RowStart;
+    return rs[r + 1] - rs[r];
+}
+
+int bg_rows(int bgn) { return bgn == 1 ? NR_BG1_ROWS : NR_BG2_ROWS; }
+
+struct TableState {
+    std::mutex mu;
+    std::vector<int16_t> alpha[2], beta[2];   // [iterations][rows]
+    int iters[2] = {0, 0};
+    int16_t degree_alpha[kMaxScaleDegree];
+
+    TableState() {
+        reset_trained(1);
+        reset_trained(2);
+        reset_degree();
+    }
+    void reset_trained(int bgn) {
+        const int rows = bg_rows(bgn);
+        iters[bgn - 1] = kDefaultTrainedIterations;
+        alpha[bgn - 1].assign((size_t)iters[bgn - 1] * rows, 0);
+        beta[bgn - 1].assign((size_t)iters[bgn - 1] * rows, 0);
+        for (int t = 0; t < iters[bgn - 1]; ++t)
+            for (int r = 0; r < rows; ++r)
+                alpha[bgn - 1][(size_t)t * rows + r] = rule_alpha(row_degree(bgn, r));
+    }
+    void reset_degree() {
+        for (int d = 1; d <= kMaxScaleDegree; ++d) degree_alpha[d - 1] = rule_alpha(d);
+    }
+};
+
+TableState &tables() {
+    static TableState t;
+    return t;
+}
+
+uint32_t scale_word(int alpha, int beta) {
+    return (uint32_t)(uint16_t)alpha | ((uint32_t)(uint16_t)beta << 16);
+}
+}  // namespace
+
+int set_trained_tables(int bgn, const int16_t *alpha, const int16_t *beta, int iterations) {
+    if (bgn != 1 && bgn != 2) return -4;
+    TableState &ts = tables();
+    std::lock_guard<std::mutex> g(ts.mu);
+    if (!alpha) {
+        ts.reset_trained(bgn);
+        return 0;
+    }
+    if (!beta || iterations < 1 || iterations > kMaxTrainedIterations) return -4;
+    const size_t n = (size_t)iterations * bg_rows(bgn);
+    ts.alpha[bgn - 1].assign(alpha, alpha + n);
+    ts.beta[bgn - 1].assign(beta, beta + n);
+    ts.iters[bgn - 1] = iterations;
+    return 0;
+}
+
+int get_trained_tables(int bgn, int16_t *alpha, int16_t *beta, int capacity) {
+    if (bgn != 1 && bgn != 2) return -4;
+    TableState &ts = tables();
+    std::lock_guard<std::mutex> g(ts.mu);
+    const int it = ts.iters[bgn - 1];
+    const size_t n = (size_t)std::min(std::max(capacity, 0), it) * bg_rows(bgn);
+    if (alpha) std::copy(ts.alpha[bgn - 1].begin(), ts.alpha[bgn - 1].begin() + n, alpha);
+    if (beta) std::copy(ts.beta[bgn - 1].begin(), ts.beta[bgn - 1].begin() + n, beta);
+    return it;
+}
+
+void set_degree_alpha(const int16_t *values) {
+    TableState &ts = tables();
+    std::lock_guard<std::mutex> g(ts.mu);
+    if (values)
+        std::copy(values, values + kMaxScaleDegree, ts.degree_alpha);
+    else
+        ts.reset_degree();
+}
+
+void get_degree_alpha(int16_t *values) {
+    TableState &ts = tables();
+    std::lock_guard<std::mutex> g(ts.mu);
+    std::copy(ts.degree_alpha, ts.degree_alpha + kMaxScaleDegree, values);
+}
+
+void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s) {
+    TableState &ts = tables();
+    std::lock_guard<std::mutex> g(ts.mu);
+    const int beta_row = (int16_t)c.beta;
+    s.algo = c.algo;
+    s.words.clear();
+    s.fast = true;
+    // algo 2 reads iteration t's words for t < max_iterations; algo 3 one set
+    s.iters = c.algo == 2 ? std::max((int)c.max_iterations, 1) : 1;
+    for (int bgn = 1; bgn <= 2; ++bgn) {
+        const int rows = bg_rows(bgn);
+        s.off[bgn - 1] = (uint32_t)s.words.size();
+        s.table_iters[bgn - 1] = ts.iters[bgn - 1];
+        for (int t = 0; t < s.iters; ++t)
+            for (int r = 0; r < rows; ++r) {
+                int a, b;
+                if (c.algo == 2) {
+                    // rows past a table that is too short are never read: a
+                    // decode whose max_iterations exceeds its base graph's
+                    // table is refused
+                    const int tt = std::min(t, ts.iters[bgn - 1] - 1);
+                    a = ts.alpha[bgn - 1][(size_t)tt * rows + r];
+                    b = ts.beta[bgn - 1][(size_t)tt * rows + r];
+                } else {
+                    // exact degree d at entry d-1; the BG1/BG2 degrees are
+                    // 3..19, inside the table
+                    // (c_lib/degree_fixedpoint_ldpc_decoder.c:65-80)
+                    a = ts.degree_alpha[row_degree(bgn, r) - 1];
+                    b = beta_row;
+                }
+                s.words.push_back(scale_word(a, b));
+                if (!i16_fast_ok(a, b, c.max_llr, c.min_llr)) s.fast = false;
+            }
+    }
+}
+
+bool scale_iterations_ok(const ScaleSet &s, int bgn, int max_iterations) {
+    return s.algo != 2 || max_iterations <= s.table_iters[bgn - 1];
 }
 
 CbDesc codeblock_desc(const Params &p, uint32_t etab_off) {

@@ -32,7 +32,42 @@ void rate_match_bits(const Params &p, const uint8_t *cw, uint8_t *out_bits);
 
 // Kernel-side views of a decoder row and of one codeblock (ldpc_device.h).
 // The descriptor's buffer offsets are set to "none"; callers fill them.
+// decoder_params leaves the scale tables of algos 2/3 unset (see ScaleSet).
 DecParams decoder_params(const ldpc_decoder_cfg_t &c);
+
+// True when the int16 fast row form is exact for the pair (alpha, beta)
+// under the clip bounds: alpha >= 0, min_llr <= max_llr and every scaled
+// minimum (alpha * max(m - beta, 0) + 2^14) >> 15 with m <= max_llr stays
+// within [0, 32767], so no message wraps to int16 -32768.
+bool i16_fast_ok(int alpha, int beta, int max_llr, int min_llr);
+
+// Algo-2 trained tables ([iterations][rows] int16 alpha, beta per base
+// graph) and algo-3 degree alpha (kMaxScaleDegree Q15 values, entry d-1 for
+// check degree d): process-wide, defaults as the reference ships.
+constexpr int kMaxScaleDegree = 19;
+constexpr int kDefaultTrainedIterations = 32;
+constexpr int kMaxTrainedIterations = 1024;
+int set_trained_tables(int bgn, const int16_t *alpha, const int16_t *beta, int iterations);
+int get_trained_tables(int bgn, int16_t *alpha, int16_t *beta, int capacity);
+void set_degree_alpha(const int16_t *values);
+void get_degree_alpha(int16_t *values);
+
+// The scale words a decode of algo 2 or 3 reads (DecParams::scale): base
+// graph 1's iters * 46 words at off[0], base graph 2's iters * 42 at off[1],
+// built from the tables current when it is made.  fast: every word passes
+// i16_fast_ok under the row's clip bounds.
+struct ScaleSet {
+    int algo = 0;
+    std::vector<uint32_t> words;
+    uint32_t off[2] = {0, 0};
+    int iters = 0;
+    int table_iters[2] = {0, 0};
+    bool fast = true;
+};
+void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s);
+// Algo 2 decodes at most as many iterations as its base graph's table holds
+// (the reference reads past the table); true for other algos.
+bool scale_iterations_ok(const ScaleSet &s, int bgn, int max_iterations);
 CbDesc codeblock_desc(const Params &p, uint32_t etab_off);
 // Groups codeblock descriptors by (BG, Zc) into packed tiles of up to
 // k_of(bgn, zc) codeblocks; order receives descriptor indices tile by tile,

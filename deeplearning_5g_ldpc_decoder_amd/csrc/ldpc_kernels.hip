@@ -141,12 +141,12 @@ __device__ __forceinline__ uint32_t push_sign(uint32_t acc, int v) {
 // (alpha * max(m - beta, 0) + 2^14) >> 15 as int16 (c_lib/fixedpoint_ldpc_decoder.c:38-46).
 // alpha is in [-32768, 32767] and the offset in [0, 65535], so a 24-bit
 // multiply is exact and the product fits in 32 bits.
-__device__ inline int scale_i16(int m, const DecParams &p) {
-    const int off = max(m - p.beta, 0);
+__device__ inline int scale_i16(int m, int alpha, int beta) {
+    const int off = max(m - beta, 0);
 #ifdef NRLDPC_CPU_EMU
-    const int scaled = p.alpha * off;
+    const int scaled = alpha * off;
 #else
-    const int scaled = __mul24(p.alpha, off);
+    const int scaled = __mul24(alpha, off);
 #endif
     return wrap16((scaled + (1 << 14)) >> 15);
 }
@@ -164,15 +164,18 @@ constexpr uint32_t kKeyMax = (32768u << 5) | 31u;
 // one of the two smallest magnitudes both clipped minima are equal, so the
 // messages are the same; likewise when every magnitude is 32767 (reference
 // argmin -1).  Sign bits are kept in reverse edge order (edge j at bit
-// deg-1-j).  kGeneral = false is the fast path for decoder rows with
-// alpha >= 0 (as int16) and min_llr <= max_llr: new messages then never reach
-// -32768 and the LQ clip is a plain clamp; true is exact for every row
-// (message wrapped to int16 before the add, clip as the reference orders its
-// two tests).
+// deg-1-j).  kGeneral = false is the fast path for decoder rows whose
+// scaled minima stay in [0, 32767] (alpha >= 0 and
+// (alpha * max(max_llr - beta, 0) + 2^14) >> 15 <= 32767) and whose
+// min_llr <= max_llr: messages are then plain negations and the LQ clip is a
+// clamp; true is exact for every row (message wrapped to int16 before the
+// add, clip as the reference orders its two tests).  The host decides
+// (DecParams::general).  (alpha, beta) is the row's pair: the decoder row's
+// for algo 1, the table's for algos 2 and 3.
 template <int BG, int R, bool kGeneral>
 __device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &st,
                                         const uint32_t *__restrict__ etab, int zx,
-                                        int Zc, const DecParams &p) {
+                                        int Zc, const DecParams &p, int alpha, int beta) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     const uint32_t mag = st.mag[R], sb = st.sgn[R];
@@ -198,7 +201,7 @@ __device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &
     }
     const int idx = (int)(k1 & 31u);
     const int m1 = min((int)(k1 >> 5), p.max_llr), m2 = min((int)(k2 >> 5), p.max_llr);
-    const int n1 = scale_i16(m1, p), n2 = scale_i16(m2, p);
+    const int n1 = scale_i16(m1, alpha, beta), n2 = scale_i16(m2, alpha, beta);
     constexpr uint32_t mask = (1u << deg) - 1u;
     const uint32_t bits = ((__popc(negs) & 1) ? ~negs : negs) & mask;
 #pragma unroll
@@ -219,15 +222,34 @@ __device__ __forceinline__ void row_i16(int16_t *__restrict__ lq, StateI16<BG> &
 }
 
 // All rows of one iteration; lanes with run = false only take part in the
-// barriers.
-template <int BG, int R, bool kGeneral>
+// barriers.  kTable: row R's (alpha, beta) is the scale word ab[R] (algos 2
+// and 3, c_lib/neural_fixedpoint_ldpc_decoder.c:248-250,
+// c_lib/degree_fixedpoint_ldpc_decoder.c:230-231); ab is uniform, so the
+// word is a scalar load.
+template <int BG, int R, bool kGeneral, bool kTable>
 __device__ __forceinline__ void rows_i16(int16_t *lq, StateI16<BG> &st, const uint32_t *etab,
-                                         int zx, int Zc, bool run, const DecParams &p) {
+                                         int zx, int Zc, bool run, const DecParams &p,
+                                         const uint32_t *ab) {
     if constexpr (R < Graph<BG>::kRows) {
-        if (run) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p);
+        int alpha = p.alpha, beta = p.beta;
+        if constexpr (kTable) {
+            const uint32_t w = ab[R];
+            alpha = (int)(int16_t)(w & 0xFFFFu);
+            beta = (int)(int16_t)(w >> 16);
+        }
+        if (run) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p, alpha, beta);
         __syncthreads();
-        rows_i16<BG, R + 1, kGeneral>(lq, st, etab, zx, Zc, run, p);
+        rows_i16<BG, R + 1, kGeneral, kTable>(lq, st, etab, zx, Zc, run, p, ab);
     }
+}
+
+// Scale words of iteration it (null unless kTable).
+template <int BG, bool kTable>
+__device__ __forceinline__ const uint32_t *scale_row(const DecParams &p, int it) {
+    if constexpr (kTable)
+        return p.scale[BG - 1] + min(it, p.scale_iters - 1) * Graph<BG>::kRows;
+    else
+        return nullptr;
 }
 
 template <int BG>
@@ -239,7 +261,7 @@ __device__ __forceinline__ void init_state(StateI16<BG> &st) {
     }
 }
 
-template <int BG, bool kGeneral>
+template <int BG, bool kGeneral, bool kTable>
 __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
                        const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
     StateI16<BG> st;
@@ -248,7 +270,8 @@ __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, d.Zc, zx < d.Zc, p);
+        rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, d.Zc, zx < d.Zc, p,
+                                          scale_row<BG, kTable>(p, it));
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -463,9 +486,10 @@ template <typename T> __device__ inline T from_soft(int16_t v);
 template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
 template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
 
-// kGeneral selects the exact-for-every-row int16 path (see row_i16); the
-// other algorithms ignore it.
-template <Algo A, typename T, bool kGeneral = false>
+// kGeneral selects the exact-for-every-row int16 path (see row_i16) and
+// kTable the per-row scale words of algos 2/3; the other algorithms ignore
+// both.
+template <Algo A, typename T, bool kGeneral = false, bool kTable = false>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_decode_kernel(DecOut o, DecParams p) {
     NRLDPC_DYNAMIC_LDS(smem);
@@ -493,8 +517,8 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     if (p.max_it > 0) {
         if constexpr (A == Algo::kMinSumI16) {
             it = (d.bgn == 1)
-                     ? run_i16<1, kGeneral>(lq, etab, d, p, crc, crc_tab, slots, &status)
-                     : run_i16<2, kGeneral>(lq, etab, d, p, crc, crc_tab, slots, &status);
+                     ? run_i16<1, kGeneral, kTable>(lq, etab, d, p, crc, crc_tab, slots, &status)
+                     : run_i16<2, kGeneral, kTable>(lq, etab, d, p, crc, crc_tab, slots, &status);
         } else if constexpr (A == Algo::kMinSumF32) {
             it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
@@ -527,7 +551,7 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
 // of two banks (bank = iteration parity); the bank not in use is cleared
 // right after each CRC barrier, when every lane has read it.
 // ---------------------------------------------------------------------------
-template <int BG, bool kGeneral>
+template <int BG, bool kGeneral, bool kTable>
 __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int cbl,
                              bool member, const CrcCtx &crc, const uint32_t *tab,
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
@@ -538,7 +562,8 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
     bool tile_active = __syncthreads_or(member);
     const Crc &c = crc.spec;
     for (int it = 0; it < p.max_it && tile_active; ++it) {
-        rows_i16<BG, 0, kGeneral>(lq, st, etab, zx, Zc, !done, p);
+        rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, Zc, !done, p,
+                                          scale_row<BG, kTable>(p, it));
         const int b = it & 1;
         if (!done) {
             uint32_t r = 0;
@@ -561,7 +586,7 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
     }
 }
 
-template <bool kGeneral>
+template <bool kGeneral, bool kTable>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p) {
     NRLDPC_DYNAMIC_LDS(smem);
@@ -611,11 +636,11 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     const uint32_t *tab = crc_tab[member ? d.crctype : 0];
     if (p.max_it > 0) {
         if (td.bgn == 1)
-            run_tile_i16<1, kGeneral>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p, &status,
-                                      &iters);
+            run_tile_i16<1, kGeneral, kTable>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p,
+                                              &status, &iters);
         else
-            run_tile_i16<2, kGeneral>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p, &status,
-                                      &iters);
+            run_tile_i16<2, kGeneral, kTable>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p,
+                                              &status, &iters);
     }
     if (!member) return;
     if (p.max_it > 0)
@@ -633,50 +658,77 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
 }  // namespace nrldpc
 
 #ifndef NRLDPC_CPU_EMU
-// Host-callable launchers (C linkage inside the library only).
+namespace {
+using namespace nrldpc;
+
+// int16 decode kernel of (general, table).
+const void *i16_kernel(bool general, bool table) {
+    if (table)
+        return general ? reinterpret_cast<const void *>(
+                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, true>)
+                       : reinterpret_cast<const void *>(
+                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, true>);
+    return general ? reinterpret_cast<const void *>(
+                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, false>)
+                   : reinterpret_cast<const void *>(
+                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>);
+}
+
+const void *tile_kernel(bool general, bool table) {
+    if (table)
+        return general ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, true>)
+                       : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, true>);
+    return general ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, false>)
+                   : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, false>);
+}
+}  // namespace
+
+// Host-callable launchers (C linkage inside the library only).  Algos 1-3
+// share the int16 kernels (2 and 3 with per-row scale words).
 extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
                                            const nrldpc::DecParams *p, int ncb, int threads,
                                            int max_soft, hipStream_t stream) {
     using namespace nrldpc;
-    const size_t esz = (algo == 1) ? sizeof(int16_t) : sizeof(float);
+    const bool i16 = algo >= 1 && algo <= 3;
+    const size_t esz = i16 ? sizeof(int16_t) : sizeof(float);
     const size_t lds = ((size_t)max_soft * esz + 15) & ~(size_t)15;
     const dim3 grid(ncb), block(threads);
+    DecOut ov = *o;
+    DecParams pv = *p;
+    void *args[] = {&ov, &pv};
+    const void *fn = nullptr;
     switch (algo) {
     case 1:
-        if (p->alpha >= 0 && p->min_llr <= p->max_llr)
-            hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>), grid, block,
-                               lds, stream, *o, *p);
-        else
-            hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>), grid, block,
-                               lds, stream, *o, *p);
+    case 2:
+    case 3:
+        if (algo != 1 && (pv.scale_iters < 1 || !pv.scale[0] || !pv.scale[1]))
+            return hipErrorInvalidValue;
+        fn = i16_kernel(pv.general != 0, algo != 1);
         break;
     case 4:
-        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kMinSumF32, float>), grid, block, lds,
-                           stream, *o, *p);
+        fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>);
         break;
 #if NRLDPC_BP_KERNEL
     case 0:
-        hipLaunchKernelGGL((ldpc_decode_kernel<Algo::kBpF32, float>), grid, block, lds, stream,
-                           *o, *p);
+        fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>);
         break;
 #endif
     default:
         return hipErrorInvalidValue;
     }
+    const hipError_t e = hipLaunchKernel(fn, grid, block, args, lds, stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
 extern "C" hipError_t nrldpc_configure_kernels() {
     using namespace nrldpc;
     const int max_lds = 160 * 1024 - 2048;
-    hipError_t e;
-    e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    hipError_t e = hipSuccess;
+    for (int g = 0; g < 2 && e == hipSuccess; ++g)
+        for (int t = 0; t < 2 && e == hipSuccess; ++t)
+            e = hipFuncSetAttribute(i16_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    max_lds);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
@@ -691,28 +743,32 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     // tiles hold at most 384 lanes' worth of beliefs (~56 KB) next to 5.6 KB
     // of static LDS
     const int tile_lds = 96 * 1024;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ldpc_tile_kernel<false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&ldpc_tile_kernel<true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
+    for (int g = 0; g < 2 && e == hipSuccess; ++g)
+        for (int t = 0; t < 2 && e == hipSuccess; ++t)
+            e = hipFuncSetAttribute(tile_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    tile_lds);
+    return e;
 }
 
 extern "C" int nrldpc_bp_kernel_enabled() { return NRLDPC_BP_KERNEL; }
 
-// Packed-tile launch (algo 1): ntiles workgroups of kMaxThreads lanes.
-extern "C" hipError_t nrldpc_launch_tiles(const nrldpc::DecOut *o, const nrldpc::TileDesc *tiles,
-                                          const uint32_t *order, const nrldpc::DecParams *p,
-                                          int ntiles, int lds_entries, hipStream_t stream) {
+// Packed-tile launch (algos 1-3): ntiles workgroups of kMaxThreads lanes.
+extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
+                                          const nrldpc::TileDesc *tiles, const uint32_t *order,
+                                          const nrldpc::DecParams *p, int ntiles,
+                                          int lds_entries, hipStream_t stream) {
     using namespace nrldpc;
+    if (algo < 1 || algo > 3) return hipErrorInvalidValue;
+    if (algo != 1 && (p->scale_iters < 1 || !p->scale[0] || !p->scale[1]))
+        return hipErrorInvalidValue;
     const size_t lds = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
     const dim3 grid(ntiles), block(kMaxThreads);
-    if (p->alpha >= 0 && p->min_llr <= p->max_llr)
-        hipLaunchKernelGGL((ldpc_tile_kernel<false>), grid, block, lds, stream, *o, tiles, order,
-                           *p);
-    else
-        hipLaunchKernelGGL((ldpc_tile_kernel<true>), grid, block, lds, stream, *o, tiles, order,
-                           *p);
+    DecOut ov = *o;
+    DecParams pv = *p;
+    void *args[] = {&ov, &tiles, &order, &pv};
+    const hipError_t e =
+        hipLaunchKernel(tile_kernel(pv.general != 0, algo != 1), grid, block, args, lds, stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 #endif  // NRLDPC_CPU_EMU

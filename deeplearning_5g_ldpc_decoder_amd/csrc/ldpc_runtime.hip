@@ -20,7 +20,7 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
 extern "C" hipError_t nrldpc_configure_kernels();
 extern "C" int nrldpc_bp_kernel_enabled();
 extern "C" hipError_t nrldpc_launch_synth(int bgn, const SynthArgs *a, hipStream_t s);
-extern "C" hipError_t nrldpc_launch_tiles(const DecOut *o, const TileDesc *tiles,
+extern "C" hipError_t nrldpc_launch_tiles(int algo, const DecOut *o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams *p, int ntiles,
                                           int lds_entries, hipStream_t stream);
 
@@ -41,11 +41,14 @@ constexpr int kHardBytes = 8448 / 8 + 16;         // B/8 max, padded
 constexpr size_t kSoftBytes = (size_t)kMaxSoft * 2;
 
 // Per-codeblock staging layouts (one H2D and one D2H copy per call).
+constexpr size_t kScaleBytes =
+    (size_t)kMaxTrainedIterations * (BG1_NUM_ROWS + BG2_NUM_ROWS) * sizeof(uint32_t);
 struct InLayout {
     static constexpr size_t desc = 0;
     static constexpr size_t llr = 64;
     static constexpr size_t soft = llr + 65536 * 2;
-    static constexpr size_t total = soft + kSoftBytes;
+    static constexpr size_t scale = soft + kSoftBytes;
+    static constexpr size_t total = scale + kScaleBytes;
 };
 struct OutLayout {
     static constexpr size_t status = 0;
@@ -105,7 +108,26 @@ Runtime &rt() {
     return r;
 }
 
-bool algo_supported(int algo) { return algo == 0 || algo == 1 || algo == 4; }
+bool algo_supported(int algo) { return algo >= 0 && algo <= 4; }
+
+// Algos 1-3 decode in int16 (posteriors int16); 0 and 4 in FP32.
+bool i16_algo(int algo) { return algo >= 1 && algo <= 3; }
+bool table_algo(int algo) { return algo == 2 || algo == 3; }
+
+void report_table_iterations(int bgn, int max_it, int table_iters) {
+    fprintf(stderr,
+            "libnrldpc_mi355: algo 2: max_iterations %d exceeds the %d iterations of the BG%d "
+            "trained table\n",
+            max_it, table_iters, bgn);
+}
+
+// Points the kernel parameters at scale words uploaded to dev (algos 2/3).
+void bind_scale(DecParams &dp, const ScaleSet &s, const uint32_t *dev) {
+    dp.scale[0] = dev + s.off[0];
+    dp.scale[1] = dev + s.off[1];
+    dp.scale_iters = s.iters;
+    dp.general = s.fast ? 0 : 1;
+}
 
 // Algo 0 is part of the ABI but its GPU kernel is compiled out of this build
 // (see ldpc_kernels.hip); calls fail loudly rather than falling back.
@@ -145,6 +167,15 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     const bool valid_algo = algo_supported(decoder_config->algo);
     const bool runnable = valid_algo && algo_runnable(decoder_config->algo);
     const int algo = runnable ? decoder_config->algo : 1;
+    ScaleSet scale;
+    if (table_algo(algo)) {
+        make_scale_set(*decoder_config, scale);
+        if (!scale_iterations_ok(scale, p.bgn, decoder_config->max_iterations)) {
+            report_table_iterations(p.bgn, decoder_config->max_iterations,
+                                    scale.table_iters[p.bgn - 1]);
+            return -2;
+        }
+    }
     DecParams dp = make_params(*decoder_config);
     if (!runnable) dp.max_it = 0;   // rate recovery only, like the reference
     const int n = p.N + 2 * p.Zc;
@@ -161,6 +192,13 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     const size_t in_bytes = p.harq_on ? InLayout::soft + (size_t)n * 2
                                       : InLayout::llr + (size_t)p.E * 2;
     HIP_OK(hipMemcpyAsync(r.d_in, r.h_in, in_bytes, hipMemcpyHostToDevice, r.stream));
+    if (table_algo(algo)) {
+        const size_t sb = scale.words.size() * sizeof(uint32_t);
+        memcpy(r.h_in + InLayout::scale, scale.words.data(), sb);
+        HIP_OK(hipMemcpyAsync(r.d_in + InLayout::scale, r.h_in + InLayout::scale, sb,
+                              hipMemcpyHostToDevice, r.stream));
+        bind_scale(dp, scale, reinterpret_cast<const uint32_t *>(r.d_in + InLayout::scale));
+    }
     // the kernel reads the soft buffer only when HARQ_on; it always writes it
     DecOut o;
     o.desc = reinterpret_cast<const CbDesc *>(r.d_in + InLayout::desc);
@@ -177,7 +215,7 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     HIP_OK(hipMemcpyAsync(r.h_out, r.d_out, OutLayout::soft, hipMemcpyDeviceToHost, r.stream));
     HIP_OK(hipMemcpyAsync(r.h_out + OutLayout::soft, r.d_in + InLayout::soft, (size_t)n * 2,
                           hipMemcpyDeviceToHost, r.stream));
-    const size_t lq_bytes = (size_t)n * (algo == 1 ? 2 : 4);
+    const size_t lq_bytes = (size_t)n * (i16_algo(algo) ? 2 : 4);
     if (d.lq_off != kNone)
         HIP_OK(hipMemcpyAsync(r.h_out + OutLayout::lq, r.d_out + OutLayout::lq, lq_bytes,
                               hipMemcpyDeviceToHost, r.stream));
@@ -214,6 +252,7 @@ struct ldpc_mi355_plan {
     DecParams dp;
     CbDesc *d_desc = nullptr;
     float *d_work = nullptr;
+    uint32_t *d_scale = nullptr;   // algos 2/3: scale words (ScaleSet)
     uint64_t hard_bytes = 0, lq_elems = 0;
     std::vector<uint32_t> hard_off, lq_off;
     bool want_lq = false;
@@ -262,7 +301,7 @@ struct BatchLayout {
 };
 
 int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, const Runtime &r,
-                 BatchLayout &L, int32_t *bad_index) {
+                 const ScaleSet *scale, int max_it, BatchLayout &L, int32_t *bad_index) {
     L.descs.resize((size_t)n);
     L.hard_off.resize((size_t)n);
     L.lq_off.resize((size_t)n);
@@ -273,6 +312,11 @@ int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, cons
         Params p;
         if (const char *why = derive(cbs[i].cfg, p)) {
             print_invalid(cbs[i].cfg, why);
+            if (bad_index) *bad_index = i;
+            return -2;
+        }
+        if (scale && !scale_iterations_ok(*scale, p.bgn, max_it)) {
+            report_table_iterations(p.bgn, max_it, scale->table_iters[p.bgn - 1]);
             if (bad_index) *bad_index = i;
             return -2;
         }
@@ -295,10 +339,10 @@ int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, cons
     return 0;
 }
 
-// Packed tiles for algo 1 when asked (flags bit 1) or when some lifting size
+// Packed tiles for the int16 algos when asked (flags bit 1) or when some lifting size
 // leaves room for two or more codeblocks per workgroup (bit 2 forbids it).
 bool want_packing(int algo, uint32_t flags, const std::vector<CbDesc> &descs) {
-    if (algo != 1 || (flags & 4u)) return false;
+    if (!i16_algo(algo) || (flags & 4u)) return false;
     if (flags & 2u) return true;
     for (const CbDesc &d : descs)
         if (tile_codeblocks(d.bgn, d.Zc) > 1) return true;
@@ -325,8 +369,12 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     pl->algo = dec->algo;
     pl->dp = make_params(*dec);
     pl->want_lq = (flags & 1u) != 0;
+    ScaleSet scale;
+    if (table_algo(pl->algo)) make_scale_set(*dec, scale);
     BatchLayout L;
-    if (int st = build_layout(cbs, n, pl->algo, pl->want_lq, r, L, bad_index)) {
+    if (int st = build_layout(cbs, n, pl->algo, pl->want_lq, r,
+                              table_algo(pl->algo) ? &scale : nullptr, dec->max_iterations, L,
+                              bad_index)) {
         delete pl;
         return st;
     }
@@ -343,6 +391,17 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
         fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
         ldpc_mi355_plan_destroy(pl);
         return -1;
+    }
+    if (table_algo(pl->algo)) {
+        const size_t sb = scale.words.size() * sizeof(uint32_t);
+        if (hipMalloc(&pl->d_scale, sb) != hipSuccess ||
+            hipMemcpy(pl->d_scale, scale.words.data(), sb, hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
+            ldpc_mi355_plan_destroy(pl);
+            return -1;
+        }
+        bind_scale(pl->dp, scale, pl->d_scale);
     }
     if (want_packing(pl->algo, flags, L.descs)) {
         std::vector<TileDesc> tiles;
@@ -370,7 +429,7 @@ extern "C" int32_t ldpc_mi355_plan_layout(const ldpc_mi355_plan_t *plan, uint64_
     if (!plan) return -1;
     if (hard_bytes) *hard_bytes = plan->hard_bytes;
     if (lq_elems) *lq_elems = plan->want_lq ? plan->lq_elems : 0;
-    if (lq_elem_size) *lq_elem_size = plan->algo == 1 ? 2 : 4;
+    if (lq_elem_size) *lq_elem_size = i16_algo(plan->algo) ? 2 : 4;
     if (hard_offsets) memcpy(hard_offsets, plan->hard_off.data(), 4 * (size_t)plan->n);
     if (lq_offsets) memcpy(lq_offsets, plan->lq_off.data(), 4 * (size_t)plan->n);
     return 0;
@@ -397,8 +456,8 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     // the handle PyTorch reports for its default stream.
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (plan->ntiles > 0)
-        HIP_OK(nrldpc_launch_tiles(&o, plan->d_tiles, plan->d_order, &plan->dp, plan->ntiles,
-                                   plan->tile_lds, s));
+        HIP_OK(nrldpc_launch_tiles(plan->algo, &o, plan->d_tiles, plan->d_order, &plan->dp,
+                                   plan->ntiles, plan->tile_lds, s));
     else
         HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
                                     plan->max_soft, s));
@@ -411,6 +470,7 @@ extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
     if (plan->d_work) (void)hipFree(plan->d_work);
     if (plan->d_tiles) (void)hipFree(plan->d_tiles);
     if (plan->d_order) (void)hipFree(plan->d_order);
+    if (plan->d_scale) (void)hipFree(plan->d_scale);
     delete plan;
 }
 
@@ -424,10 +484,12 @@ extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
 // is still running first waits for it.
 // ---------------------------------------------------------------------------
 struct ldpc_mi355_batch {
-    int capacity = 0, algo = 1;
+    int capacity = 0, algo = 1, max_it = 0;
     uint32_t flags = 0;
     bool want_lq = false;
     DecParams dp;
+    ScaleSet scale;                 // algos 2/3: tables current at creation
+    uint32_t *d_scale = nullptr;
     size_t stage_bytes = 0;
     unsigned char *h_stage = nullptr, *d_stage = nullptr;
     hipEvent_t done = nullptr;
@@ -458,6 +520,7 @@ extern "C" int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t c
     b->algo = dec->algo;
     b->flags = flags;
     b->want_lq = (flags & 1u) != 0;
+    b->max_it = dec->max_iterations;
     b->dp = make_params(*dec);
     b->stage_bytes = batch_stage_bytes(capacity);
     if (hipHostMalloc(&b->h_stage, b->stage_bytes, hipHostMallocDefault) != hipSuccess ||
@@ -466,6 +529,18 @@ extern "C" int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t c
         fprintf(stderr, "libnrldpc_mi355: batch context allocation failed\n");
         ldpc_mi355_batch_destroy(b);
         return -1;
+    }
+    if (table_algo(b->algo)) {
+        make_scale_set(*dec, b->scale);
+        const size_t sb = b->scale.words.size() * sizeof(uint32_t);
+        if (hipMalloc(&b->d_scale, sb) != hipSuccess ||
+            hipMemcpy(b->d_scale, b->scale.words.data(), sb, hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            fprintf(stderr, "libnrldpc_mi355: batch context allocation failed\n");
+            ldpc_mi355_batch_destroy(b);
+            return -1;
+        }
+        bind_scale(b->dp, b->scale, b->d_scale);
     }
     *batch = b;
     return 0;
@@ -488,7 +563,9 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
     BatchLayout &L = b->layout;
     b->n = 0;
     b->ntiles = 0;
-    if (int st = build_layout(cbs, n, b->algo, b->want_lq, r, L, bad_index)) return st;
+    if (int st = build_layout(cbs, n, b->algo, b->want_lq, r,
+                              table_algo(b->algo) ? &b->scale : nullptr, b->max_it, L, bad_index))
+        return st;
     const size_t doff = 0;
     const size_t toff = ((sizeof(CbDesc) * (size_t)b->capacity) + 255) & ~(size_t)255;
     const size_t ooff = toff + (((sizeof(TileDesc) * (size_t)b->capacity) + 255) & ~(size_t)255);
@@ -517,7 +594,8 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
     o.work = nullptr;
     o.etab = r.d_etab;
     if (b->ntiles > 0)
-        HIP_OK(nrldpc_launch_tiles(&o, reinterpret_cast<const TileDesc *>(b->d_stage + toff),
+        HIP_OK(nrldpc_launch_tiles(b->algo, &o,
+                                   reinterpret_cast<const TileDesc *>(b->d_stage + toff),
                                    reinterpret_cast<const uint32_t *>(b->d_stage + ooff),
                                    &b->dp, b->ntiles, tile_lds, s));
     else
@@ -568,6 +646,7 @@ extern "C" void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *b) {
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->d_stage) (void)hipFree(b->d_stage);
+    if (b->d_scale) (void)hipFree(b->d_scale);
     delete b;
 }
 
@@ -612,6 +691,30 @@ extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t 
     a.k0 = p.k0;
     a.crctype = p.crctype;
     HIP_OK(nrldpc_launch_synth(p.bgn, &a, reinterpret_cast<hipStream_t>(stream)));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Algo-2 trained tables and algo-3 degree alpha (process-wide)
+// ---------------------------------------------------------------------------
+extern "C" int32_t ldpc_mi355_set_trained_tables(int32_t bgn, const int16_t *alpha,
+                                                 const int16_t *beta, int32_t iterations) {
+    return set_trained_tables(bgn, alpha, beta, iterations);
+}
+
+extern "C" int32_t ldpc_mi355_get_trained_tables(int32_t bgn, int16_t *alpha, int16_t *beta,
+                                                 int32_t capacity_iterations) {
+    return get_trained_tables(bgn, alpha, beta, capacity_iterations);
+}
+
+extern "C" int32_t ldpc_mi355_set_degree_alpha(const int16_t *degree_alpha_values) {
+    set_degree_alpha(degree_alpha_values);
+    return 0;
+}
+
+extern "C" int32_t ldpc_mi355_get_degree_alpha(int16_t *degree_alpha_values) {
+    if (!degree_alpha_values) return -4;
+    get_degree_alpha(degree_alpha_values);
     return 0;
 }
 

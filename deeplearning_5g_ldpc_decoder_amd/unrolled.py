@@ -512,57 +512,66 @@ def _c_rows(vals, per_line=14):
     return "\n".join(lines)
 
 
-def export_algo2_tables(model, out_dir, num_iterations=ALGO2_NUM_ITERATIONS, beta_scale=128.0):
-    """Write BG<n>_trained_ldpc_weights.h/.c (the layout of
-    c_lib/BG1_trained_ldpc_weights.{h,c}) for the model's base graph.
+def write_algo2_weight_files(bgn, alpha, beta, out_dir, meta=None, note=""):
+    """Write BG<bgn>_trained_ldpc_weights.h/.c in the layout of
+    c_lib/BG1_trained_ldpc_weights.{h,c}: int16 alpha / beta tables
+    [iterations, rows] flattened iteration-major, with the NUM_ITERATIONS,
+    NUM_ROWS and ALPHA_BETA_LIST_SIZE macros and the trained-config metadata
+    macros (meta: dict with A, B, E, Zc, crctype, rv, Qm; zeros if absent).
     Returns the two paths."""
     import os
-    cfg = model.config
-    fa, fb = algo2_tables(model, num_iterations, beta_scale)
-    R = cfg.num_rows
-    tag = f"BG{cfg.bgn}_TRAINED_LDPC"
-    name = f"BG{cfg.bgn}_trained_ldpc_weights"
+    alpha = np.asarray(alpha)
+    beta = np.asarray(beta)
+    rows = BG1_NUM_ROWS if bgn == 1 else BG2_NUM_ROWS
+    if alpha.shape != beta.shape or alpha.ndim != 2 or alpha.shape[1] != rows:
+        raise ValueError(f"tables must be [iterations, {rows}]")
+    if alpha.dtype != np.int16 or beta.dtype != np.int16:
+        raise ValueError("tables must be int16")
+    iters = alpha.shape[0]
+    m = dict(A=0, B=0, E=0, Zc=0, crctype=0, rv=0, Qm=0)
+    m.update(meta or {})
+    tag = f"BG{bgn}_TRAINED_LDPC"
+    name = f"BG{bgn}_trained_ldpc_weights"
     os.makedirs(out_dir, exist_ok=True)
     h = f"""#ifndef {name.upper()}_H
 #define {name.upper()}_H
 
 #include <stdint.h>
 
-#define {tag}_BGN {cfg.bgn}
-#define {tag}_NUM_ITERATIONS {num_iterations}
-#define {tag}_NUM_ROWS {R}
+#define {tag}_BGN {bgn}
+#define {tag}_NUM_ITERATIONS {iters}
+#define {tag}_NUM_ROWS {rows}
 
 /* table_index = iteration * {tag}_NUM_ROWS + HBG_row */
-#define {tag}_ALPHA_BETA_LIST_SIZE {num_iterations * R}
+#define {tag}_ALPHA_BETA_LIST_SIZE {iters * rows}
 
 /* configuration the weights were trained for (metadata) */
-#define {tag}_A {cfg.A}
-#define {tag}_B {cfg.B}
-#define {tag}_E {cfg.E}
-#define {tag}_ZC {cfg.Zc}
-#define {tag}_CRC_TYPE {cfg.crctype}
-#define {tag}_RV {cfg.rv}
-#define {tag}_QM {cfg.Qm}
+#define {tag}_A {m['A']}
+#define {tag}_B {m['B']}
+#define {tag}_E {m['E']}
+#define {tag}_ZC {m['Zc']}
+#define {tag}_CRC_TYPE {m['crctype']}
+#define {tag}_RV {m['rv']}
+#define {tag}_QM {m['Qm']}
 
-extern const int16_t BG{cfg.bgn}_trained_alpha_list[{tag}_ALPHA_BETA_LIST_SIZE];
-extern const int16_t BG{cfg.bgn}_trained_beta_list[{tag}_ALPHA_BETA_LIST_SIZE];
+extern const int16_t BG{bgn}_trained_alpha_list[{tag}_ALPHA_BETA_LIST_SIZE];
+extern const int16_t BG{bgn}_trained_beta_list[{tag}_ALPHA_BETA_LIST_SIZE];
 
 #endif /* {name.upper()}_H */
 """
     c = f"""#include "{name}.h"
 
 /*
- * Trained min-sum weights, [iteration][HBG row]:
- *   alpha_fixed = round(alpha * 32767)        (Q15)
- *   beta_fixed  = round(beta * {beta_scale:g})  (int16 LLR units)
- * trained iterations: {model.num_iterations}; later iterations repeat the last one.
- */
-const int16_t BG{cfg.bgn}_trained_alpha_list[{tag}_ALPHA_BETA_LIST_SIZE] = {{
-{_c_rows(fa)}
+ * Min-sum weights, [iteration][HBG row]:
+ *   alpha_fixed = round(alpha * 32767)   (Q15)
+ *   beta_fixed  = beta in int16 LLR units
+{note} */
+const int16_t BG{bgn}_trained_alpha_list[{tag}_ALPHA_BETA_LIST_SIZE] = {{
+{_c_rows(alpha)}
 }};
 
-const int16_t BG{cfg.bgn}_trained_beta_list[{tag}_ALPHA_BETA_LIST_SIZE] = {{
-{_c_rows(fb)}
+const int16_t BG{bgn}_trained_beta_list[{tag}_ALPHA_BETA_LIST_SIZE] = {{
+{_c_rows(beta)}
 }};
 """
     hp = os.path.join(out_dir, name + ".h")
@@ -572,6 +581,30 @@ const int16_t BG{cfg.bgn}_trained_beta_list[{tag}_ALPHA_BETA_LIST_SIZE] = {{
     with open(cp, "w") as f:
         f.write(c)
     return hp, cp
+
+
+def export_algo2_tables(model, out_dir, num_iterations=ALGO2_NUM_ITERATIONS, beta_scale=128.0):
+    """Write BG<n>_trained_ldpc_weights.h/.c (the layout of
+    c_lib/BG1_trained_ldpc_weights.{h,c}) for the model's base graph.
+    Returns the two paths."""
+    cfg = model.config
+    fa, fb = algo2_tables(model, num_iterations, beta_scale)
+    note = (f" * beta_fixed = round(beta * {beta_scale:g}); trained iterations: "
+            f"{model.num_iterations}, later iterations repeat the last one.\n")
+    meta = dict(A=cfg.A, B=cfg.B, E=cfg.E, Zc=cfg.Zc, crctype=cfg.crctype, rv=cfg.rv, Qm=cfg.Qm)
+    return write_algo2_weight_files(cfg.bgn, fa, fb, out_dir, meta, note)
+
+
+def load_algo2_tables(model, num_iterations=ALGO2_NUM_ITERATIONS, beta_scale=128.0):
+    """Installs the model's exported tables (algo2_tables) as the library's
+    algo-2 tables for its base graph, so c_ldpc_5g_decode with algo 2 (and
+    plans / batch contexts created afterwards) decodes with them -- the
+    reference's export, rebuild and decode loop without the rebuild.
+    Returns (alpha, beta)."""
+    from .tables import set_trained_tables
+    fa, fb = algo2_tables(model, num_iterations, beta_scale)
+    set_trained_tables(model.config.bgn, fa, fb)
+    return fa, fb
 
 
 def synth_float_batch(table_config, noise_power_db, seed=0, threads=8):

@@ -61,8 +61,10 @@ typedef struct {
 
 /* c_lib/nr_ldpc.h:82-90 — the 7-field decoder row
  * [algo, max_iterations, alpha, beta, filling_bit_value, max_llr, min_llr].
- * algo: 0 FP32 layered BP, 1 int16 min-sum, 4 FP32 min-sum (2 and 3 are
- * not provided by this library and return -3). */
+ * algo: 0 FP32 layered BP, 1 int16 min-sum, 2 int16 min-sum with trained
+ * per-(iteration, row) alpha/beta tables (alpha, beta ignored), 3 int16
+ * min-sum with per-degree alpha (alpha ignored, beta used), 4 FP32
+ * min-sum. */
 typedef struct {
     uint16_t algo;
     uint16_t max_iterations;
@@ -104,7 +106,11 @@ float dequantize_int16_to_llr(int16_t quantized_llr);
  * hardbits: B/8 bytes MSB-first (untouched when max_iterations = 0).
  * Returns 0 CRC pass, 1 not converged, -2 invalid config, -3 invalid algo
  * (rate recovery still updates HARQ_memory, as in the reference), -1 GPU or
- * runtime error.  Serialized internally (safe from several threads). */
+ * runtime error.  Algo 2 with max_iterations beyond the trained table of
+ * the codeblock's base graph (32 iterations by default) returns -2 before
+ * rate recovery: the reference reads past its table there
+ * (c_lib/neural_fixedpoint_ldpc_decoder.c:248).  Serialized internally
+ * (safe from several threads). */
 int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
                          ldpc_decoder_cfg_t *decoder_config);
 
@@ -125,9 +131,39 @@ int tx_rx_process(const uint8_t *inbytes, size_t num_bits, uint16_t Qm, float no
 /* Part 2: MI355X extensions (no counterpart in the reference)         */
 /* ------------------------------------------------------------------ */
 
+/* Algo-2 trained tables of base graph bgn: alpha and beta are int16
+ * [iterations][rows] (rows 46 for BG1, 42 for BG2), flattened iteration
+ * first, Q15 alpha and beta in int16 LLR units -- the layout of the
+ * reference's compiled-in BG<n>_trained_alpha_list / _beta_list
+ * (c_lib/BG1_trained_ldpc_weights.h:7-17, indexed at
+ * c_lib/neural_fixedpoint_ldpc_decoder.c:248-250), which it can only change
+ * by recompiling.  iterations 1..1024; alpha NULL restores the default
+ * tables (32 iterations, alpha by the row-degree rule, beta 0).  The tables
+ * are process-wide: c_ldpc_5g_decode reads the tables current at the call,
+ * plans and batch contexts keep the tables current when they were created.
+ * 0, or -4 for invalid arguments. */
+int32_t ldpc_mi355_set_trained_tables(int32_t bgn, const int16_t *alpha, const int16_t *beta,
+                                      int32_t iterations);
+
+/* Copies up to capacity_iterations iterations of base graph bgn's current
+ * algo-2 tables into alpha / beta (either may be NULL) and returns the
+ * tables' iteration count, or -4 for an invalid bgn. */
+int32_t ldpc_mi355_get_trained_tables(int32_t bgn, int16_t *alpha, int16_t *beta,
+                                      int32_t capacity_iterations);
+
+/* Algo-3 degree alpha: MAX_DEGREE Q15 values, entry d-1 for check degree d
+ * -- the table layered_degree_minsum_ldpc_decode_with_alpha takes
+ * (c_lib/nr_ldpc.h:113, c_lib/degree_fixedpoint_ldpc_decoder.c:15-17,
+ * 65-80); NULL restores the defaults (26623 for degree <= 4, 24575 for 5..8,
+ * 22527 above).  Process-wide, read like the trained tables.  Returns 0. */
+int32_t ldpc_mi355_set_degree_alpha(const int16_t *degree_alpha_values);
+
+/* Copies the current MAX_DEGREE degree alpha values; 0, or -4 for NULL. */
+int32_t ldpc_mi355_get_degree_alpha(int16_t *degree_alpha_values);
+
 /* c_ldpc_5g_decode plus what the reference keeps internal: the number of
- * iterations executed and the N+2Zc posteriors at exit (int16 for algo 1,
- * float for algos 0/4); either output may be NULL. */
+ * iterations executed and the N+2Zc posteriors at exit (int16 for algos
+ * 1-3, float for algos 0/4); either output may be NULL. */
 int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
                              ldpc_decoder_cfg_t *decoder_config, int32_t *iterations,
                              void *posteriors);
@@ -185,12 +221,14 @@ typedef struct {
 typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
 
 /* Validates n codeblock descriptors against one decoder row and uploads
- * them to the device.  flags: bit 0 = produce posteriors; bit 1 = pack
- * codeblocks of one (BG, Zc) into shared workgroups (algo 1); bit 2 = never
- * pack.  Without bits 1/2, algo-1 plans pack whenever some lifting size
- * leaves room for two or more codeblocks per workgroup.  Returns 0, -2 if
- * any codeblock config is invalid (index in *bad_index when non-NULL), -3
- * for an unsupported algo, -1 on a GPU error. */
+ * them to the device (for algos 2/3 also the scale tables current at this
+ * call).  flags: bit 0 = produce posteriors; bit 1 = pack codeblocks of one
+ * (BG, Zc) into shared workgroups (int16 algos 1-3); bit 2 = never pack.
+ * Without bits 1/2, int16 plans pack whenever some lifting size leaves room
+ * for two or more codeblocks per workgroup.  Returns 0, -2 if any codeblock
+ * config is invalid or, for algo 2, needs more iterations than its base
+ * graph's trained table holds (index in *bad_index when non-NULL), -3 for
+ * an unsupported algo, -1 on a GPU error. */
 int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_mi355_cb_t *cbs, int32_t n,
                                const ldpc_decoder_cfg_t *decoder_config, uint32_t flags,
                                int32_t *bad_index);
@@ -231,8 +269,10 @@ int32_t ldpc_mi355_set_tile_codeblocks(int32_t bgn, int32_t zc, int32_t k);
  * decode on a context whose previous decode is still in flight first waits
  * for it; ldpc_mi355_batch_query polls (1 done, 0 running), batch_wait
  * blocks.  flags as for plans (bit 0 posteriors, bit 1 pack, bit 2 never
- * pack).  Algos 1 and 4.  Returns 0, -2 invalid codeblock config
- * (*bad_index), -3 unsupported algo, -4 invalid argument, -1 GPU error. */
+ * pack).  Algos 1-4; algos 2/3 keep the scale tables current at creation.
+ * Returns 0, -2 invalid codeblock config or algo-2 iterations beyond the
+ * table (*bad_index), -3 unsupported algo, -4 invalid argument, -1 GPU
+ * error. */
 typedef struct ldpc_mi355_batch ldpc_mi355_batch_t;
 int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t capacity,
                                 const ldpc_decoder_cfg_t *decoder_config, uint32_t flags);

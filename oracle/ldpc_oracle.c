@@ -208,28 +208,34 @@ static inline int16_t clip_i16(int32_t v, int16_t hi, int16_t lo)
     return (int16_t)v;
 }
 
-int orc_decode_i16(const orc_code_t *c, const orc_dec_t *d,
-                   const int16_t *soft, uint8_t *hard, int16_t *lq_out,
-                   int *iters)
+/* The layered schedule of c_lib/fixedpoint_ldpc_decoder.c:162-222 with the
+ * (alpha, beta) pair of iteration it, row r taken from
+ * alpha[it * stride + r], beta[it * stride + r] (stride 0: one pair per row
+ * for every iteration).  Algos 1, 2 and 3 differ only in that lookup. */
+static int decode_i16_rows(const orc_code_t *c, const orc_dec_t *d,
+                           const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                           int *iters, const int16_t *alpha,
+                           const int16_t *beta, int stride)
 {
     const int Zc = c->Zc, len = c->N + 2 * Zc;
     int16_t *LQ = (int16_t *)malloc(sizeof(int16_t) * (size_t)len);
     int16_t *Lr = (int16_t *)calloc((size_t)c->nnz * Zc, sizeof(int16_t));
     memcpy(LQ, soft, sizeof(int16_t) * (size_t)len);
-    const int16_t alpha = (int16_t)d->alpha, beta = (int16_t)d->beta;
     int16_t rel[32], msg[32];
     int colv[32];
     int status = 1, it;
     for (it = 0; it < d->max_iterations; it++) {
         for (int r = 0; r < c->rows; r++) {
             const int e0 = c->row_off[r], deg = c->row_off[r + 1] - e0;
+            const int16_t a = alpha[(size_t)it * stride + r];
+            const int16_t b = beta[(size_t)it * stride + r];
             for (int zx = 0; zx < Zc; zx++) {
                 int16_t *lr = Lr + (size_t)e0 * Zc + (size_t)zx * deg;
                 for (int j = 0; j < deg; j++) {
                     colv[j] = lifted_col(c, e0 + j, zx);
                     rel[j] = wrap16((int32_t)LQ[colv[j]] - lr[j]);
                 }
-                orc_check_minsum_i16(deg, rel, alpha, beta, d->max_llr, msg);
+                orc_check_minsum_i16(deg, rel, a, b, d->max_llr, msg);
                 for (int j = 0; j < deg; j++) {
                     lr[j] = msg[j];
                     LQ[colv[j]] = clip_i16((int32_t)rel[j] + msg[j],
@@ -244,6 +250,82 @@ int orc_decode_i16(const orc_code_t *c, const orc_dec_t *d,
     if (lq_out) memcpy(lq_out, LQ, sizeof(int16_t) * (size_t)len);
     free(LQ); free(Lr);
     return status;
+}
+
+int orc_decode_i16(const orc_code_t *c, const orc_dec_t *d,
+                   const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                   int *iters)
+{
+    int16_t a[ORC_MAX_ROWS], b[ORC_MAX_ROWS];
+    for (int r = 0; r < c->rows; r++) {
+        a[r] = (int16_t)d->alpha;   /* the int16 view the reference's helpers take */
+        b[r] = (int16_t)d->beta;
+    }
+    return decode_i16_rows(c, d, soft, hard, lq_out, iters, a, b, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Algo 2: trained-table min-sum,                                      */
+/* c_lib/neural_fixedpoint_ldpc_decoder.c:223-281 (alpha/beta of       */
+/* iteration it, row r at it * rows + r, :248-250).                    */
+/* ------------------------------------------------------------------ */
+int orc_decode_i16_table(const orc_code_t *c, const orc_dec_t *d,
+                         const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                         int *iters, const int16_t *alpha,
+                         const int16_t *beta, int table_iterations)
+{
+    if (iters) *iters = 0;
+    if (d->max_iterations > table_iterations) return -2;
+    return decode_i16_rows(c, d, soft, hard, lq_out, iters, alpha, beta,
+                           c->rows);
+}
+
+/* Degree rule of the reference's default tables: Q15 alpha 26623 for
+ * check degree <= 4, 24575 for 5..8, 22527 above 8
+ * (c_lib/degree_fixedpoint_ldpc_decoder.c:9-43; the default trained
+ * tables of c_lib/BG1_trained_ldpc_weights.c:27-33 apply the same rule to
+ * every iteration with beta 0). */
+int16_t orc_default_degree_alpha(int degree)
+{
+    if (degree <= 4) return 26623;
+    if (degree <= 8) return 24575;
+    return 22527;
+}
+
+void orc_default_trained_tables(int bgn, int iterations, int16_t *alpha,
+                                int16_t *beta)
+{
+    const uint16_t *off = bgn == 1 ? orc_bg1_row_off : orc_bg2_row_off;
+    const int rows = bgn == 1 ? 46 : 42;
+    for (int it = 0; it < iterations; it++)
+        for (int r = 0; r < rows; r++) {
+            alpha[it * rows + r] = orc_default_degree_alpha(off[r + 1] - off[r]);
+            beta[it * rows + r] = 0;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Algo 3: degree min-sum,                                             */
+/* c_lib/degree_fixedpoint_ldpc_decoder.c:65-80 (alpha by exact degree */
+/* d at entry d-1; degree 0 -> entry 0, degree > 19 -> entry 18),      */
+/* 200-268 (schedule; beta from the decoder row).                      */
+/* ------------------------------------------------------------------ */
+int orc_decode_i16_degree(const orc_code_t *c, const orc_dec_t *d,
+                          const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                          int *iters, const int16_t *degree_alpha)
+{
+    int16_t a[ORC_MAX_ROWS], b[ORC_MAX_ROWS];
+    for (int r = 0; r < c->rows; r++) {
+        const int deg = c->row_off[r + 1] - c->row_off[r];
+        if (degree_alpha) {
+            const int k = deg == 0 ? 0 : (deg > ORC_MAX_DEGREE ? ORC_MAX_DEGREE - 1 : deg - 1);
+            a[r] = degree_alpha[k];
+        } else {
+            a[r] = orc_default_degree_alpha(deg);
+        }
+        b[r] = (int16_t)d->beta;
+    }
+    return decode_i16_rows(c, d, soft, hard, lq_out, iters, a, b, 0);
 }
 
 /* ------------------------------------------------------------------ */
@@ -393,21 +475,39 @@ DEFINE_BP(double, orc_decode_f64_bp, bp_check_f64)
 /* ------------------------------------------------------------------ */
 /* Whole path: c_lib/c_ldpc_5g_decoder.c:34-79.                          */
 /* ------------------------------------------------------------------ */
+int orc_decode_path_tables(const int16_t *llr, uint8_t *hard, int bgn, int A,
+                           int crctype, int E, int rv, int Qm, int harq_on,
+                           const orc_dec_t *d, int16_t *soft, int16_t *lq_i16,
+                           float *lq_f32, int *iters, const int16_t *trained_alpha,
+                           const int16_t *trained_beta, int trained_iterations,
+                           const int16_t *degree_alpha)
+{
+    orc_code_t c;
+    if (iters) *iters = 0;
+    if (orc_code_init(&c, bgn, A, crctype, E, rv, Qm, harq_on) != 0) return -2;
+    if (d->algo == 2 && d->max_iterations > trained_iterations) return -2;
+    orc_rate_recover(&c, d->filling_bit_value, llr, soft);
+    switch (d->algo) {
+    case 0: return orc_decode_f32_bp(&c, d, soft, hard, lq_f32, iters);
+    case 1: return orc_decode_i16(&c, d, soft, hard, lq_i16, iters);
+    case 2: return orc_decode_i16_table(&c, d, soft, hard, lq_i16, iters, trained_alpha,
+                                        trained_beta, trained_iterations);
+    case 3: return orc_decode_i16_degree(&c, d, soft, hard, lq_i16, iters, degree_alpha);
+    case 4: return orc_decode_f32_ms(&c, d, soft, hard, lq_f32, iters);
+    default: return -3;
+    }
+}
+
 int orc_decode_path(const int16_t *llr, uint8_t *hard, int bgn, int A,
                     int crctype, int E, int rv, int Qm, int harq_on,
                     const orc_dec_t *d, int16_t *soft, int16_t *lq_i16,
                     float *lq_f32, int *iters)
 {
-    orc_code_t c;
-    if (iters) *iters = 0;
-    if (orc_code_init(&c, bgn, A, crctype, E, rv, Qm, harq_on) != 0) return -2;
-    orc_rate_recover(&c, d->filling_bit_value, llr, soft);
-    switch (d->algo) {
-    case 0: return orc_decode_f32_bp(&c, d, soft, hard, lq_f32, iters);
-    case 1: return orc_decode_i16(&c, d, soft, hard, lq_i16, iters);
-    case 4: return orc_decode_f32_ms(&c, d, soft, hard, lq_f32, iters);
-    default: return -3;
-    }
+    int16_t ta[32 * ORC_MAX_ROWS], tb[32 * ORC_MAX_ROWS];
+    if (d->algo == 2 && (bgn == 1 || bgn == 2))
+        orc_default_trained_tables(bgn, 32, ta, tb);
+    return orc_decode_path_tables(llr, hard, bgn, A, crctype, E, rv, Qm, harq_on, d, soft,
+                                  lq_i16, lq_f32, iters, ta, tb, 32, NULL);
 }
 
 /* ------------------------------------------------------------------ */

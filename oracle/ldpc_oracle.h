@@ -19,6 +19,11 @@
  *   orc_rate_recover    reference HARQ_memory fixtures incl. HARQ chains;
  *                       closed-form gather (tests) and mod-2^16 additivity
  *   orc_decode_i16      reference hardbits/status/iterations/LQ fixtures
+ *   orc_decode_i16_table / _degree
+ *                       reference algo-2 / algo-3 fixtures (default tables,
+ *                       and algo 2 with exported non-uniform tables linked
+ *                       into the from-source reference); reduction to
+ *                       orc_decode_i16 for uniform tables (tests)
  *   orc_decode_f32_ms   reference FP32 fixtures (bit-exact) + FP64 twin
  *   orc_decode_f32_bp   reference FP32 fixtures (same glibc libm) + FP64 twin
  *   orc_encode / _rm    reference encoder fixtures; H*c = 0 syndrome check
@@ -46,6 +51,9 @@ typedef struct {
     const uint16_t *col;     /* nnz */
     const uint16_t *vij;     /* nnz, for this iLS */
 } orc_code_t;
+
+#define ORC_MAX_ROWS 46     /* BG1 rows (c_lib/nr_ldpc.h) */
+#define ORC_MAX_DEGREE 19   /* MAX_DEGREE (c_lib/nr_ldpc.h:18) */
 
 typedef struct {
     /* decoder row, ldpc_decoder_cfg_t (c_lib/nr_ldpc.h:82-90) */
@@ -77,6 +85,22 @@ void orc_rate_recover(const orc_code_t *c, int16_t filler, const int16_t *llr,
 int orc_decode_i16(const orc_code_t *c, const orc_dec_t *d,
                    const int16_t *soft, uint8_t *hard, int16_t *lq_out,
                    int *iters);
+/* Algo 2: alpha/beta [table_iterations][rows]; -2 when max_iterations
+ * exceeds table_iterations (the reference reads past its table then). */
+int orc_decode_i16_table(const orc_code_t *c, const orc_dec_t *d,
+                         const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                         int *iters, const int16_t *alpha,
+                         const int16_t *beta, int table_iterations);
+/* Algo 3: degree_alpha = ORC_MAX_DEGREE Q15 values (entry d-1 for degree d)
+ * or NULL for the reference defaults; beta from the decoder row. */
+int orc_decode_i16_degree(const orc_code_t *c, const orc_dec_t *d,
+                          const int16_t *soft, uint8_t *hard, int16_t *lq_out,
+                          int *iters, const int16_t *degree_alpha);
+/* The reference's default alpha of a check degree and its default trained
+ * tables ([iterations][rows], beta 0). */
+int16_t orc_default_degree_alpha(int degree);
+void orc_default_trained_tables(int bgn, int iterations, int16_t *alpha,
+                                int16_t *beta);
 int orc_decode_f32_ms(const orc_code_t *c, const orc_dec_t *d,
                       const int16_t *soft, uint8_t *hard, float *lq_out,
                       int *iters);
@@ -97,6 +121,15 @@ int orc_decode_path(const int16_t *llr, uint8_t *hard, int bgn, int A,
                     int crctype, int E, int rv, int Qm, int harq_on,
                     const orc_dec_t *d, int16_t *soft, int16_t *lq_i16,
                     float *lq_f32, int *iters);
+/* The same with explicit algo-2 tables ([trained_iterations][rows]) and
+ * algo-3 degree alpha (NULL = defaults); orc_decode_path uses the
+ * reference's default tables (32 iterations). */
+int orc_decode_path_tables(const int16_t *llr, uint8_t *hard, int bgn, int A,
+                           int crctype, int E, int rv, int Qm, int harq_on,
+                           const orc_dec_t *d, int16_t *soft, int16_t *lq_i16,
+                           float *lq_f32, int *iters, const int16_t *trained_alpha,
+                           const int16_t *trained_beta, int trained_iterations,
+                           const int16_t *degree_alpha);
 
 /* Check-node kernels on one lifted check, exposed for hand-built tests.
  * rel[deg] in, msg[deg] out (new check-to-variable messages). */

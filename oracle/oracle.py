@@ -61,6 +61,15 @@ def lib():
                                          ctypes.POINTER(OrcDec), P, P, P, P]
         L.orc_decode_path.argtypes = [P, P] + [ctypes.c_int] * 7 + [
             ctypes.POINTER(OrcDec), P, P, P, P]
+        L.orc_decode_path_tables.argtypes = [P, P] + [ctypes.c_int] * 7 + [
+            ctypes.POINTER(OrcDec), P, P, P, P, P, P, ctypes.c_int, P]
+        L.orc_decode_i16_table.argtypes = [ctypes.POINTER(OrcCode), ctypes.POINTER(OrcDec),
+                                           P, P, P, P, P, P, ctypes.c_int]
+        L.orc_decode_i16_degree.argtypes = [ctypes.POINTER(OrcCode), ctypes.POINTER(OrcDec),
+                                            P, P, P, P, P]
+        L.orc_default_degree_alpha.argtypes = [ctypes.c_int]
+        L.orc_default_degree_alpha.restype = ctypes.c_int16
+        L.orc_default_trained_tables.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
         L.orc_check_minsum_i16.argtypes = [ctypes.c_int, P, ctypes.c_int16,
                                            ctypes.c_int16, ctypes.c_int16, P]
         L.orc_check_bp_f64.argtypes = [ctypes.c_int, P, P]
@@ -126,24 +135,68 @@ _DEC_FN = {
 }
 
 
-def decode_soft(c, row, soft, precision=None):
-    """Decode a rate-recovered buffer.  Returns (status, hard, lq, iters)."""
+MAX_DEGREE = 19
+NUM_ROWS = {1: 46, 2: 42}
+
+
+def default_degree_alpha():
+    """The reference's algo-3 defaults, entry d-1 for check degree d."""
+    return np.array([lib().orc_default_degree_alpha(d) for d in range(1, MAX_DEGREE + 1)],
+                    np.int16)
+
+
+def default_trained_tables(bgn, iterations=32):
+    """The reference's default algo-2 tables: (alpha, beta) int16
+    [iterations, rows]."""
+    a = np.zeros((iterations, NUM_ROWS[bgn]), np.int16)
+    b = np.zeros_like(a)
+    lib().orc_default_trained_tables(bgn, iterations, _p(a), _p(b))
+    return a, b
+
+
+def _tables_for(bgn, tables):
+    if tables is None:
+        tables = default_trained_tables(bgn)
+    a = np.ascontiguousarray(tables[0], np.int16)
+    b = np.ascontiguousarray(tables[1], np.int16)
+    if a.shape != b.shape or a.ndim != 2 or a.shape[1] != NUM_ROWS[bgn]:
+        raise ValueError("algo-2 tables must be [iterations, rows] int16 pairs")
+    return a, b
+
+
+def decode_soft(c, row, soft, precision=None, tables=None, degree_alpha=None):
+    """Decode a rate-recovered buffer.  Returns (status, hard, lq, iters).
+    Algo 2 takes tables = (alpha, beta) [iterations, rows] (default: the
+    reference's), algo 3 degree_alpha (19 values; default: the reference's)."""
     d = dec(row)
+    soft = np.ascontiguousarray(soft, dtype=np.int16)
+    hard = np.zeros(c.B // 8, np.uint8)
+    it = ctypes.c_int(0)
+    if d.algo in (2, 3):
+        lq = np.zeros(c.N + 2 * c.Zc, np.int16)
+        if d.algo == 2:
+            a, b = _tables_for(c.bgn, tables)
+            st = lib().orc_decode_i16_table(ctypes.byref(c), ctypes.byref(d), _p(soft), _p(hard),
+                                            _p(lq), ctypes.byref(it), _p(a), _p(b), a.shape[0])
+        else:
+            da = None if degree_alpha is None else np.ascontiguousarray(degree_alpha, np.int16)
+            st = lib().orc_decode_i16_degree(ctypes.byref(c), ctypes.byref(d), _p(soft),
+                                             _p(hard), _p(lq), ctypes.byref(it),
+                                             None if da is None else _p(da))
+        return st, hard, lq, it.value
     if precision is None:
         precision = "i16" if d.algo == 1 else "f32"
     name, dt = _DEC_FN[(precision, d.algo)]
-    soft = np.ascontiguousarray(soft, dtype=np.int16)
-    hard = np.zeros(c.B // 8, np.uint8)
     lq = np.zeros(c.N + 2 * c.Zc, dt)
-    it = ctypes.c_int(0)
     st = getattr(lib(), name)(ctypes.byref(c), ctypes.byref(d), _p(soft),
                               _p(hard), _p(lq), ctypes.byref(it))
     return st, hard, lq, it.value
 
 
-def decode_path(llr, cfg, row, harq=None):
+def decode_path(llr, cfg, row, harq=None, tables=None, degree_alpha=None):
     """Whole c_ldpc_5g_decode path.  cfg = (bgn, A, crctype, E, rv, Qm, harq_on).
-    Returns dict(status, hard, soft, lq, iters)."""
+    tables / degree_alpha as in decode_soft.  Returns dict(status, hard, soft,
+    lq, iters)."""
     bgn, A, crctype, E, rv, Qm, harq_on = cfg
     st, p = params(bgn, A, crctype, E, rv, Qm, harq_on)
     n = (p["N"] + 2 * p["Zc"]) if st == 0 else 68 * 384
@@ -157,10 +210,18 @@ def decode_path(llr, cfg, row, harq=None):
     it = ctypes.c_int(0)
     d = dec(row)
     llr = np.ascontiguousarray(llr, dtype=np.int16)
-    status = lib().orc_decode_path(_p(llr), _p(hard), bgn, A, crctype, E, rv, Qm,
-                                   harq_on, ctypes.byref(d), _p(soft), _p(lq16),
-                                   _p(lq32), ctypes.byref(it))
-    lq = lq16[:n] if d.algo == 1 else lq32[:n]
+    ta = tb = da = None
+    titers = 0
+    if d.algo == 2 and bgn in (1, 2):
+        ta, tb = _tables_for(bgn, tables)
+        titers = ta.shape[0]
+    if d.algo == 3 and degree_alpha is not None:
+        da = np.ascontiguousarray(degree_alpha, np.int16)
+    status = lib().orc_decode_path_tables(
+        _p(llr), _p(hard), bgn, A, crctype, E, rv, Qm, harq_on, ctypes.byref(d), _p(soft),
+        _p(lq16), _p(lq32), ctypes.byref(it), None if ta is None else _p(ta),
+        None if tb is None else _p(tb), titers, None if da is None else _p(da))
+    lq = lq16[:n] if d.algo in (1, 2, 3) else lq32[:n]
     return dict(status=status, hard=hard, soft=soft[:n], lq=lq, iters=it.value)
 
 

@@ -32,17 +32,30 @@ def load_decode_cases(name):
     harq = _split(z["harq_in"], z["harq_len"])
     cases = []
     for i in range(len(z["cfg"])):
-        cases.append(dict(idx=i, cfg=tuple(int(v) for v in z["cfg"][i]),
-                          row=[int(v) for v in z["row"][i]],
-                          status=int(z["status"][i]), iters=int(z["iters"][i]),
-                          llr=llr[i], hard=hard[i], soft=soft[i], lq=lq[i],
-                          harq_in=harq[i] if len(harq[i]) else None))
+        c = dict(idx=i, cfg=tuple(int(v) for v in z["cfg"][i]),
+                 row=[int(v) for v in z["row"][i]],
+                 status=int(z["status"][i]), iters=int(z["iters"][i]),
+                 llr=llr[i], hard=hard[i], soft=soft[i], lq=lq[i],
+                 harq_in=harq[i] if len(harq[i]) else None)
+        if "table" in z.files:
+            # algo 2: the (alpha, beta) [32, rows] tables the reference build had
+            t, bgn = int(z["table"][i]), c["cfg"][0]
+            c["table_id"] = t
+            c["tables"] = (z[f"alpha_bg{bgn}"][t], z[f"beta_bg{bgn}"][t])
+        if "degree_table" in z.files:
+            # algo 3: None = the reference defaults through c_ldpc_5g_decode
+            t = int(z["degree_table"][i])
+            c["table_id"] = t
+            c["degree_alpha"] = None if t < 0 else z["degree_alpha"][t]
+        cases.append(c)
     return cases
 
 
 def case_id(c):
     bgn, A, crc, E, rv, Qm, h = c["cfg"]
-    return f"{c['idx']}-bg{bgn}-A{A}-E{E}-rv{rv}-Qm{Qm}-h{h}-algo{c['row'][0]}-it{c['row'][1]}"
+    tab = f"-tab{c['table_id']}" if "table_id" in c else ""
+    return (f"{c['idx']}-bg{bgn}-A{A}-E{E}-rv{rv}-Qm{Qm}-h{h}-algo{c['row'][0]}"
+            f"-it{c['row'][1]}{tab}")
 
 
 @pytest.fixture(scope="session")

@@ -53,8 +53,8 @@ def P(a):
 
 
 class Ref:
-    def __init__(self):
-        self.L = ctypes.CDLL(REF)
+    def __init__(self, path=REF):
+        self.L = ctypes.CDLL(path)
         self.L.c_ldpc_5g_decode.restype = ctypes.c_int32
         self.L.c_ldpc_5g_encode.restype = ctypes.c_int32
         self.L.calculate_crc.restype = ctypes.c_uint32
@@ -64,6 +64,10 @@ class Ref:
                                          ctypes.c_void_p]
         self.L.ref_probe_fixed_lq.restype = ctypes.POINTER(ctypes.c_int16)
         self.L.ref_probe_float_lq.restype = ctypes.POINTER(ctypes.c_float)
+        self.L.ref_probe_neural_lq.restype = ctypes.POINTER(ctypes.c_int16)
+        self.L.ref_probe_degree_lq.restype = ctypes.POINTER(ctypes.c_int16)
+        self.L.layered_degree_minsum_ldpc_decode_with_alpha.restype = ctypes.c_int32
+        self.L.layered_degree_minsum_ldpc_decode_with_alpha.argtypes = [ctypes.c_void_p] * 5
         self.harq = np.ctypeslib.as_array(
             (ctypes.c_int16 * (68 * 384)).in_dll(self.L, "HARQ_memory"))
         self.libc = ctypes.CDLL("libc.so.6")
@@ -88,13 +92,40 @@ class Ref:
             return status, hard, None, None
         n = p[5] + 2 * p[1]
         soft = self.harq[:n].copy()
-        if row[0] == 1:
-            lq = np.ctypeslib.as_array(self.L.ref_probe_fixed_lq(), (n,)).copy()
-        elif row[0] in (0, 4):
-            lq = np.ctypeslib.as_array(self.L.ref_probe_float_lq(), (n,)).copy()
-        else:
-            lq = None
+        probe = {1: self.L.ref_probe_fixed_lq, 2: self.L.ref_probe_neural_lq,
+                 3: self.L.ref_probe_degree_lq, 0: self.L.ref_probe_float_lq,
+                 4: self.L.ref_probe_float_lq}.get(row[0])
+        lq = np.ctypeslib.as_array(probe(), (n,)).copy() if probe else None
         return status, hard, soft, lq
+
+    def decode_with_alpha(self, llr, cfg, row, degree_alpha):
+        """Rate recovery (c_ldpc_5g_decode with no iterations), then
+        layered_degree_minsum_ldpc_decode_with_alpha on HARQ_memory with a
+        custom degree table; returns status, hard, soft, lq, iters."""
+        r0 = list(row)
+        r0[1] = 0
+        st, _, soft, _ = self.decode(llr, cfg, r0)
+        assert st == 1
+        c = LdpcCfg()
+        assert self.L.gen_ldpc_parameters(ctypes.byref(c), ctypes.byref(InCfg(*cfg))) == 0
+        harq = ctypes.addressof(ctypes.c_int16.in_dll(self.L, "HARQ_memory"))
+        da = np.ascontiguousarray(degree_alpha, np.int16)
+        n = len(soft)
+
+        def run(k):
+            hard = np.zeros(c.B // 8, np.uint8)
+            r = list(row)
+            r[1] = k
+            st = self.L.layered_degree_minsum_ldpc_decode_with_alpha(
+                harq, P(hard), ctypes.byref(c), ctypes.byref(DecCfg(*r)), P(da))
+            return st, hard
+
+        status, hard = run(row[1])
+        lq = np.ctypeslib.as_array(self.L.ref_probe_degree_lq(), (n,)).copy()
+        iters = row[1]
+        if status == 0:
+            iters = next(k for k in range(1, row[1] + 1) if run(k)[0] == 0)
+        return status, hard, soft, lq, iters
 
     def decode_with_iters(self, llr, cfg, row):
         """Full decode plus the executed iteration count (min-k search).
@@ -339,24 +370,157 @@ def gen_decode(ref, rng):
         split[c[1][0]].append(c)
     for algo, name in ((1, "decode_algo1.npz"), (4, "decode_algo4.npz"),
                        (0, "decode_algo0.npz")):
-        cs = split[algo]
-        harq_in = [c[8] if c[8] is not None else np.zeros(0, np.int16) for c in cs]
-        np.savez_compressed(
-            os.path.join(HERE, name),
-            cfg=np.array([c[0] for c in cs], np.int32),
-            row=np.array([c[1] for c in cs], np.int32),
-            status=np.array([c[3] for c in cs], np.int32),
-            iters=np.array([c[7] for c in cs], np.int32),
-            llr=np.concatenate([c[2] for c in cs]),
-            llr_len=np.array([len(c[2]) for c in cs], np.int32),
-            hard=np.concatenate([c[4] for c in cs]),
-            hard_len=np.array([len(c[4]) for c in cs], np.int32),
-            soft=np.concatenate([c[5] for c in cs]),
-            lq=np.concatenate([c[6] for c in cs]),
-            buf_len=np.array([len(c[5]) for c in cs], np.int32),
-            harq_in=np.concatenate(harq_in),
-            harq_len=np.array([len(h) for h in harq_in], np.int32))
-        print(name, len(cs), "cases")
+        save_cases(name, split[algo])
+
+
+def save_cases(name, cs, **extra):
+    harq_in = [c[8] if c[8] is not None else np.zeros(0, np.int16) for c in cs]
+    np.savez_compressed(
+        os.path.join(HERE, name),
+        cfg=np.array([c[0] for c in cs], np.int32),
+        row=np.array([c[1] for c in cs], np.int32),
+        status=np.array([c[3] for c in cs], np.int32),
+        iters=np.array([c[7] for c in cs], np.int32),
+        llr=np.concatenate([c[2] for c in cs]),
+        llr_len=np.array([len(c[2]) for c in cs], np.int32),
+        hard=np.concatenate([c[4] for c in cs]),
+        hard_len=np.array([len(c[4]) for c in cs], np.int32),
+        soft=np.concatenate([c[5] for c in cs]),
+        lq=np.concatenate([c[6] for c in cs]),
+        buf_len=np.array([len(c[5]) for c in cs], np.int32),
+        harq_in=np.concatenate(harq_in),
+        harq_len=np.array([len(h) for h in harq_in], np.int32), **extra)
+    print(name, len(cs), "cases")
+
+
+def _trained_tables(rng, bgn, kind):
+    """[32, rows] int16 alpha/beta tables: 'smooth' (alpha decaying over
+    iterations around 0.6-0.9, small positive beta) or 'adversarial' (a
+    quarter of the entries anywhere in int16, including negative alpha,
+    negative beta and the extremes)."""
+    rows = 46 if bgn == 1 else 42
+    t = np.arange(32)[:, None]
+    a = 32767 * (0.9 - 0.25 * (1 - np.exp(-t / 6.0)) - 0.05 * rng.random((32, rows)))
+    b = rng.integers(0, 97, (32, rows))
+    if kind == "adversarial":
+        m = rng.random((32, rows)) < 0.25
+        a = np.where(m, rng.integers(-32768, 32768, (32, rows)), a)
+        m = rng.random((32, rows)) < 0.25
+        b = np.where(m, rng.integers(-32768, 32768, (32, rows)), b)
+        a[3, :5] = 32767
+        b[3, :5] = -32768
+        a[4, :3] = -32768
+    return np.round(a).astype(np.int16), np.asarray(b).astype(np.int16)
+
+
+def build_trained_ref(name, tables):
+    """Export the tables with this library's weight-file writer into
+    oracle/_ref/<name>/ and build the reference decoder linked with them."""
+    import subprocess
+    from deeplearning_5g_ldpc_decoder_amd.unrolled import write_algo2_weight_files
+    d = os.path.join(ROOT, "oracle", "_ref", name)
+    for bgn in (1, 2):
+        write_algo2_weight_files(bgn, tables[bgn][0], tables[bgn][1], d)
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref", "trained", f"TRAINED={name}"],
+                   cwd=ROOT, check=True)
+    return Ref(os.path.join(d, "libnrLDPC_ref_trained.so"))
+
+
+def gen_decode_tables(ref, rng):
+    """Algo 2 (trained tables) and algo 3 (degree alpha) fixtures.
+
+    Algo 2 runs on three builds of the reference: its own default tables and
+    two table sets written by this library's exporter and linked into the
+    from-source reference decoder.  Algo 3 runs through c_ldpc_5g_decode
+    (default degree alpha) and through
+    layered_degree_minsum_ldpc_decode_with_alpha with custom tables."""
+    cases2, cases3 = [], []
+    tabs = {0: {bgn: O.default_trained_tables(bgn) for bgn in (1, 2)}}
+    tabs[1] = {bgn: _trained_tables(rng, bgn, "smooth") for bgn in (1, 2)}
+    tabs[2] = {bgn: _trained_tables(rng, bgn, "adversarial") for bgn in (1, 2)}
+    refs = {0: ref, 1: build_trained_ref("trained_smooth", tabs[1]),
+            2: build_trained_ref("trained_adversarial", tabs[2])}
+    dalpha = [O.default_degree_alpha(),
+              np.round(32767 * (0.92 - 0.016 * np.arange(1, 20))).astype(np.int16),
+              rng.integers(-32768, 32768, 19).astype(np.int16)]
+    dalpha[2][[0, 2, 4, 7, 18]] = [32767, -32768, 0, 32767, -1]
+
+    def add2(table, cfg7, row, llr, harq_in=None):
+        R = refs[table]
+        if harq_in is not None:
+            R.harq[:] = 0
+            R.harq[:len(harq_in)] = harq_in
+        st, hard, soft, lq, iters = R.decode_with_iters(llr, cfg7, row)
+        cases2.append((list(cfg7), list(row), llr.copy(), st, hard.copy(), soft.copy(),
+                       lq.copy(), iters,
+                       None if harq_in is None else np.asarray(harq_in, np.int16).copy(), table))
+
+    def add3(da, cfg7, row, llr):
+        if da < 0:
+            st, hard, soft, lq, iters = ref.decode_with_iters(llr, cfg7, row)
+        else:
+            st, hard, soft, lq, iters = ref.decode_with_alpha(llr, cfg7, row, dalpha[da])
+        cases3.append((list(cfg7), list(row), llr.copy(), st, hard.copy(), soft.copy(),
+                       lq.copy(), iters, None, da))
+
+    cfgs = [((1, 488, 1, 1536, 0, 1), -2.0), ((2, 176, 1, 512, 0, 1), -1.0),
+            ((2, 1016, 2, 5200, 2, 4), -4.0), ((2, 2056, 2, 4160, 0, 4), -6.0),
+            ((1, 3816, 1, 11448, 3, 6), -10.0), ((2, 3816, 3, 9600, 1, 8), -16.0),
+            ((2, 8, 0, 200, 0, 2), -3.0), ((1, 64, 0, 720, 2, 2), -2.0),
+            ((2, 3000, 2, 7600, 0, 2), -0.5), ((1, 8424, 2, 25344, 0, 2), -1.0)]
+    for cfg, noise in cfgs:
+        cb = O.make_codeblock(*cfg, noise, rng)
+        for table, row in ((0, [2, 8, 0, 0, 1280, 32767, -32767]),
+                           (0, [2, 32, 0, 0, -900, 32767, -32768]),
+                           (1, [2, 16, 0, 0, 1280, 32767, -32767]),
+                           (2, [2, 12, 0, 0, 1280, 32767, -32767])):
+            add2(table, cfg + (0,), row, cb["llr"])
+        for da, row in ((-1, [3, 8, 0, 0, 1280, 32767, -32767]),
+                        (-1, [3, 12, 0, 32, -900, 32767, -32767]),
+                        (1, [3, 16, 0, 0, 1280, 32767, -32767]),
+                        (2, [3, 12, 0, 16, 1280, 32767, -32767])):
+            add3(da, cfg + (0,), row, cb["llr"])
+
+    # decoder-row edge cases on one mid-size code
+    cfg = (2, 1016, 2, 3120, 0, 2)
+    cb = O.make_codeblock(*cfg, -2.5, rng)
+    edge = ([0, 0, 0, 1280, 32767, -32767], [1, 0, 0, 1280, 32767, -32767],
+            [8, 0, 200, 1280, 32767, -32767], [8, 0, 40000, 1280, 32767, -32767],
+            [8, 0, 65535, 1280, 32767, -32767], [8, 0, 0, 1280, 2000, -1500],
+            [8, 0, 0, 1280, -200, -32768], [8, 0, 0, -1280, 32767, -32767],
+            [20, 0, 64, 32767, 32767, -32768])
+    for tail in edge:
+        for table in (0, 1, 2):
+            add2(table, cfg + (0,), [2] + tail, cb["llr"])
+        for da in (-1, 2):
+            add3(da, cfg + (0,), [3] + tail, cb["llr"])
+    # saturated channel values: int16 wrap of rel = LQ - Lr
+    llr = np.where(cb["rm"] == 0, 32767, -32768).astype(np.int16)
+    llr[::7] = -llr[::7]
+    for table in (0, 2):
+        add2(table, cfg + (0,), [2, 8, 0, 0, 32767, 32767, -32768], llr)
+    for da in (-1, 1, 2):
+        add3(da, cfg + (0,), [3, 8, 0, 40000, 32767, 32767, -32768], llr)
+
+    # HARQ chain rv0 -> rv2 -> rv3 -> rv1 with the smooth trained tables
+    r = np.random.default_rng(31)
+    b = O.attach_crc(r.integers(0, 2, 2024).astype(np.uint8), 2)
+    for i, rv in enumerate((0, 2, 3, 1)):
+        cfg7 = (1, 2024, 2, 3000, rv, 2, 0 if i == 0 else 1)
+        c = O.code(*cfg7)
+        llr = O.channel_llr(O.rate_match(c, O.encode(c, b)), 2, -5.5, r)
+        R = refs[1]
+        start = R.harq[:c.N + 2 * c.Zc].copy() if i else None
+        add2(1, cfg7, [2, 8, 0, 0, 1280, 32767, -32767], llr, harq_in=start)
+
+    ids2 = np.array([c[9] for c in cases2], np.int32)
+    save_cases("decode_algo2.npz", cases2, table=ids2,
+               alpha_bg1=np.stack([tabs[k][1][0] for k in range(3)]),
+               beta_bg1=np.stack([tabs[k][1][1] for k in range(3)]),
+               alpha_bg2=np.stack([tabs[k][2][0] for k in range(3)]),
+               beta_bg2=np.stack([tabs[k][2][1] for k in range(3)]))
+    save_cases("decode_algo3.npz", cases3, degree_table=np.array([c[9] for c in cases3], np.int32),
+               degree_alpha=np.stack(dalpha))
 
 
 def main():
@@ -369,6 +533,7 @@ def main():
     gen_encoder(ref, rng)
     gen_rate_recovery(ref, rng)
     gen_decode(ref, rng)
+    gen_decode_tables(ref, np.random.default_rng(20261204))
 
 
 if __name__ == "__main__":

@@ -29,6 +29,14 @@ def lib():
         L.emu_unrolled_backward.restype = I
         L.emu_synth.argtypes = [I] * 7 + [P, ctypes.c_uint64] + [P] * 7
         L.emu_synth.restype = I
+        L.emu_set_trained_tables.argtypes = [I, P, P, I]
+        L.emu_set_trained_tables.restype = I
+        L.emu_set_degree_alpha.argtypes = [P]
+        L.emu_set_degree_alpha.restype = None
+        L.emu_row_general.argtypes = [P]
+        L.emu_row_general.restype = I
+        L.emu_force_general.argtypes = [I]
+        L.emu_force_general.restype = None
         _lib = L
     return _lib
 
@@ -57,7 +65,7 @@ def decode_batch(cfgs, row, llrs, soft_pool=None, soft_slots=None, want_lq=True,
     st = np.zeros(n, np.int32)
     it = np.zeros(n, np.int32)
     nlq = sum(p["N"] + 2 * p["Zc"] for p in ps)
-    lq = np.zeros(nlq, np.int16 if row[0] == 1 else np.float32) if want_lq else None
+    lq = np.zeros(nlq, np.int16 if row[0] in (1, 2, 3) else np.float32) if want_lq else None
     hoff = np.zeros(n, np.uint32)
     loff = np.zeros(n, np.uint32)
     nwork = sum((316 if c[0] == 1 else 197) * p["Zc"] for c, p in zip(cfgs, ps))
@@ -139,3 +147,28 @@ def synth(cfg, n, noise_db, seed=0, payload=None, beliefs=False):
     if beliefs:
         out["beliefs"] = bel
     return out
+
+
+def set_trained_tables(bgn, alpha=None, beta=None):
+    """Algo-2 tables [iterations, rows] of the CPU build (None: defaults)."""
+    if alpha is None:
+        lib().emu_set_trained_tables(bgn, None, None, 0)
+        return
+    a = np.ascontiguousarray(alpha, np.int16)
+    b = np.ascontiguousarray(beta, np.int16)
+    if lib().emu_set_trained_tables(bgn, _p(a), _p(b), a.shape[0]) != 0:
+        raise ValueError("invalid tables")
+
+
+def set_degree_alpha(values=None):
+    lib().emu_set_degree_alpha(None if values is None else
+                               _p(np.ascontiguousarray(values, np.int16)))
+
+
+def row_general(row):
+    return bool(lib().emu_row_general(_p(np.array(row, np.int32))))
+
+
+def force_general(g):
+    """Force the int16 row form: True exact, False fast, None the library's."""
+    lib().emu_force_general(-1 if g is None else int(bool(g)))

@@ -34,6 +34,7 @@ EdgeTables &tables() {
 struct Batch {
     std::vector<CbDesc> descs;
     DecParams p;
+    ScaleSet scale;   // algos 2/3
     int max_zc = 2, max_soft = 0;
     uint64_t work = 0;
 };
@@ -45,6 +46,13 @@ int make_batch(int n, const int32_t *cfgs, const uint32_t *llr_off, const int32_
     ldpc_decoder_cfg_t dc{(uint16_t)row[0], (uint16_t)row[1], (uint16_t)row[2], (uint16_t)row[3],
                           (int16_t)row[4], (int16_t)row[5], (int16_t)row[6]};
     b.p = decoder_params(dc);
+    if (dc.algo == 2 || dc.algo == 3) {
+        make_scale_set(dc, b.scale);
+        b.p.scale[0] = b.scale.words.data() + b.scale.off[0];
+        b.p.scale[1] = b.scale.words.data() + b.scale.off[1];
+        b.p.scale_iters = b.scale.iters;
+        b.p.general = b.scale.fast ? 0 : 1;
+    }
     b.descs.resize(n);
     uint32_t h = 0, q = 0;
     for (int i = 0; i < n; ++i) {
@@ -53,6 +61,8 @@ int make_batch(int n, const int32_t *cfgs, const uint32_t *llr_off, const int32_
                             (uint16_t)c[4], (uint16_t)c[5], (uint16_t)c[6]};
         Params pr;
         if (derive(in, pr)) return -2;
+        if ((dc.algo == 2 || dc.algo == 3) && !scale_iterations_ok(b.scale, pr.bgn, dc.max_iterations))
+            return -2;
         CbDesc d = codeblock_desc(pr, tables().off[pr.bgn - 1][pr.Zc]);
         d.llr_off = llr_off[i];
         d.hard_off = h;
@@ -86,6 +96,7 @@ DecOut outputs(const Batch &b, const int16_t *llr, int16_t *soft_pool, uint8_t *
     return o;
 }
 
+int g_force_general = -1;   // -1: the library's choice; 0 / 1 forces a path
 int g_fixed_k = 0;
 int fixed_k(int bgn, int zc) {
     (void)bgn;
@@ -109,15 +120,29 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     const DecOut o = outputs(b, llr, soft_pool, hard, status, iters, lq, work);
     const DecParams p = b.p;
     const unsigned threads = (unsigned)workgroup_threads(b.max_zc);
+    const size_t lds16 = (size_t)b.max_soft * 2;
     switch (row[0]) {
     case 1:
-        if (p.alpha >= 0 && p.min_llr <= p.max_llr)
-            emu_launch(n, threads, (size_t)b.max_soft * 2,
-                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false>(o, p); });
+    case 2:
+    case 3: {
+        // the launcher's choice (nrldpc_launch_decode): DecParams::general
+        // and the table flag of algos 2/3
+        const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
+        const bool t = row[0] != 1;
+        if (!g && !t)
+            emu_launch(n, threads, lds16,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>(o, p); });
+        else if (g && !t)
+            emu_launch(n, threads, lds16,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, false>(o, p); });
+        else if (!g && t)
+            emu_launch(n, threads, lds16,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, true>(o, p); });
         else
-            emu_launch(n, threads, (size_t)b.max_soft * 2,
-                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true>(o, p); });
+            emu_launch(n, threads, lds16,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, true>(o, p); });
         return 0;
+    }
     case 4:
         emu_launch(n, threads, (size_t)b.max_soft * 4,
                    [&] { ldpc_decode_kernel<Algo::kMinSumF32, float>(o, p); });
@@ -131,14 +156,14 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     }
 }
 
-// Same batch through the packed-tile kernel (algo 1) with k codeblocks per
-// tile (0 = as many as fit).  Returns the number of tiles, or < 0.
+// Same batch through the packed-tile kernel (algos 1-3) with k codeblocks
+// per tile (0 = as many as fit).  Returns the number of tiles, or < 0.
 extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_off,
                                 const int32_t *row, const int16_t *llr, int16_t *soft_pool,
                                 const int32_t *soft_slot, uint8_t *hard, int32_t *status,
                                 int32_t *iters, void *lq, uint32_t *hard_off, uint32_t *lq_off,
                                 int k) {
-    if (row[0] != 1) return -3;
+    if (row[0] < 1 || row[0] > 3) return -3;
     Batch b;
     if (int rc = make_batch(n, cfgs, llr_off, row, soft_slot, lq != nullptr, hard_off, lq_off, b))
         return rc;
@@ -151,11 +176,36 @@ extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_
     const DecParams p = b.p;
     const TileDesc *td = tiles.data();
     const uint32_t *ord = order.data();
-    if (p.alpha >= 0 && p.min_llr <= p.max_llr)
-        emu_launch((unsigned)tiles.size(), kMaxThreads, (size_t)lds_entries * 2,
-                   [&] { ldpc_tile_kernel<false>(o, td, ord, p); });
+    const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
+    const bool t = row[0] != 1;
+    const unsigned nt = (unsigned)tiles.size();
+    const size_t lds = (size_t)lds_entries * 2;
+    if (!g && !t)
+        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p); });
+    else if (g && !t)
+        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p); });
+    else if (!g && t)
+        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<false, true>(o, td, ord, p); });
     else
-        emu_launch((unsigned)tiles.size(), kMaxThreads, (size_t)lds_entries * 2,
-                   [&] { ldpc_tile_kernel<true>(o, td, ord, p); });
+        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<true, true>(o, td, ord, p); });
     return (int)tiles.size();
+}
+
+// The library's process-wide algo-2 / algo-3 tables (this build's copy).
+extern "C" int emu_set_trained_tables(int bgn, const int16_t *alpha, const int16_t *beta,
+                                      int iterations) {
+    return set_trained_tables(bgn, alpha, beta, iterations);
+}
+
+extern "C" void emu_set_degree_alpha(const int16_t *values) { set_degree_alpha(values); }
+
+// Forces the int16 row form (0 fast, 1 exact) for negative controls; -1
+// restores the library's choice.
+extern "C" void emu_force_general(int g) { g_force_general = g; }
+
+// DecParams::general of a decoder row (algo 1) as the library computes it.
+extern "C" int emu_row_general(const int32_t *row) {
+    ldpc_decoder_cfg_t dc{(uint16_t)row[0], (uint16_t)row[1], (uint16_t)row[2], (uint16_t)row[3],
+                          (int16_t)row[4], (int16_t)row[5], (int16_t)row[6]};
+    return decoder_params(dc).general;
 }

@@ -44,7 +44,7 @@ def abi_decode(L, llr, cfg, row, harq_in=None, sentinel=0xA5):
         hv[:len(harq_in)] = harq_in
     hard = np.full(B // 8, sentinel, np.uint8)
     it = ctypes.c_int32(-1)
-    lq = np.zeros(68 * 384, np.int16 if row[0] == 1 else np.float32)
+    lq = np.zeros(68 * 384, np.int16 if row[0] in (1, 2, 3) else np.float32)
     llr = np.ascontiguousarray(llr, np.int16)
     st = L.ldpc_mi355_decode_ex(ptr(llr), ptr(hard), ctypes.byref(LdpcInConfig(*cfg)),
                                 ctypes.byref(DecoderConfig(*row)), ctypes.byref(it), ptr(lq))

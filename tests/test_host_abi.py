@@ -184,3 +184,46 @@ def test_synth_batch_codewords(oracle_mod):
         out = oracle_mod.decode_path(llr1[i], cfg, [1, 8, 22936, 0, 1280, 32767, -32767])
         assert out["status"] == 0
         assert np.array_equal(np.unpackbits(out["hard"])[:1016], pay1[i])
+
+
+def test_default_scale_tables_equal_reference_defaults(oracle_mod):
+    """The library's algo-2 / algo-3 defaults equal the tables the reference
+    ships (its trained-table fixtures carry BG1/BG2 default tables as decoded
+    by the from-source reference build)."""
+    from deeplearning_5g_ldpc_decoder_amd import tables
+    z = np.load(os.path.join(GOLDEN, "decode_algo2.npz"))
+    for bgn in (1, 2):
+        a, b = tables.get_trained_tables(bgn)
+        assert a.shape == (32, 46 if bgn == 1 else 42)
+        assert np.array_equal(a, z[f"alpha_bg{bgn}"][0])
+        assert np.array_equal(b, z[f"beta_bg{bgn}"][0])
+    d = np.load(os.path.join(GOLDEN, "decode_algo3.npz"))["degree_alpha"][0]
+    assert np.array_equal(tables.get_degree_alpha(), d)
+    assert np.array_equal(d, oracle_mod.default_degree_alpha())
+
+
+def test_scale_table_setters_round_trip(L):
+    from deeplearning_5g_ldpc_decoder_amd import tables
+    rng = np.random.default_rng(3)
+    try:
+        a = rng.integers(-32768, 32768, (7, 42)).astype(np.int16)
+        b = rng.integers(-32768, 32768, (7, 42)).astype(np.int16)
+        tables.set_trained_tables(2, a, b)
+        ga, gb = tables.get_trained_tables(2)
+        assert np.array_equal(ga, a) and np.array_equal(gb, b)
+        assert tables.get_trained_tables(1)[0].shape == (32, 46)   # BG1 untouched
+        with pytest.raises(ValueError):
+            tables.set_trained_tables(2, a[:, :41], b[:, :41])
+        assert L.ldpc_mi355_set_trained_tables(3, ptr(a), ptr(b), 7) == -4
+        assert L.ldpc_mi355_set_trained_tables(2, ptr(a), ptr(b), 0) == -4
+        assert L.ldpc_mi355_set_trained_tables(2, ptr(a), ptr(b), 1025) == -4
+        assert L.ldpc_mi355_set_trained_tables(2, ptr(a), None, 7) == -4
+        assert L.ldpc_mi355_get_trained_tables(0, None, None, 0) == -4
+        v = rng.integers(-32768, 32768, 19).astype(np.int16)
+        tables.set_degree_alpha(v)
+        assert np.array_equal(tables.get_degree_alpha(), v)
+        assert L.ldpc_mi355_get_degree_alpha(None) == -4
+    finally:
+        tables.set_trained_tables(2)
+        tables.set_degree_alpha()
+    assert tables.get_trained_tables(2)[0].shape == (32, 42)

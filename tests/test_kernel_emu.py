@@ -19,6 +19,8 @@ import emu  # noqa: E402
 ALGO1 = load_decode_cases("decode_algo1.npz")
 ALGO4 = load_decode_cases("decode_algo4.npz")
 ALGO0 = load_decode_cases("decode_algo0.npz")
+ALGO2 = load_decode_cases("decode_algo2.npz")
+ALGO3 = load_decode_cases("decode_algo3.npz")
 
 
 def _small(c):
@@ -37,7 +39,7 @@ def _run_case(case):
     assert out["iters"] == case["iters"]
     if case["row"][1]:
         assert np.array_equal(out["hard"], case["hard"])
-    if case["row"][0] == 1:
+    if case["row"][0] in (1, 2, 3):
         assert np.array_equal(out["lq"], case["lq"])
     else:
         assert np.array_equal(out["lq"].view(np.uint32), case["lq"].view(np.uint32))
@@ -62,6 +64,96 @@ def test_emu_algo4_matches_reference(case):
 @pytest.mark.parametrize("case", [c for c in ALGO0 if _small(c)], ids=case_id)
 def test_emu_algo0_matches_reference(case):
     _run_case(case)
+
+
+@pytest.fixture
+def case_tables():
+    """Loads a fixture case's algo-2 / algo-3 tables into the CPU build and
+    restores the defaults afterwards."""
+    def load(case):
+        if "tables" in case:
+            emu.set_trained_tables(case["cfg"][0], *case["tables"])
+        if "degree_alpha" in case:
+            emu.set_degree_alpha(case["degree_alpha"])
+    yield load
+    emu.set_trained_tables(1)
+    emu.set_trained_tables(2)
+    emu.set_degree_alpha(None)
+
+
+@pytest.mark.parametrize("case", [c for c in ALGO2 + ALGO3
+                                  if c["cfg"][1] <= 1100 or c["harq_in"] is not None],
+                         ids=case_id)
+def test_emu_table_algos_match_reference(case, case_tables):
+    """Algo 2 (trained tables: reference defaults, exported smooth and
+    adversarial sets) and algo 3 (default and custom degree alpha); the
+    larger fixture codes run on the GPU only."""
+    case_tables(case)
+    _run_case(case)
+
+
+def test_emu_algo2_refuses_iterations_past_table():
+    case = ALGO2[0]
+    with pytest.raises(RuntimeError):
+        emu.decode_batch([case["cfg"]], [2, 33, 0, 0, 1280, 32767, -32767], [case["llr"]])
+
+
+def test_fast_path_choice_covers_message_wrap(oracle_mod):
+    """alpha 22936, beta 40000 (-25536 as int16): a check whose smallest
+    magnitude is 21279 scales to 32768, which the reference stores as int16
+    -32768 and negates to -32768 again.  The library must pick the exact row
+    form for this decoder row; the fast form (forced here as a negative
+    control) gets the negated message wrong."""
+    row = [1, 1, 22936, 40000, 21279, 32767, -32767]   # filler bits at 21279 too
+    assert emu.row_general(row)
+    assert not emu.row_general([1, 1, 22936, 0, 0, 32767, -32767])
+    assert not emu.row_general([1, 1, 32767, 65535, 0, 32767, -32767])   # beta -1: 32767 max
+    assert emu.row_general([1, 1, 32767, 65534, 0, 32767, -32767])       # beta -2: 32768
+    cfg = (2, 16, 0, 200, 0, 2, 1)
+    c = oracle_mod.code(*cfg)
+    n = c.N + 2 * c.Zc
+    soft = np.full(n, 21279, np.int16)
+    soft[1::3] = -21279
+    llr = np.zeros(200, np.int16)
+    s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr, soft))
+    results = {}
+    for g in (None, False):
+        emu.force_general(g)
+        try:
+            pool = np.zeros(68 * 384, np.int16)
+            pool[:n] = soft
+            results[g] = emu.decode_batch([cfg], row, [llr], soft_pool=pool, soft_slots=[0])[0]
+        finally:
+            emu.force_general(None)
+    assert np.array_equal(results[None]["lq"], q)
+    assert not np.array_equal(results[False]["lq"], q)
+
+
+@pytest.mark.parametrize("algo", [2, 3])
+def test_emu_packed_tiles_table_algos(oracle_mod, algo):
+    """Packed tiles over the lifting sizes with the adversarial algo-2 tables
+    (exact row form) and custom degree alpha, bit-exact with the oracle."""
+    rng = np.random.default_rng(40 + algo)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=2, zcs=[2, 7, 26, 88])
+    z = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo2.npz"))
+    tabs = {b: (z[f"alpha_bg{b}"][2], z[f"beta_bg{b}"][2]) for b in (1, 2)}
+    dz = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo3.npz"))["degree_alpha"][1]
+    row = [algo, 4, 0, 24, 1280, 32767, -32767]
+    try:
+        for b in (1, 2):
+            emu.set_trained_tables(b, *tabs[b])
+        emu.set_degree_alpha(dz)
+        outs = emu.decode_batch(cfgs, row, llrs, tile_k=0)
+    finally:
+        emu.set_trained_tables(1)
+        emu.set_trained_tables(2)
+        emu.set_degree_alpha(None)
+    for cfg, llr, out in zip(cfgs, llrs, outs):
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr),
+                                            tables=tabs[cfg[0]], degree_alpha=dz)
+        assert (out["status"], out["iters"]) == (s, k), cfg
+        assert np.array_equal(out["lq"], q), cfg
 
 
 MIXED = [(1, 8424, 2, 25344, 0, 2), (2, 1016, 2, 3120, 1, 2), (1, 488, 1, 1536, 2, 4),

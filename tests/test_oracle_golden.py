@@ -13,6 +13,8 @@ from conftest import GOLDEN, case_id, load_decode_cases
 ALGO1 = load_decode_cases("decode_algo1.npz")
 ALGO4 = load_decode_cases("decode_algo4.npz")
 ALGO0 = load_decode_cases("decode_algo0.npz")
+ALGO2 = load_decode_cases("decode_algo2.npz")
+ALGO3 = load_decode_cases("decode_algo3.npz")
 
 
 def test_params_table_matches_reference(oracle_mod):
@@ -118,6 +120,63 @@ def test_algo1_matches_reference(oracle_mod, case):
     assert np.array_equal(out["hard"], case["hard"]) or case["row"][1] == 0
     assert out["iters"] == case["iters"]
     assert np.array_equal(out["lq"], case["lq"])
+
+
+@pytest.mark.parametrize("case", ALGO2 + ALGO3, ids=case_id)
+def test_table_algos_match_reference(oracle_mod, case):
+    """Algo 2 with the reference's default tables and with two exported table
+    sets linked into the reference build; algo 3 with the default and custom
+    degree alpha tables (layered_degree_minsum_ldpc_decode_with_alpha)."""
+    out = oracle_mod.decode_path(case["llr"], case["cfg"], case["row"], harq=case["harq_in"],
+                                 tables=case.get("tables"),
+                                 degree_alpha=case.get("degree_alpha"))
+    assert out["status"] == case["status"]
+    assert np.array_equal(out["soft"], case["soft"])
+    assert np.array_equal(out["hard"], case["hard"]) or case["row"][1] == 0
+    assert out["iters"] == case["iters"]
+    assert np.array_equal(out["lq"], case["lq"])
+
+
+def test_table_fixtures_cover_the_table_kinds():
+    """Default, smooth and adversarial trained tables; converged and
+    non-converged decodes; default and custom degree tables."""
+    assert {c["table_id"] for c in ALGO2} == {0, 1, 2}
+    assert {c["table_id"] for c in ALGO3} == {-1, 1, 2}
+    for cs in (ALGO2, ALGO3):
+        st = [c["status"] for c in cs]
+        assert st.count(0) >= 10 and st.count(1) >= 10
+    # the reference's default trained tables are the degree rule, beta 0
+    t = ALGO2[0]["tables"] if ALGO2[0]["table_id"] == 0 else None
+    assert t is not None and not t[1].any()
+
+
+def test_uniform_tables_reduce_to_algo1(oracle_mod):
+    """An algo-2 table holding one (alpha, beta) pair everywhere, and an
+    algo-3 degree table holding one alpha, decode exactly like algo 1 with
+    that pair (algo 1 is pinned by its own fixtures)."""
+    for case in ALGO1[8:30]:
+        row = case["row"]
+        if row[1] == 0:
+            continue
+        bgn = case["cfg"][0]
+        rows = 46 if bgn == 1 else 42
+        a = np.full((max(row[1], 1), rows), np.int16(np.uint16(row[2]).view(np.int16)))
+        b = np.full_like(a, np.int16(np.uint16(row[3]).view(np.int16)))
+        o2 = oracle_mod.decode_path(case["llr"], case["cfg"], [2] + row[1:],
+                                    harq=case["harq_in"], tables=(a, b))
+        o3 = oracle_mod.decode_path(case["llr"], case["cfg"], [3] + row[1:],
+                                    harq=case["harq_in"], degree_alpha=np.full(19, a[0, 0]))
+        for o in (o2, o3):
+            assert o["status"] == case["status"] and o["iters"] == case["iters"]
+            assert np.array_equal(o["lq"], case["lq"])
+
+
+def test_algo2_refuses_iterations_past_its_table(oracle_mod):
+    case = ALGO2[0]
+    a, b = oracle_mod.default_trained_tables(case["cfg"][0], 8)
+    out = oracle_mod.decode_path(case["llr"], case["cfg"], [2, 9, 0, 0, 1280, 32767, -32767],
+                                 tables=(a, b))
+    assert out["status"] == -2
 
 
 @pytest.mark.parametrize("case", ALGO4 + ALGO0, ids=case_id)
