@@ -324,16 +324,20 @@ def bench_serve(args):
 def bench_bler(args):
     """FP32 min-sum (algo 4, 20 iterations, CRC early stop) BLER sweep of the
     BG2 workload: Zc 104 / 208 (B = 1040 / 2080), R 0.2 / 0.5, Qm 2 / 4 (Gray
-    QAM, max-log LLRs), 8 SNR points over [-2, 3] dB (the harness's
-    snr_power_db: noise_power_db = -snr - 10 log10 Qm), args.batch codeblocks
-    per point (default 4096), each point one batched launch; BLER with its
-    99 % Wilson interval, and decode throughput."""
+    QAM, max-log LLRs), 8 SNR points per config spanning its waterfall
+    (Es/N0 from 2.5 dB below to 1.5 dB above the waterfall model of
+    workloads.py; reported as the harness's snr_power_db, noise_power_db =
+    -snr - 10 log10 Qm), args.batch codeblocks
+    per point (default 4096) synthesized on the GPU (ldpc_mi355_synth_device),
+    each point one batched launch; BLER with its 99 % Wilson interval, and
+    decode throughput."""
     import torch
-    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, synth_batch
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, synth_device
     from deeplearning_5g_ldpc_decoder_amd.ldpc_sim import bler_confidence_interval
+    from deeplearning_5g_ldpc_decoder_amd.workloads import waterfall_esn0_db
     row = [4, 20, 22936, 0, 1280, 32767, -32767]
     n = args.batch if args.batch != BATCH else 4096
-    snrs = np.linspace(-2.0, 3.0, 8)
+    npts = 8
     table = []
     gpu_s, info = 0.0, 0
     for A in (1016, 2056):
@@ -342,13 +346,14 @@ def bench_bler(args):
                 B = A + 24
                 E = int(round(B / R / qm)) * qm
                 cfg = (2, A, 2, E, 0, qm, 0)
+                wf = waterfall_esn0_db(B / E, qm, B)
+                snrs = np.linspace(wf - 2.5, wf + 1.5, npts) - 10 * np.log10(qm)
                 plan = BatchPlan([cfg] * n, row)
                 out = plan.alloc_outputs("cuda")
                 pts = []
                 for si, snr in enumerate(snrs):
-                    llr, pay = synth_batch(cfg, n, float(-snr - 10 * np.log10(qm)), seed=si,
-                                           threads=min(16, os.cpu_count() or 8))
-                    d = torch.from_numpy(llr.reshape(-1)).cuda()
+                    d = synth_device(cfg, n, float(-snr - 10 * np.log10(qm)),
+                                     seed=1000 * si + qm)["llr"].reshape(-1)
                     plan.launch(d, out)
                     torch.cuda.synchronize()
                     t = time.perf_counter()
@@ -367,8 +372,8 @@ def bench_bler(args):
                     " ".join(f"{p['bler']:.3f}" for p in pts))
     line = {"metric": "FP32 min-sum decoded info Gbps over the BG2 BLER sweep",
             "value": info / gpu_s / 1e9, "unit": "Gbps", "n_gpus": 1,
-            "steps": len(table) * len(snrs), "warmup": 1,
-            "ms_per_step": gpu_s * 1e3 / (len(table) * len(snrs)), "higher_is_better": True,
+            "steps": len(table) * npts, "warmup": 1,
+            "ms_per_step": gpu_s * 1e3 / (len(table) * npts), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
             "config": {"workload": f"bler_bg2_zc104_208_fp32ms_20it_batch{n}",
                        "decoder_row": row, "parallelism": "single-gpu"},
