@@ -59,6 +59,7 @@ class BatchPlan:
 
     PACK = 2        # plan flag: pack codeblocks of one (BG, Zc) per workgroup
     NO_PACK = 4     # plan flag: one codeblock per workgroup
+    GENERIC = 8     # plan flag: no kernel specialized for a single (BG, Zc)
 
     def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False,
                  flags=0):

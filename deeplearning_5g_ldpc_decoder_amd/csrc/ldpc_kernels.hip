@@ -33,6 +33,8 @@
 #define NRLDPC_DYNAMIC_LDS(name) extern __shared__ __align__(16) unsigned char name[]
 #endif
 
+#include <utility>
+
 #include "ldpc_device.h"
 #include "ldpc_graph.h"
 #include "ldpc_crc.h"
@@ -42,6 +44,15 @@
 // algo 0 then fails with an explicit error instead of running.
 #ifndef NRLDPC_BP_KERNEL
 #define NRLDPC_BP_KERNEL 0
+#endif
+
+// Opaque per-row redefinition of a loop-invariant value: keeps the compiler
+// from hoisting every row's (constant-offset) address arithmetic out of the
+// iteration loop, which would hold hundreds of registers across it.
+#ifdef NRLDPC_CPU_EMU
+#define NRLDPC_OPAQUE_LANE(x) (void)(x)
+#else
+#define NRLDPC_OPAQUE_LANE(x) asm volatile("" : "+v"(x))
 #endif
 
 namespace nrldpc {
@@ -146,7 +157,9 @@ __device__ inline int scale_i16(int m, int alpha, int beta) {
 #ifdef NRLDPC_CPU_EMU
     const int scaled = alpha * off;
 #else
-    const int scaled = __mul24(alpha, off);
+    // full-rate 24-bit multiply (__mul24 lowers to a quarter-rate v_mul_lo_u32 here)
+    int scaled;
+    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(scaled) : "v"(alpha), "v"(off));
 #endif
     return wrap16((scaled + (1 << 14)) >> 15);
 }
@@ -272,6 +285,209 @@ __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const
     while (it < p.max_it) {
         rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, d.Zc, zx < d.Zc, p,
                                           scale_row<BG, kTable>(p, it));
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Lifting-size specialized int16 decode: every codeblock of the launch has
+// base graph BG and lifting size ZC, so each edge's circulant shift and
+// column offset are compile-time constants (LDS addresses become immediates,
+// no per-edge scalar work), and each lane keeps its checks' messages
+// explicitly, one int16 per edge in half a VGPR, instead of re-expanding
+// them from (min1', min2', argmin, signs).  Arithmetic, schedule and results
+// are those of row_i16 (c_lib/fixedpoint_ldpc_decoder.c:103-144).
+// ---------------------------------------------------------------------------
+
+// Lifting-size set index of TS 38.212 Table 5.3.2-1: zc = a * 2^j with
+// a = 2, 3, 5, 7, 9, 11, 13, 15 for sets 0..7; -1 if zc is not a lifting size.
+__host__ __device__ constexpr int lifting_set_ct(int zc) {
+    constexpr int a[8] = {2, 3, 5, 7, 9, 11, 13, 15};
+    for (int i = 0; i < 8; ++i) {
+        if (zc <= 0 || zc > kMaxZc || zc % a[i]) continue;
+        const int q = zc / a[i];
+        if ((q & (q - 1)) == 0) return i;
+    }
+    return -1;
+}
+static_assert(lifting_set_ct(2) == 0 && lifting_set_ct(384) == 1 && lifting_set_ct(320) == 2 &&
+                  lifting_set_ct(224) == 3 && lifting_set_ct(288) == 4 &&
+                  lifting_set_ct(352) == 5 && lifting_set_ct(208) == 6 &&
+                  lifting_set_ct(240) == 7 && lifting_set_ct(12) == 1 && lifting_set_ct(17) == -1,
+              "lifting-size sets");
+
+template <int BG, int ZC>
+struct FixedCode {
+    static constexpr int kILS = lifting_set_ct(ZC);
+    static_assert(kILS >= 0, "ZC is not a lifting size");
+    static constexpr int kMsgWords = (Graph<BG>::kEdges + 1) / 2;
+    // byte offset of edge e's column block and its shift in bytes
+    __host__ __device__ static constexpr uint32_t col_bytes(int e) {
+        return 2u * (uint32_t)Graph<BG>::col(e) * (uint32_t)ZC;
+    }
+    __host__ __device__ static constexpr uint32_t shift_bytes(int e) {
+        return 2u * (uint32_t)((BG == 1 ? kBG1Shift[kILS][e] : kBG2Shift[kILS][e]) % ZC);
+    }
+};
+
+// The fixed kernel has no static LDS, so its dynamic LDS -- the beliefs --
+// starts at LDS address 0 (checked on the host when the kernels are
+// configured).  Addressing the beliefs from a constant LDS address 0 lets the
+// compiler fold each edge's column offset into the DS instruction's
+// immediate; from the array's symbol it adds the (link-time) base per access.
+#ifdef NRLDPC_CPU_EMU
+typedef char lds_char;
+typedef int16_t lds_i16;
+__device__ __forceinline__ lds_char *fixed_lds_base(int16_t *lq) {
+    return reinterpret_cast<char *>(lq);
+}
+#else
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(3))) int16_t lds_i16;
+__device__ __forceinline__ lds_char *fixed_lds_base(int16_t *) {
+    return (lds_char *)(uintptr_t)0;
+}
+#endif
+
+// wrap16(a - m) for the int16 message m held in the low / high half of pair:
+// one SDWA subtract that reads the half directly.
+__device__ __forceinline__ int sub_wrap16_lo(int a, uint32_t pair) {
+#ifdef NRLDPC_CPU_EMU
+    return wrap16(a - (int)(int16_t)(pair & 0xFFFFu));
+#else
+    int r;
+    asm("v_sub_u32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_SEXT "
+        "src0_sel:DWORD src1_sel:WORD_0"
+        : "=v"(r) : "v"(a), "v"(pair));
+    return r;
+#endif
+}
+
+__device__ __forceinline__ int sub_wrap16_hi(int a, uint32_t pair) {
+#ifdef NRLDPC_CPU_EMU
+    return wrap16(a - (int)(pair >> 16));
+#else
+    int r;
+    asm("v_sub_u32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_SEXT "
+        "src0_sel:DWORD src1_sel:WORD_1"
+        : "=v"(r) : "v"(a), "v"(pair));
+    return r;
+#endif
+}
+
+// Stores the low 16 bits of m as edge E's message (the reference's int16
+// store, wrap included).
+template <int E, int N>
+__device__ __forceinline__ void msg_store(uint32_t (&msg)[N], int m) {
+    if constexpr (E & 1)
+        msg[E >> 1] = (msg[E >> 1] & 0xFFFFu) | ((uint32_t)m << 16);
+    else
+        msg[E >> 1] = (msg[E >> 1] & 0xFFFF0000u) | ((uint32_t)m & 0xFFFFu);
+}
+
+// Per-row working set between the two passes of a fixed row.
+template <int DEG>
+struct FixedRowState {
+    uint32_t off[DEG];   // byte offset of the edge's variable inside its column block
+    int rel[DEG];        // extrinsic input wrap16(LQ - old message)
+    uint32_t k1, k2, negs;
+};
+
+template <int BG, int ZC, int E, int J, int DEG>
+__device__ __forceinline__ void fixed_edge_in(const lds_char *__restrict__ lqb, const uint32_t *msg,
+                                              uint32_t zx2, FixedRowState<DEG> &rs) {
+    using F = FixedCode<BG, ZC>;
+    constexpr uint32_t sh = F::shift_bytes(E), cb = F::col_bytes(E);
+    const uint32_t c = zx2 + sh;
+    const uint32_t o = min(c, c - 2u * (uint32_t)ZC);
+    rs.off[J] = o;
+    const int q = *(const lds_i16 *)(lqb + cb + o);
+    const int v = (E & 1) ? sub_wrap16_hi(q, msg[E >> 1]) : sub_wrap16_lo(q, msg[E >> 1]);
+    rs.rel[J] = v;
+    rs.negs = push_sign(rs.negs, v);
+    const uint32_t key = ((uint32_t)max(v, -v) << 5) | (uint32_t)J;
+    rs.k2 = med3_u32(rs.k1, key, rs.k2);
+    rs.k1 = min(rs.k1, key);
+}
+
+template <int BG, int ZC, int E, int J, int DEG, bool kGeneral, int N>
+__device__ __forceinline__ void fixed_edge_out(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                               const FixedRowState<DEG> &rs, int idx, int n1,
+                                               int n2, uint32_t bits, const DecParams &p) {
+    using F = FixedCode<BG, ZC>;
+    constexpr uint32_t cb = F::col_bytes(E);
+    const int nv = (J == idx) ? n2 : n1;
+    const int s = sext_bit(bits, DEG - 1 - J);
+    const int m = (nv ^ s) - s;
+    msg_store<E>(msg, m);
+    int v;
+    if constexpr (kGeneral) {
+        v = rs.rel[J] + wrap16(m);
+        v = v > p.max_llr ? p.max_llr : max(v, p.min_llr);
+    } else {
+        v = med3_i32(p.min_llr, rs.rel[J] + m, p.max_llr);
+    }
+    *(lds_i16 *)(lqb + cb + rs.off[J]) = (int16_t)v;
+}
+
+template <int BG, int ZC, int R, bool kGeneral, int N, int... J>
+__device__ __forceinline__ void fixed_row(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                          uint32_t zx2, const DecParams &p, int alpha, int beta,
+                                          std::integer_sequence<int, J...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = (int)sizeof...(J);
+    NRLDPC_OPAQUE_LANE(zx2);
+    FixedRowState<deg> rs;
+    rs.k1 = kKeyMax;
+    rs.k2 = kKeyMax;
+    rs.negs = 0;
+    (fixed_edge_in<BG, ZC, e0 + J, J, deg>(lqb, msg, zx2, rs), ...);
+    const int idx = (int)(rs.k1 & 31u);
+    const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
+    const int n1 = scale_i16(m1, alpha, beta), n2 = scale_i16(m2, alpha, beta);
+    constexpr uint32_t mask = (1u << deg) - 1u;
+    const uint32_t bits = ((__popc(rs.negs) & 1) ? ~rs.negs : rs.negs) & mask;
+    (fixed_edge_out<BG, ZC, e0 + J, J, deg, kGeneral>(lqb, msg, rs, idx, n1, n2, bits, p), ...);
+}
+
+template <int BG, int ZC, int R, bool kGeneral, bool kTable, int N>
+__device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], uint32_t zx2, bool run,
+                                           const DecParams &p, const uint32_t *ab) {
+    if constexpr (R < Graph<BG>::kRows) {
+        int alpha = p.alpha, beta = p.beta;
+        if constexpr (kTable) {
+            const uint32_t w = ab[R];
+            alpha = (int)(int16_t)(w & 0xFFFFu);
+            beta = (int)(int16_t)(w >> 16);
+        }
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        if (run)
+            fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
+                                           std::make_integer_sequence<int, deg>{});
+        __syncthreads();
+        fixed_rows<BG, ZC, R + 1, kGeneral, kTable>(lqb, msg, zx2, run, p, ab);
+    }
+}
+
+template <int BG, int ZC, bool kGeneral, bool kTable>
+__device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
+                             const uint32_t *tab, uint32_t *slots, int *status) {
+    uint32_t msg[FixedCode<BG, ZC>::kMsgWords];
+#pragma unroll
+    for (int i = 0; i < FixedCode<BG, ZC>::kMsgWords; ++i) msg[i] = 0u;
+    const int zx = threadIdx.x;
+    const uint32_t zx2 = 2u * (uint32_t)zx;
+    lds_char *lqb = fixed_lds_base(lq);
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        fixed_rows<BG, ZC, 0, kGeneral, kTable>(lqb, msg, zx2, zx < ZC, p,
+                                                scale_row<BG, kTable>(p, it));
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -543,6 +759,56 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     }
 }
 
+// Lifting-size specialized int16 decode kernel: one codeblock of base graph
+// BG and lifting size ZC per workgroup (run_fixed_i16).  All of its LDS is
+// dynamic with the beliefs at offset 0, so every belief access is a DS
+// instruction with an immediate column offset: LQ (N + 2Zc int16), then the
+// CRC table and the per-wave CRC slots (fixed_lds_bytes).
+template <int BG, int ZC>
+struct FixedLds {
+    static constexpr int kBeliefs = Graph<BG>::kCols * ZC;           // N + 2Zc
+    static constexpr int kTabOff = ((kBeliefs * 2 + 15) / 16) * 16;
+    static constexpr int kSlotsOff = kTabOff + 256 * 4;
+    static constexpr int kBytes = kSlotsOff + (kMaxThreads / 64) * 4;
+};
+
+template <int BG, int ZC, bool kGeneral, bool kTable>
+__global__ void __launch_bounds__(kMaxThreads)
+ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
+    using LdsL = FixedLds<BG, ZC>;
+    NRLDPC_DYNAMIC_LDS(smem);
+    int16_t *lq = reinterpret_cast<int16_t *>(smem);
+    uint32_t *crc_tab = reinterpret_cast<uint32_t *>(smem + LdsL::kTabOff);
+    uint32_t *slots = reinterpret_cast<uint32_t *>(smem + LdsL::kSlotsOff);
+
+    const CbDesc d = o.desc[blockIdx.x];
+    constexpr int n = LdsL::kBeliefs;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int16_t *llr = o.llr + d.llr_off;
+    int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+    for (int i = t; i < n; i += nt) {
+        const int16_t v = recover_one(i, d, llr, soft, p.filler);
+        if (soft) soft[i] = v;
+        lq[i] = v;
+    }
+    const CrcCtx crc = crc_setup(d.crctype, d.B / 8, crc_tab);
+    __syncthreads();
+
+    int status = 1, it = 0;
+    if (p.max_it > 0) {
+        it = run_fixed_i16<BG, ZC, kGeneral, kTable>(lq, p, crc, crc_tab, slots, &status);
+        for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
+    }
+    if (t == 0) {
+        o.status[blockIdx.x] = status;
+        o.iters[blockIdx.x] = it;
+    }
+    if (d.lq_off != kNone) {
+        int16_t *out = reinterpret_cast<int16_t *>(o.lq) + d.lq_off;
+        for (int i = t; i < n; i += nt) out[i] = lq[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Packed tiles (algo 1): several codeblocks of one (BG, Zc) per workgroup, for
 // batches of mixed sizes.  Each codeblock keeps its own LDS beliefs, CRC and
@@ -674,6 +940,39 @@ const void *i16_kernel(bool general, bool table) {
                          &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>);
 }
 
+// Lifting-size specialized kernels (ldpc_decode_fixed_kernel): BG1 at the
+// largest lifting size.
+template <int BG, int ZC>
+const void *fixed_kernel_of(bool general, bool table) {
+    if (table)
+        return general
+                   ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, true>)
+                   : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, true>);
+    return general
+               ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, false>)
+               : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false>);
+}
+
+constexpr uint32_t fixed_pair(int bg, int zc) { return ((uint32_t)bg << 16) | (uint32_t)zc; }
+constexpr uint32_t kFixedPairs[] = {fixed_pair(1, 384)};
+
+// Set by nrldpc_configure_kernels once every specialized kernel is known to
+// have no static LDS (their belief addressing assumes LDS address 0).
+bool g_fixed_ok = false;
+
+std::pair<const void *, int> fixed_kernel_any(bool general, bool table, uint32_t uniform) {
+    if (uniform == fixed_pair(1, 384))
+        return {fixed_kernel_of<1, 384>(general, table), FixedLds<1, 384>::kBytes};
+    return {nullptr, 0};
+}
+
+// (kernel, dynamic LDS bytes) of the specialized kernel for `uniform`, or
+// (nullptr, 0) when there is none.
+std::pair<const void *, int> fixed_kernel(bool general, bool table, uint32_t uniform) {
+    if (!g_fixed_ok) return {nullptr, 0};
+    return fixed_kernel_any(general, table, uniform);
+}
+
 const void *tile_kernel(bool general, bool table) {
     if (table)
         return general ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, true>)
@@ -685,13 +984,16 @@ const void *tile_kernel(bool general, bool table) {
 
 // Host-callable launchers (C linkage inside the library only).  Algos 1-3
 // share the int16 kernels (2 and 3 with per-row scale words).
+// uniform: (bgn << 16) | Zc when every codeblock of the launch shares one base
+// graph and lifting size, else 0; a specialized int16 kernel runs when one
+// exists for that pair.
 extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
                                            const nrldpc::DecParams *p, int ncb, int threads,
-                                           int max_soft, hipStream_t stream) {
+                                           int max_soft, uint32_t uniform, hipStream_t stream) {
     using namespace nrldpc;
     const bool i16 = algo >= 1 && algo <= 3;
     const size_t esz = i16 ? sizeof(int16_t) : sizeof(float);
-    const size_t lds = ((size_t)max_soft * esz + 15) & ~(size_t)15;
+    size_t lds = ((size_t)max_soft * esz + 15) & ~(size_t)15;
     const dim3 grid(ncb), block(threads);
     DecOut ov = *o;
     DecParams pv = *p;
@@ -704,6 +1006,13 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
         if (algo != 1 && (pv.scale_iters < 1 || !pv.scale[0] || !pv.scale[1]))
             return hipErrorInvalidValue;
         fn = i16_kernel(pv.general != 0, algo != 1);
+        if (uniform) {
+            const auto fk = fixed_kernel(pv.general != 0, algo != 1, uniform);
+            if (fk.first) {
+                fn = fk.first;
+                lds = (size_t)fk.second;
+            }
+        }
         break;
     case 4:
         fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>);
@@ -726,10 +1035,25 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     const int max_lds = 160 * 1024 - 2048;
     hipError_t e = hipSuccess;
     for (int g = 0; g < 2 && e == hipSuccess; ++g)
-        for (int t = 0; t < 2 && e == hipSuccess; ++t)
+        for (int t = 0; t < 2 && e == hipSuccess; ++t) {
             e = hipFuncSetAttribute(i16_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     max_lds);
+            for (uint32_t u : kFixedPairs)
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute(fixed_kernel_any(g, t, u).first,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        }
     if (e != hipSuccess) return e;
+    bool fixed_ok = true;
+    for (int g = 0; g < 2; ++g)
+        for (int t = 0; t < 2; ++t)
+            for (uint32_t u : kFixedPairs) {
+                hipFuncAttributes fa;
+                e = hipFuncGetAttributes(&fa, fixed_kernel_any(g, t, u).first);
+                if (e != hipSuccess) return e;
+                if (fa.sharedSizeBytes != 0) fixed_ok = false;
+            }
+    g_fixed_ok = fixed_ok;
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);

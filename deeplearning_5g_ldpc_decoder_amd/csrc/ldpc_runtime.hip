@@ -15,7 +15,7 @@
 using namespace nrldpc;
 
 extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecParams *p,
-                                           int ncb, int threads, int max_soft,
+                                           int ncb, int threads, int max_soft, uint32_t uniform,
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
 extern "C" int nrldpc_bp_kernel_enabled();
@@ -145,6 +145,8 @@ CbDesc make_desc(const Params &p, const Runtime &r) {
 
 int threads_for(int zc) { return workgroup_threads(zc); }
 
+uint32_t uniform_pair(int bgn, int zc) { return ((uint32_t)bgn << 16) | (uint32_t)zc; }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -210,7 +212,8 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     o.lq = r.d_out + OutLayout::lq;
     o.work = r.d_work;
     o.etab = r.d_etab;
-    HIP_OK(nrldpc_launch_decode(algo, &o, &dp, 1, threads_for(p.Zc), n, r.stream));
+    HIP_OK(nrldpc_launch_decode(algo, &o, &dp, 1, threads_for(p.Zc), n,
+                                uniform_pair(p.bgn, p.Zc), r.stream));
     // results: status/iterations/hard contiguous, then the soft buffer
     HIP_OK(hipMemcpyAsync(r.h_out, r.d_out, OutLayout::soft, hipMemcpyDeviceToHost, r.stream));
     HIP_OK(hipMemcpyAsync(r.h_out + OutLayout::soft, r.d_in + InLayout::soft, (size_t)n * 2,
@@ -249,6 +252,7 @@ extern "C" int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_con
 // ---------------------------------------------------------------------------
 struct ldpc_mi355_plan {
     int n = 0, algo = 1, threads = 64, max_soft = 0;
+    uint32_t uniform = 0;          // (bgn << 16) | Zc of a single-size batch, else 0
     DecParams dp;
     CbDesc *d_desc = nullptr;
     float *d_work = nullptr;
@@ -298,6 +302,7 @@ struct BatchLayout {
     std::vector<uint32_t> hard_off, lq_off;
     uint64_t hard_bytes = 0, lq_elems = 0, work = 0;
     int max_zc = 2, max_soft = 0;
+    uint32_t uniform = 0;   // (bgn << 16) | Zc when every codeblock shares them, else 0
 };
 
 int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, const Runtime &r,
@@ -334,6 +339,8 @@ int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, cons
         if (algo == 0) L.work += (uint64_t)p.edges * p.Zc;
         if (p.Zc > L.max_zc) L.max_zc = p.Zc;
         if (p.N + 2 * p.Zc > L.max_soft) L.max_soft = p.N + 2 * p.Zc;
+        const uint32_t u = uniform_pair(p.bgn, p.Zc);
+        L.uniform = (i == 0 || L.uniform == u) ? u : 0u;
         L.descs[i] = d;
     }
     return 0;
@@ -384,6 +391,7 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     pl->lq_elems = L.lq_elems;
     pl->max_soft = L.max_soft;
     pl->threads = threads_for(L.max_zc);
+    pl->uniform = (flags & 8u) ? 0u : L.uniform;
     if (hipMalloc(&pl->d_desc, sizeof(CbDesc) * (size_t)n) != hipSuccess ||
         hipMemcpy(pl->d_desc, L.descs.data(), sizeof(CbDesc) * (size_t)n,
                   hipMemcpyHostToDevice) != hipSuccess ||
@@ -460,7 +468,7 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
                                    plan->ntiles, plan->tile_lds, s));
     else
         HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
-                                    plan->max_soft, s));
+                                    plan->max_soft, plan->uniform, s));
     return 0;
 }
 
@@ -600,7 +608,7 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
                                    &b->dp, b->ntiles, tile_lds, s));
     else
         HIP_OK(nrldpc_launch_decode(b->algo, &o, &b->dp, n, threads_for(L.max_zc), L.max_soft,
-                                    s));
+                                    (b->flags & 8u) ? 0u : L.uniform, s));
     HIP_OK(hipEventRecord(b->done, s));
     b->pending = true;
     b->n = n;

@@ -225,7 +225,9 @@ typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
  * call).  flags: bit 0 = produce posteriors; bit 1 = pack codeblocks of one
  * (BG, Zc) into shared workgroups (int16 algos 1-3); bit 2 = never pack.
  * Without bits 1/2, int16 plans pack whenever some lifting size leaves room
- * for two or more codeblocks per workgroup.  Returns 0, -2 if any codeblock
+ * for two or more codeblocks per workgroup.  bit 3 = never use a kernel
+ * specialized for the batch's single (base graph, lifting size) (int16
+ * algos; used when all codeblocks share one, e.g. BG1 Zc=384).  Returns 0, -2 if any codeblock
  * config is invalid or, for algo 2, needs more iterations than its base
  * graph's trained table holds (index in *bad_index when non-NULL), -3 for
  * an unsupported algo, -1 on a GPU error. */

@@ -37,6 +37,8 @@ def lib():
         L.emu_row_general.restype = I
         L.emu_force_general.argtypes = [I]
         L.emu_force_general.restype = None
+        L.emu_use_fixed.argtypes = [I]
+        L.emu_use_fixed.restype = None
         _lib = L
     return _lib
 
@@ -172,3 +174,8 @@ def row_general(row):
 def force_general(g):
     """Force the int16 row form: True exact, False fast, None the library's."""
     lib().emu_force_general(-1 if g is None else int(bool(g)))
+
+
+def use_fixed(on):
+    """Whether uniform batches run the lifting-size specialized kernel."""
+    lib().emu_use_fixed(int(bool(on)))

@@ -37,6 +37,7 @@ struct Batch {
     ScaleSet scale;   // algos 2/3
     int max_zc = 2, max_soft = 0;
     uint64_t work = 0;
+    uint32_t uniform = 0;   // (bgn << 16) | Zc when all codeblocks share them
 };
 
 // Descriptors exactly as ldpc_mi355_plan_create lays them out.
@@ -76,6 +77,8 @@ int make_batch(int n, const int32_t *cfgs, const uint32_t *llr_off, const int32_
         b.work += (uint64_t)pr.edges * pr.Zc;
         b.max_zc = std::max(b.max_zc, pr.Zc);
         b.max_soft = std::max(b.max_soft, pr.N + 2 * pr.Zc);
+        const uint32_t u = ((uint32_t)pr.bgn << 16) | (uint32_t)pr.Zc;
+        b.uniform = (i == 0 || b.uniform == u) ? u : 0u;
         b.descs[i] = d;
     }
     return 0;
@@ -97,6 +100,35 @@ DecOut outputs(const Batch &b, const int16_t *llr, int16_t *soft_pool, uint8_t *
 }
 
 int g_force_general = -1;   // -1: the library's choice; 0 / 1 forces a path
+int g_use_fixed = 1;        // run the lifting-size specialized kernel when one exists
+
+// The specialized kernel of (BG, ZC) for a uniform batch.  The library
+// builds BG1 Zc=384; the small pairs exist here only, to test the template
+// cheaply on the CPU.
+template <int BG, int ZC>
+void launch_fixed(unsigned n, unsigned threads, const DecOut &o, const DecParams &p, bool g,
+                  bool t) {
+    const size_t lds = (size_t)FixedLds<BG, ZC>::kBytes;
+    if (!g && !t)
+        emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, false, false>(o, p); });
+    else if (g && !t)
+        emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, true, false>(o, p); });
+    else if (!g && t)
+        emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, false, true>(o, p); });
+    else
+        emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, true, true>(o, p); });
+}
+
+bool try_fixed(uint32_t uniform, unsigned n, unsigned threads, const DecOut &o,
+               const DecParams &p, bool g, bool t) {
+    if (!g_use_fixed) return false;
+    switch (uniform) {
+    case (1u << 16) | 384u: launch_fixed<1, 384>(n, threads, o, p, g, t); return true;
+    case (2u << 16) | 26u: launch_fixed<2, 26>(n, threads, o, p, g, t); return true;
+    case (1u << 16) | 15u: launch_fixed<1, 15>(n, threads, o, p, g, t); return true;
+    default: return false;
+    }
+}
 int g_fixed_k = 0;
 int fixed_k(int bgn, int zc) {
     (void)bgn;
@@ -129,6 +161,7 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
         // and the table flag of algos 2/3
         const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
         const bool t = row[0] != 1;
+        if (try_fixed(b.uniform, n, threads, o, p, g, t)) return 0;
         if (!g && !t)
             emu_launch(n, threads, lds16,
                        [&] { ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>(o, p); });
@@ -198,6 +231,9 @@ extern "C" int emu_set_trained_tables(int bgn, const int16_t *alpha, const int16
 }
 
 extern "C" void emu_set_degree_alpha(const int16_t *values) { set_degree_alpha(values); }
+
+// 0: never run a lifting-size specialized kernel (generic kernels only).
+extern "C" void emu_use_fixed(int on) { g_use_fixed = on; }
 
 // Forces the int16 row form (0 fast, 1 exact) for negative controls; -1
 // restores the library's choice.

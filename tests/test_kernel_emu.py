@@ -343,3 +343,39 @@ def test_emu_synth_matches_oracle(oracle_mod, cfg):
     assert np.array_equal(gen["bits"][0][:cfg[1]], synth_ref.payload_bits(5, 0, cfg[1]))
     assert np.array_equal(gen["bits"][0], O.attach_crc(gen["bits"][0][:cfg[1]], cfg[2]))
     assert O.syndrome_ok(c, gen["cw"][0])
+
+
+@pytest.mark.parametrize("pair", [(2, 26), (1, 15)])
+@pytest.mark.parametrize("row", [[1, 8, 22936, 0, 1280, 32767, -32767],
+                                 [1, 6, 22936, 40000, 21279, 32767, -32767],
+                                 [1, 6, 40000, 16, 1280, 300, 2000],
+                                 [2, 6, 0, 24, 1280, 32767, -32767],
+                                 [3, 6, 0, 40000, 1280, 32767, -32767]])
+def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, case_tables):
+    """The lifting-size specialized kernel (compile-time shifts, explicit
+    per-edge messages) on uniform batches: bit-exact with the generic kernel
+    and with the oracle, for algo 1 fast and exact row forms and algos 2/3
+    with the adversarial tables."""
+    bgn, zc = pair
+    rng = np.random.default_rng(1000 * bgn + zc + row[0])
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=4, zcs=[zc])
+    cfgs, llrs = zip(*[(c, l) for c, l in zip(cfgs, llrs) if c[0] == bgn])
+    z = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo2.npz"))
+    tabs = (z[f"alpha_bg{bgn}"][2], z[f"beta_bg{bgn}"][2])
+    dalpha = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo3.npz"))["degree_alpha"][2]
+    case_tables(dict(cfg=(bgn,), tables=tabs, degree_alpha=dalpha))
+    outs = {}
+    try:
+        for fixed in (True, False):
+            emu.use_fixed(fixed)
+            outs[fixed] = emu.decode_batch(list(cfgs), row, list(llrs))
+    finally:
+        emu.use_fixed(True)
+    for cfg, llr, a, b in zip(cfgs, llrs, outs[True], outs[False]):
+        assert (a["status"], a["iters"]) == (b["status"], b["iters"]), cfg
+        assert np.array_equal(a["hard"], b["hard"]) and np.array_equal(a["lq"], b["lq"]), cfg
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr),
+                                            tables=tabs, degree_alpha=dalpha)
+        assert (a["status"], a["iters"]) == (s, k), cfg
+        assert np.array_equal(a["lq"], q), cfg
