@@ -4,8 +4,9 @@
 // byte-table CRC over its slice, multiplies its remainder by
 // x^(8 * bytes after the slice) mod g (GF(2) linearity), and the remainders
 // are XOR-reduced.  Include after the HIP runtime header (or the CPU
-// emulation header) and ldpc_device.h.
+// emulation header).
 #pragma once
+#include "ldpc_device.h"
 
 namespace nrldpc {
 
@@ -126,4 +127,22 @@ __device__ inline uint32_t block_crc(const T *lq, const CrcCtx &ctx,
     return total;
 }
 
+}  // namespace nrldpc
+
+namespace nrldpc {
+// crc_setup with the tables precomputed on the host (DecOut::crc): the byte
+// table is copied to tab (LDS) and the lane's multiplier is one load.
+__device__ inline CrcCtx crc_setup_tab(int crctype, int nbytes, uint32_t *tab,
+                                       const uint32_t *crc_tables) {
+    CrcCtx ctx;
+    ctx.spec = crc_spec(crctype);
+    const uint32_t *g = crc_tables + (size_t)crctype * kCrcTabWords;
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int i = t; i < 256; i += nt) tab[i] = g[i];
+    const int per = (nbytes + nt - 1) / nt;
+    ctx.b_lo = t * per < nbytes ? t * per : nbytes;
+    ctx.b_hi = (t + 1) * per < nbytes ? (t + 1) * per : nbytes;
+    ctx.mult = (ctx.b_hi > ctx.b_lo) ? g[256 + nbytes - ctx.b_hi] : 0u;
+    return ctx;
+}
 }  // namespace nrldpc

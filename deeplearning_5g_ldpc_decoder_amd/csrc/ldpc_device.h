@@ -19,6 +19,10 @@ namespace nrldpc {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kMaxZc = 384;
+// CRC tables on the device, per CRC type (CRC16, 24A, 24B, 24C): the 256-entry
+// byte table, then x^(8m) mod g for m = 0 .. kMaxCrcBytes.
+constexpr int kMaxCrcBytes = 8448 / 8;
+constexpr int kCrcTabWords = 256 + kMaxCrcBytes + 1;
 constexpr int kMaxThreads = 384;         // one lane per lifted check
 constexpr int kMaxSoft = 68 * kMaxZc;    // N + 2Zc for BG1 Zc=384
 
@@ -76,6 +80,7 @@ struct DecOut {
     void *lq;
     float *work;
     const uint32_t *etab;
+    const uint32_t *crc;   // [4][kCrcTabWords] CRC tables
 };
 
 // Device input synthesis (ldpc_synth.hip): n codeblocks of one config.

@@ -391,6 +391,26 @@ const CrcTables &crc_tables() {
 }
 }  // namespace
 
+namespace nrldpc {
+// Per CRC type: the byte table, then x^(8m) mod g for m = 0..kMaxCrcBytes
+// (t * x^8 mod g = ((t << 8) mod x^w) ^ tab[t >> (w - 8)]).  A lane whose
+// byte slice has m bytes after it multiplies its partial remainder by entry m.
+void crc_device_tables(uint32_t *out) {
+    const CrcTables &T = crc_tables();
+    for (int t = 0; t < 4; ++t) {
+        uint32_t *o = out + (size_t)t * kCrcTabWords;
+        const int w = T.width[t];
+        const uint32_t mask = (1u << w) - 1u;
+        for (int i = 0; i < 256; ++i) o[i] = T.tab[t][i];
+        uint32_t r = 1u;
+        for (int m = 0; m <= kMaxCrcBytes; ++m) {
+            o[256 + m] = r;
+            r = ((r << 8) & mask) ^ T.tab[t][(r >> (w - 8)) & 0xFFu];
+        }
+    }
+}
+}  // namespace nrldpc
+
 extern "C" uint32_t calculate_crc(const uint8_t *data, int len, int crctype) {
     if (crctype < 0 || crctype > 3) return 0;
     const CrcTables &T = crc_tables();

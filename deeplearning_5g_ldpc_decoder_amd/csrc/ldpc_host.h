@@ -27,6 +27,9 @@ const uint16_t *all_lifting_sizes(int *count);
 // Device edge table of (bgn, zc): per edge (column * zc) | (V_ij mod zc) << 16.
 void edge_table(int bgn, int zc, int ils, uint32_t *out);
 
+// Device CRC tables (DecOut::crc): 4 * kCrcTabWords words.
+void crc_device_tables(uint32_t *out);
+
 void encode_codeword(const Params &p, const uint8_t *bits_b, uint8_t *cw);
 void rate_match_bits(const Params &p, const uint8_t *cw, uint8_t *out_bits);
 

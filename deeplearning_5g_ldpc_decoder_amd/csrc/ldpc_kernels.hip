@@ -62,27 +62,168 @@ namespace nrldpc {
 // (c_lib/ldpc_rate_recovery.c:24-65).  Gather form: buffer position p takes
 // the sum of every LLR k with compact index q(p) = (ck0 + k) mod (N - F),
 // k < E; the sum wraps mod 2^16 exactly like the reference's int16 +=.
+//
+// recover_fill writes positions first, first + stride, ... < n of one
+// codeblock into lq (as T) and, when the codeblock has a soft buffer, into
+// it.  A lane works on kRecoverBatch positions at a time and issues their
+// first LLR loads back to back, so the gather runs at memory throughput
+// instead of one dependent global load per position; further contributions
+// (E > N - F, repetition) are added in a second, usually empty, loop.
 // ---------------------------------------------------------------------------
-__device__ inline int16_t recover_one(int p, const CbDesc &d, const int16_t *llr,
-                                      const int16_t *soft_in, int filler) {
-    const int Zc = d.Zc, B = d.B, K = d.K, F = K - B;
-    int acc = (d.harq_on && soft_in) ? (int)soft_in[p] : 0;
-    if (p < 2 * Zc) return (int16_t)acc;
-    if (p >= B && p < K) return (int16_t)filler;
-    const int pos = p - 2 * Zc;
-    const int lo = B - 2 * Zc, hi = K - 2 * Zc;
-    const int nc = d.N - F;
-    const int ck0 = d.k0 < lo ? d.k0 : (d.k0 < hi ? lo : d.k0 - F);
-    const int q = pos < lo ? pos : pos - F;
-    int k = q - ck0;
-    if (k < 0) k += nc;
-    const int S = d.E / d.Qm;
-    for (; k < d.E; k += nc) {
-        int sym = k, bit = 0;
-        while (sym >= S) { sym -= S; ++bit; }
-        acc += llr[sym * d.Qm + bit];
+constexpr int kRecoverBatch = 8;
+
+// Belief as the decoder stores it: int16 as is, float / 128 (algos 0 and 4,
+// c_lib/floatpoint_ldpc_decoder.c:325-327).
+template <typename T> __device__ inline T from_soft(int16_t v);
+template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
+template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
+
+struct RecoverCtx {
+    int Zc, B, K, F, lo, nc, ck0, S, E, Qm;
+};
+
+__device__ __forceinline__ RecoverCtx recover_ctx(const CbDesc &d) {
+    RecoverCtx c;
+    c.Zc = d.Zc;
+    c.B = d.B;
+    c.K = d.K;
+    c.F = d.K - d.B;
+    c.lo = d.B - 2 * d.Zc;
+    const int hi = d.K - 2 * d.Zc;
+    c.nc = d.N - c.F;
+    c.ck0 = d.k0 < c.lo ? d.k0 : (d.k0 < hi ? c.lo : d.k0 - c.F);
+    c.E = d.E;
+    c.Qm = d.Qm;
+    c.S = d.E / d.Qm;
+    return c;
+}
+
+// LLR index of rate-matched bit k (bit-interleaved: symbol k mod S, bit k / S).
+__device__ __forceinline__ int llr_index(int k, const RecoverCtx &c) {
+    int sym = k, bit = 0;
+    while (sym >= c.S) {
+        sym -= c.S;
+        ++bit;
     }
-    return (int16_t)acc;
+    return sym * c.Qm + bit;
+}
+
+template <typename T>
+__device__ void recover_fill(T *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+                             int16_t *soft, int filler, int first, int stride, int n) {
+    const RecoverCtx c = recover_ctx(d);
+    const bool harq = d.harq_on && soft;
+    for (int i0 = first; i0 < n; i0 += stride * kRecoverBatch) {
+        int acc[kRecoverBatch], k[kRecoverBatch];
+#pragma unroll
+        for (int u = 0; u < kRecoverBatch; ++u) {
+            const int p = i0 + u * stride;
+            acc[u] = 0;
+            k[u] = c.E;   // no contribution
+            if (p < n) {
+                if (harq) acc[u] = soft[p];
+                if (p >= c.B && p < c.K) {
+                    acc[u] = filler;
+                } else if (p >= 2 * c.Zc) {
+                    const int pos = p - 2 * c.Zc;
+                    int kk = (pos < c.lo ? pos : pos - c.F) - c.ck0;
+                    if (kk < 0) kk += c.nc;
+                    k[u] = kk;
+                }
+            }
+        }
+        int add[kRecoverBatch];
+#pragma unroll
+        for (int u = 0; u < kRecoverBatch; ++u)
+            add[u] = k[u] < c.E ? (int)llr[llr_index(k[u], c)] : 0;
+#pragma unroll
+        for (int u = 0; u < kRecoverBatch; ++u) {
+            const int p = i0 + u * stride;
+            int v = acc[u] + add[u];
+            for (int kk = k[u] + c.nc; kk < c.E; kk += c.nc) v += llr[llr_index(kk, c)];
+            if (p < n) {
+                const int16_t w = (int16_t)v;   // the reference's int16 += wraps mod 2^16
+                if (soft) soft[p] = w;
+                lq[p] = from_soft<T>(w);
+            }
+        }
+    }
+}
+
+// Scatter form of the same recovery, for codeblocks without repetition
+// (E <= N - F: every buffer position receives at most one LLR).  The LLRs are
+// read once, in order and coalesced, kScatterBatch loads in flight per lane;
+// LLR j is rate-matched bit k = (j mod Qm) * S + j / Qm, which lands on
+// compact index q = (ck0 + k) mod (N - F).
+constexpr int kScatterBatch = 16;
+
+// Initial beliefs: filler positions take the filler value, the others the
+// soft buffer (HARQ combining) or 0.
+__device__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, int filler,
+                             int first, int stride, int n) {
+    const bool harq = d.harq_on && soft;
+    for (int i0 = first; i0 < n; i0 += stride * kRecoverBatch) {
+        int v[kRecoverBatch];
+#pragma unroll
+        for (int u = 0; u < kRecoverBatch; ++u) {
+            const int p = i0 + u * stride;
+            v[u] = (harq && p < n) ? (int)soft[p] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRecoverBatch; ++u) {
+            const int p = i0 + u * stride;
+            if (p < n) lq[p] = (int16_t)((p >= d.B && p < d.K) ? filler : v[u]);
+        }
+    }
+}
+
+__device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+                                int first, int stride) {
+    const RecoverCtx c = recover_ctx(d);
+    // floor(j / Qm) = umulhi(j, ceil(2^32 / Qm)), exact for j < 2^16 and Qm <= 8
+    const uint32_t magic = c.Qm > 1 ? 0xFFFFFFFFu / (uint32_t)c.Qm + 1u : 0u;
+    const int p0 = 2 * c.Zc;
+    for (int j0 = first; j0 < c.E; j0 += stride * kScatterBatch) {
+        int v[kScatterBatch];
+#pragma unroll
+        for (int u = 0; u < kScatterBatch; ++u) {
+            const int j = j0 + u * stride;
+            v[u] = j < c.E ? (int)llr[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kScatterBatch; ++u) {
+            const int j = j0 + u * stride;
+            if (j < c.E) {
+                const int sym = c.Qm > 1 ? (int)__umulhi((uint32_t)j, magic) : j;
+                const int bit = j - sym * c.Qm;
+                int q = bit * c.S + sym + c.ck0;
+                if (q >= c.nc) q -= c.nc;
+                const int p = (q < c.lo ? q : q + c.F) + p0;
+                lq[p] = (int16_t)(lq[p] + v[u]);
+            }
+        }
+    }
+}
+
+// Rate recovery of one codeblock into int16 beliefs and its soft buffer by
+// lanes first, first + stride, ... .  Every lane of the workgroup calls it
+// (it holds two workgroup barriers); lanes with active = false only join
+// them.  The caller's next barrier orders the soft-buffer write-back before
+// decoding changes the beliefs.
+__device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, int16_t *soft,
+                            int filler, int first, int stride, int n, bool active) {
+    const bool scatter = active && d.E <= d.N - (d.K - d.B);
+    if (active) {
+        if (scatter)
+            recover_init(lq, d, soft, filler, first, stride, n);
+        else
+            recover_fill<int16_t>(lq, d, llr, soft, filler, first, stride, n);
+    }
+    __syncthreads();
+    if (scatter) recover_scatter(lq, d, llr, first, stride);
+    __syncthreads();
+    if (scatter && soft)
+        for (int p = first; p < n; p += stride) soft[p] = lq[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -698,9 +839,6 @@ __device__ int run_bp(float *lq, float *lr_mem, const uint32_t *etab, const CbDe
 // ---------------------------------------------------------------------------
 enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4, kBpF32 = 0 };
 
-template <typename T> __device__ inline T from_soft(int16_t v);
-template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
-template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
 
 // kGeneral selects the exact-for-every-row int16 path (see row_i16) and
 // kTable the per-row scale words of algos 2/3; the other algorithms ignore
@@ -720,12 +858,11 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     // fused rate recovery + HARQ combine into LDS; soft buffer write-back
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    for (int i = t; i < n; i += nt) {
-        const int16_t v = recover_one(i, d, llr, soft, p.filler);
-        if (soft) soft[i] = v;
-        lq[i] = from_soft<T>(v);
-    }
-    const CrcCtx crc = crc_setup(d.crctype, d.B / 8, crc_tab);
+    if constexpr (A == Algo::kMinSumI16)
+        recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
+    else
+        recover_fill<T>(lq, d, llr, soft, p.filler, t, nt, n);
+    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
     const uint32_t *etab = o.etab + d.etab_off;
@@ -786,12 +923,8 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    for (int i = t; i < n; i += nt) {
-        const int16_t v = recover_one(i, d, llr, soft, p.filler);
-        if (soft) soft[i] = v;
-        lq[i] = v;
-    }
-    const CrcCtx crc = crc_setup(d.crctype, d.B / 8, crc_tab);
+    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
+    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
     int status = 1, it = 0;
@@ -870,30 +1003,21 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     int16_t *lq = reinterpret_cast<int16_t *>(smem) + (member ? cbl : 0) * td.lds_stride;
 
     // CRC tables of all four polynomials (codeblocks may differ in crctype)
-    for (int i = lane; i < 4 * 256; i += nt) {
-        const Crc cs = crc_spec(i >> 8);
-        uint32_t r = (uint32_t)(i & 255) << (cs.w - 8);
-        for (int k = 0; k < 8; ++k) r = crc_xtimes(r, cs);
-        crc_tab[i >> 8][i & 255] = r;
-    }
+    for (int i = lane; i < 4 * 256; i += nt) crc_tab[i >> 8][i & 255] = o.crc[(i >> 8) * kCrcTabWords + (i & 255)];
     for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
         bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
     CrcCtx crc;
     crc.spec = crc_spec(d.crctype);
     crc.b_lo = crc.b_hi = 0;
     crc.mult = 0u;
+    recover_i16(lq, d, o.llr + d.llr_off, (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr,
+                p.filler, zx, Zc, n, member);
     if (member) {
-        const int16_t *llr = o.llr + d.llr_off;
-        int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-        for (int i = zx; i < n; i += Zc) {
-            const int16_t v = recover_one(i, d, llr, soft, p.filler);
-            if (soft) soft[i] = v;
-            lq[i] = v;
-        }
         const int nbytes = d.B / 8, per = (nbytes + Zc - 1) / Zc;
         crc.b_lo = min(zx * per, nbytes);
         crc.b_hi = min((zx + 1) * per, nbytes);
-        if (crc.b_hi > crc.b_lo) crc.mult = crc_x8pow(nbytes - crc.b_hi, crc.spec);
+        if (crc.b_hi > crc.b_lo)
+            crc.mult = o.crc[d.crctype * kCrcTabWords + 256 + nbytes - crc.b_hi];
     }
     __syncthreads();
 

@@ -64,8 +64,9 @@ struct Runtime {
     int init_status = -1;
     int device = 0;
     hipStream_t stream = nullptr;
-    uint32_t *d_etab = nullptr;
+    uint32_t *d_etab = nullptr;        // edge tables, then the CRC tables
     uint32_t etab_off[2][kMaxZc + 1] = {};
+    uint32_t crc_off = 0;
     unsigned char *h_in = nullptr, *h_out = nullptr;   // pinned
     unsigned char *d_in = nullptr, *d_out = nullptr;
     float *d_work = nullptr;                           // BP messages, one codeblock
@@ -86,6 +87,9 @@ struct Runtime {
                 edge_table(bg, zc, lifting_set(zc), tab.data() + etab_off[bg - 1][zc]);
             }
         }
+        crc_off = (uint32_t)tab.size();
+        tab.resize(tab.size() + 4 * (size_t)kCrcTabWords);
+        crc_device_tables(tab.data() + crc_off);
         HIP_OK(hipMalloc(&d_etab, tab.size() * sizeof(uint32_t)));
         HIP_OK(hipMemcpy(d_etab, tab.data(), tab.size() * sizeof(uint32_t),
                          hipMemcpyHostToDevice));
@@ -212,6 +216,7 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     o.lq = r.d_out + OutLayout::lq;
     o.work = r.d_work;
     o.etab = r.d_etab;
+    o.crc = r.d_etab + r.crc_off;
     HIP_OK(nrldpc_launch_decode(algo, &o, &dp, 1, threads_for(p.Zc), n,
                                 uniform_pair(p.bgn, p.Zc), r.stream));
     // results: status/iterations/hard contiguous, then the soft buffer
@@ -460,6 +465,7 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     o.lq = lq;
     o.work = plan->d_work;
     o.etab = r.d_etab;
+    o.crc = r.d_etab + r.crc_off;
     // NULL is HIP's default (null) stream, as everywhere in HIP -- it is also
     // the handle PyTorch reports for its default stream.
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -601,6 +607,7 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
     o.lq = lq;
     o.work = nullptr;
     o.etab = r.d_etab;
+    o.crc = r.d_etab + r.crc_off;
     if (b->ntiles > 0)
         HIP_OK(nrldpc_launch_tiles(b->algo, &o,
                                    reinterpret_cast<const TileDesc *>(b->d_stage + toff),

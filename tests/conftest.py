@@ -63,3 +63,26 @@ def oracle_mod():
     from oracle import oracle as O
     O.lib()
     return O
+
+
+def rate_recovery_cases():
+    """Reference HARQ_memory fixtures after rate recovery
+    (tests/golden/gen_golden.py: repetition with int16 wrap, an rv chain with
+    HARQ combining): (cfg7, filler, llr, soft_before or None, soft_after)."""
+    z = np.load(os.path.join(GOLDEN, "rate_recovery.npz"))
+    from oracle import oracle as O
+    out = []
+    lo = so = 0
+    prev = None
+    for m in z["meta"]:
+        bgn, A, crctype, E, rv, Qm, harq_on, filler = (int(v) for v in m)
+        st, p = O.params(bgn, A, crctype, E, rv, Qm, harq_on)
+        n = p["N"] + 2 * p["Zc"]
+        llr = z["llr"][lo:lo + E]
+        lo += E
+        want = z["soft"][so:so + n]
+        so += n
+        out.append(((bgn, A, crctype, E, rv, Qm, harq_on), filler, llr,
+                    prev if harq_on else None, want))
+        prev = want
+    return out

@@ -11,8 +11,10 @@ using namespace nrldpc;
 namespace {
 struct EdgeTables {
     std::vector<uint32_t> tab;
+    std::vector<uint32_t> crc = std::vector<uint32_t>(4 * (size_t)kCrcTabWords);
     uint32_t off[2][kMaxZc + 1] = {};
     EdgeTables() {
+        crc_device_tables(crc.data());
         int nz = 0;
         const uint16_t *zs = all_lifting_sizes(&nz);
         for (int bg = 1; bg <= 2; ++bg) {
@@ -96,6 +98,7 @@ DecOut outputs(const Batch &b, const int16_t *llr, int16_t *soft_pool, uint8_t *
     o.lq = lq;
     o.work = work;
     o.etab = tables().tab.data();
+    o.crc = tables().crc.data();
     return o;
 }
 

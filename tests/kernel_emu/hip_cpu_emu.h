@@ -53,6 +53,7 @@ inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 inline unsigned char *emu_dynamic_lds() { return g_emu_block.lds.data(); }
 inline void __syncthreads() { g_emu_block.bar->arrive_and_wait(); }
 inline int __popc(uint32_t v) { return __builtin_popcount(v); }
+inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 
 inline int __syncthreads_or(int pred) {
     static std::atomic<int> acc[2];

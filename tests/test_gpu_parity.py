@@ -239,3 +239,21 @@ def test_harq_soft_pool(L, oracle_mod):
         assert out["status"].item() == s and out["iters"].item() == k
         assert np.array_equal(out["hard"].cpu().numpy(), h)
         plan.close()
+
+
+def test_rate_recovery_abi_matches_reference(L):
+    """c_ldpc_5g_decode with no iterations leaves the reference's
+    HARQ_memory: heavy repetition with int16 wrap, every rv, Qm 1-8, and an
+    rv chain combining into the soft buffer."""
+    from conftest import rate_recovery_cases
+    hv = harq_view(L)
+    for cfg, filler, llr, before, want in rate_recovery_cases():
+        if before is not None:
+            hv[:] = 0
+            hv[:len(before)] = before
+        hard = np.zeros(1100, np.uint8)
+        llr = np.ascontiguousarray(llr, np.int16)
+        st = L.c_ldpc_5g_decode(ptr(llr), ptr(hard), ctypes.byref(LdpcInConfig(*cfg)),
+                                ctypes.byref(DecoderConfig(1, 0, 22936, 0, filler, 32767, -32767)))
+        assert st == 1
+        assert np.array_equal(hv[:len(want)], want), cfg

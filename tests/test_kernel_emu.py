@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import ROOT, case_id, load_decode_cases
+from conftest import ROOT, case_id, load_decode_cases, rate_recovery_cases
 
 sys.path.insert(0, os.path.join(ROOT, "tests", "kernel_emu"))
 import emu  # noqa: E402
@@ -379,3 +379,16 @@ def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, case
                                             tables=tabs, degree_alpha=dalpha)
         assert (a["status"], a["iters"]) == (s, k), cfg
         assert np.array_equal(a["lq"], q), cfg
+
+
+def test_emu_rate_recovery_matches_reference():
+    """The kernels' batched gather (recover_fill) on the reference's
+    HARQ_memory fixtures: heavy repetition with int16 wrap, every rv, Qm 1-8,
+    and an rv chain combining into the soft buffer."""
+    for cfg, filler, llr, before, want in rate_recovery_cases():
+        pool = np.zeros(68 * 384, np.int16)
+        if before is not None:
+            pool[:len(before)] = before
+        row = [1, 0, 22936, 0, filler, 32767, -32767]
+        emu.decode_batch([cfg], row, [llr], soft_pool=pool, soft_slots=[0])
+        assert np.array_equal(pool[:len(want)], want), cfg
