@@ -392,7 +392,7 @@ __device__ __forceinline__ void rows_i16(int16_t *lq, StateI16<BG> &st, const ui
             beta = (int)(int16_t)(w >> 16);
         }
         if (run) row_i16<BG, R, kGeneral>(lq, st, etab, zx, Zc, p, alpha, beta);
-        __syncthreads();
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         rows_i16<BG, R + 1, kGeneral, kTable>(lq, st, etab, zx, Zc, run, p, ab);
     }
 }
@@ -610,7 +610,7 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
         if (run)
             fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
                                            std::make_integer_sequence<int, deg>{});
-        __syncthreads();
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         fixed_rows<BG, ZC, R + 1, kGeneral, kTable>(lqb, msg, zx2, run, p, ab);
     }
 }
