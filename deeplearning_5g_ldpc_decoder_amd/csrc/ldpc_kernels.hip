@@ -596,6 +596,166 @@ __device__ __forceinline__ void fixed_row(lds_char *__restrict__ lqb, uint32_t (
     (fixed_edge_out<BG, ZC, e0 + J, J, deg, kGeneral>(lqb, msg, rs, idx, n1, n2, bits, p), ...);
 }
 
+// ---------------------------------------------------------------------------
+// Packed form of the fixed row (fast rows: every scaled minimum in
+// [0, 32767] and min_llr <= max_llr).  Two edges of a row share one VGPR as
+// int16 halves, so the extrinsic subtract, |.| with the reference's 32767 cap,
+// the sign and the saturated LQ update run as one packed op for both; each
+// row's messages are stored pairwise (edges e0 + 2i, e0 + 2i + 1; an odd last
+// edge alone in the low half, high half kept 0).
+// ---------------------------------------------------------------------------
+#ifdef NRLDPC_CPU_EMU
+__device__ __forceinline__ int pk_lo(uint32_t v) { return (int)(int16_t)(v & 0xFFFFu); }
+__device__ __forceinline__ int pk_hi(uint32_t v) { return (int)(int16_t)(v >> 16); }
+__device__ __forceinline__ uint32_t pk_make(int lo, int hi) {
+    return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+}
+__device__ __forceinline__ int sat16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
+// per half: (a - b) mod 2^16
+__device__ __forceinline__ uint32_t pk_subw(uint32_t a, uint32_t b) {
+    return pk_make(pk_lo(a) - pk_lo(b), pk_hi(a) - pk_hi(b));
+}
+// per half: max(r, sat(-r)) = |r| capped at 32767
+__device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
+    return pk_make(max(pk_lo(r), sat16(-pk_lo(r))), max(pk_hi(r), sat16(-pk_hi(r))));
+}
+// per half: r >> 15 (arithmetic): 0 or -1
+__device__ __forceinline__ uint32_t pk_sign15(uint32_t r) {
+    return pk_make(pk_lo(r) >> 15, pk_hi(r) >> 15);
+}
+// per half: min(max(sat(a + b), lo), hi)
+__device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t lo, uint32_t hi) {
+    const int x = min(max(sat16(pk_lo(a) + pk_lo(b)), pk_lo(lo)), pk_lo(hi));
+    const int y = min(max(sat16(pk_hi(a) + pk_hi(b)), pk_hi(lo)), pk_hi(hi));
+    return pk_make(x, y);
+}
+// (half of a) * 32 + c
+__device__ __forceinline__ uint32_t key_lo(uint32_t a, uint32_t c) { return (a & 0xFFFFu) * 32u + c; }
+__device__ __forceinline__ uint32_t key_hi(uint32_t a, uint32_t c) { return (a >> 16) * 32u + c; }
+#else
+typedef short nr_s2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ nr_s2 as_s2(uint32_t v) { return __builtin_bit_cast(nr_s2, v); }
+__device__ __forceinline__ uint32_t as_u32(nr_s2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t pk_make(int lo, int hi) {
+    return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+}
+__device__ __forceinline__ uint32_t pk_subw(uint32_t a, uint32_t b) {
+    return as_u32(as_s2(a) - as_s2(b));
+}
+__device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
+    const nr_s2 v = as_s2(r);
+    return as_u32(__builtin_elementwise_max(v, __builtin_elementwise_sub_sat((nr_s2)0, v)));
+}
+__device__ __forceinline__ uint32_t pk_sign15(uint32_t r) { return as_u32(as_s2(r) >> 15); }
+__device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t lo, uint32_t hi) {
+    const nr_s2 v = __builtin_elementwise_add_sat(as_s2(a), as_s2(b));
+    return as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, as_s2(lo)), as_s2(hi)));
+}
+__device__ __forceinline__ uint32_t key_lo(uint32_t a, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u16 %0, %1, 32, %2" : "=v"(r) : "v"(a), "v"(c));
+    return r;
+}
+__device__ __forceinline__ uint32_t key_hi(uint32_t a, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u16 %0, %1, 32, %2 op_sel:[1,0,0,0]" : "=v"(r) : "v"(a), "v"(c));
+    return r;
+}
+#endif
+
+// Packed message words of base graph BG: ceil(deg / 2) per row.
+template <int BG>
+__host__ __device__ constexpr int packed_word0(int r) {
+    int w = 0;
+    for (int x = 0; x < r; ++x) w += (Graph<BG>::start(x + 1) - Graph<BG>::start(x) + 1) / 2;
+    return w;
+}
+
+template <int PAIRS>
+struct PackedRowState {
+    uint32_t oa[PAIRS], ob[PAIRS];   // byte offsets of the pair's two variables
+    uint32_t rel[PAIRS];             // extrinsic inputs, int16 halves
+    uint32_t k1, k2, px;
+};
+
+template <int BG, int ZC, int E, int I, int DEG, int PAIRS>
+__device__ __forceinline__ void packed_pair_in(const lds_char *__restrict__ lqb,
+                                               const uint32_t *msg, int w, uint32_t zx2,
+                                               PackedRowState<PAIRS> &rs) {
+    using F = FixedCode<BG, ZC>;
+    constexpr bool has_b = 2 * I + 1 < DEG;
+    constexpr uint32_t sha = F::shift_bytes(E), cba = F::col_bytes(E);
+    const uint32_t ca = zx2 + sha;
+    const uint32_t oa = min(ca, ca - 2u * (uint32_t)ZC);
+    rs.oa[I] = oa;
+    uint32_t q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
+    if constexpr (has_b) {
+        constexpr uint32_t shb = F::shift_bytes(E + 1), cbb = F::col_bytes(E + 1);
+        const uint32_t cb = zx2 + shb;
+        const uint32_t ob = min(cb, cb - 2u * (uint32_t)ZC);
+        rs.ob[I] = ob;
+        q |= (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cbb + ob) << 16;
+    }
+    const uint32_t rel = pk_subw(q, msg[w]);
+    rs.rel[I] = rel;
+    rs.px ^= rel;
+    const uint32_t a = pk_abs_sat(rel);
+    const uint32_t ka = key_lo(a, (uint32_t)(2 * I));
+    rs.k2 = med3_u32(rs.k1, ka, rs.k2);
+    rs.k1 = min(rs.k1, ka);
+    if constexpr (has_b) {
+        const uint32_t kb = key_hi(a, (uint32_t)(2 * I + 1));
+        rs.k2 = med3_u32(rs.k1, kb, rs.k2);
+        rs.k1 = min(rs.k1, kb);
+    }
+}
+
+template <int BG, int ZC, int E, int I, int DEG, int PAIRS, int N>
+__device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                                int w, const PackedRowState<PAIRS> &rs,
+                                                int ipair, uint32_t n1p, uint32_t n2p,
+                                                uint32_t pm, uint32_t lo, uint32_t hi) {
+    using F = FixedCode<BG, ZC>;
+    constexpr bool has_b = 2 * I + 1 < DEG;
+    uint32_t nv = (ipair == I) ? n2p : n1p;
+    if constexpr (!has_b) nv &= 0xFFFFu;   // keeps the unused high half at 0
+    const uint32_t rel = rs.rel[I];
+    const uint32_t sm = pk_sign15(rel ^ pm);
+    const uint32_t m = pk_subw(nv ^ sm, sm);
+    msg[w] = m;
+    const uint32_t v = pk_update(rel, m, lo, hi);
+    *(lds_i16 *)(lqb + F::col_bytes(E) + rs.oa[I]) = (int16_t)(v & 0xFFFFu);
+    if constexpr (has_b)
+        *(lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]) = (int16_t)(v >> 16);
+}
+
+template <int BG, int ZC, int R, int N, int... I>
+__device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                           uint32_t zx2, const DecParams &p, int alpha, int beta,
+                                           std::integer_sequence<int, I...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    constexpr int pairs = (int)sizeof...(I);
+    constexpr int w0 = packed_word0<BG>(R);
+    NRLDPC_OPAQUE_LANE(zx2);
+    PackedRowState<pairs> rs;
+    rs.k1 = kKeyMax;
+    rs.k2 = kKeyMax;
+    rs.px = 0;
+    (packed_pair_in<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, zx2, rs), ...);
+    const int idx = (int)(rs.k1 & 31u);
+    const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
+    const int n1 = scale_i16(m1, alpha, beta), n2 = scale_i16(m2, alpha, beta);
+    // odd number of negative inputs: every message flips sign
+    const uint32_t neg = ((rs.px >> 15) ^ (rs.px >> 31)) & 1u;
+    const uint32_t pm = neg ? 0x80008000u : 0u;
+    const uint32_t n1p = pk_make(n1, n1);
+    const uint32_t n2p = (idx & 1) ? pk_make(n1, n2) : pk_make(n2, n1);
+    const uint32_t lo = pk_make(p.min_llr, p.min_llr), hi = pk_make(p.max_llr, p.max_llr);
+    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, rs, idx >> 1, n1p, n2p,
+                                                        pm, lo, hi), ...);
+}
+
 template <int BG, int ZC, int R, bool kGeneral, bool kTable, int N>
 __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], uint32_t zx2, bool run,
                                            const DecParams &p, const uint32_t *ab) {
@@ -607,9 +767,14 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
             beta = (int)(int16_t)(w >> 16);
         }
         constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
-        if (run)
-            fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
-                                           std::make_integer_sequence<int, deg>{});
+        if (run) {
+            if constexpr (kGeneral)
+                fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
+                                               std::make_integer_sequence<int, deg>{});
+            else
+                packed_row<BG, ZC, R>(lqb, msg, zx2, p, alpha, beta,
+                                      std::make_integer_sequence<int, (deg + 1) / 2>{});
+        }
         if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         fixed_rows<BG, ZC, R + 1, kGeneral, kTable>(lqb, msg, zx2, run, p, ab);
     }
@@ -618,9 +783,13 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
 template <int BG, int ZC, bool kGeneral, bool kTable>
 __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
                              const uint32_t *tab, uint32_t *slots, int *status) {
-    uint32_t msg[FixedCode<BG, ZC>::kMsgWords];
+    // exact rows: one message per edge in global edge order; fast rows:
+    // row-paired packed messages
+    constexpr int kWords =
+        kGeneral ? FixedCode<BG, ZC>::kMsgWords : packed_word0<BG>(Graph<BG>::kRows);
+    uint32_t msg[kWords];
 #pragma unroll
-    for (int i = 0; i < FixedCode<BG, ZC>::kMsgWords; ++i) msg[i] = 0u;
+    for (int i = 0; i < kWords; ++i) msg[i] = 0u;
     const int zx = threadIdx.x;
     const uint32_t zx2 = 2u * (uint32_t)zx;
     lds_char *lqb = fixed_lds_base(lq);
