@@ -746,9 +746,9 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
     const int idx = (int)(rs.k1 & 31u);
     const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
     const int n1 = scale_i16(m1, alpha, beta), n2 = scale_i16(m2, alpha, beta);
-    // odd number of negative inputs: every message flips sign
-    const uint32_t neg = ((rs.px >> 15) ^ (rs.px >> 31)) & 1u;
-    const uint32_t pm = neg ? 0x80008000u : 0u;
+    // odd number of negative inputs (bit 15 of px ^ px >> 16): every message
+    // flips sign; pm holds that bit in the sign bit of both halves
+    const uint32_t pm = ((rs.px ^ (rs.px >> 16)) & 0x8000u) * 0x10001u;
     const uint32_t n1p = pk_make(n1, n1);
     const uint32_t n2p = (idx & 1) ? pk_make(n1, n2) : pk_make(n2, n1);
     const uint32_t lo = pk_make(p.min_llr, p.min_llr), hi = pk_make(p.max_llr, p.max_llr);
