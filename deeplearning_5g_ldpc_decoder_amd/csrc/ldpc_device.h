@@ -49,6 +49,9 @@ struct DecParams {
     // int16 algos: 1 when some row's scaled minimum can wrap to int16 -32768
     // (or alpha < 0, or min_llr > max_llr), which needs the exact row form
     int32_t general;
+    // int16 fast rows: 1 when every (alpha, beta) pair has beta = 0 and
+    // max_llr = 32767, so a scaled minimum is (alpha * m + 2^14) >> 15
+    int32_t simple;
     // algos 2/3: per (iteration, row) scale words of base graph 1 / 2,
     // alpha (int16) | beta (int16) << 16, iteration t at t * rows; iterations
     // at or past scale_iters use the last one

@@ -163,6 +163,7 @@ DecParams decoder_params(const ldpc_decoder_cfg_t &c) {
     p.fmax = (float)c.max_llr / 128.0f;
     p.fmin = (float)c.min_llr / 128.0f;
     p.general = i16_fast_ok(p.alpha, p.beta, p.max_llr, p.min_llr) ? 0 : 1;
+    p.simple = (!p.general && p.beta == 0 && p.max_llr == 32767) ? 1 : 0;
     return p;
 }
 
@@ -271,6 +272,7 @@ void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s) {
     s.algo = c.algo;
     s.words.clear();
     s.fast = true;
+    s.simple = c.max_llr == 32767;
     // algo 2 reads iteration t's words for t < max_iterations; algo 3 one set
     s.iters = c.algo == 2 ? std::max((int)c.max_iterations, 1) : 1;
     for (int bgn = 1; bgn <= 2; ++bgn) {
@@ -296,6 +298,7 @@ void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s) {
                 }
                 s.words.push_back(scale_word(a, b));
                 if (!i16_fast_ok(a, b, c.max_llr, c.min_llr)) s.fast = false;
+                if ((int16_t)b != 0) s.simple = false;
             }
     }
 }

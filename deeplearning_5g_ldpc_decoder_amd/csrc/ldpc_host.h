@@ -66,6 +66,7 @@ struct ScaleSet {
     int iters = 0;
     int table_iters[2] = {0, 0};
     bool fast = true;
+    bool simple = true;   // fast, every beta 0 and max_llr 32767 (DecParams::simple)
 };
 void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s);
 // Algo 2 decodes at most as many iterations as its base graph's table holds

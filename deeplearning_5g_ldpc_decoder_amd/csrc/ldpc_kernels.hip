@@ -305,6 +305,18 @@ __device__ inline int scale_i16(int m, int alpha, int beta) {
     return wrap16((scaled + (1 << 14)) >> 15);
 }
 
+// scale_i16 for beta = 0 and m in [0, 32767] (min(m, 32767) = m): one
+// 24-bit multiply-add and a shift.
+__device__ __forceinline__ int scale_simple(int m, int alpha) {
+#ifdef NRLDPC_CPU_EMU
+    return (alpha * m + (1 << 14)) >> 15;
+#else
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(alpha), "v"(m), "s"(1 << 14));
+    return r >> 15;
+#endif
+}
+
 // Larger than every two-minimum key (magnitudes reach 32768, see below).
 constexpr uint32_t kKeyMax = (32768u << 5) | 31u;
 
@@ -729,7 +741,9 @@ __device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint
         *(lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]) = (int16_t)(v >> 16);
 }
 
-template <int BG, int ZC, int R, int N, int... I>
+// kSimple: DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps
+// every minimum within [0, 32767]).
+template <int BG, int ZC, int R, bool kSimple, int N, int... I>
 __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
                                            uint32_t zx2, const DecParams &p, int alpha, int beta,
                                            std::integer_sequence<int, I...>) {
@@ -744,8 +758,15 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
     rs.px = 0;
     (packed_pair_in<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, zx2, rs), ...);
     const int idx = (int)(rs.k1 & 31u);
-    const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
-    const int n1 = scale_i16(m1, alpha, beta), n2 = scale_i16(m2, alpha, beta);
+    int n1, n2;
+    if constexpr (kSimple) {
+        n1 = scale_simple((int)(rs.k1 >> 5), alpha);
+        n2 = scale_simple((int)(rs.k2 >> 5), alpha);
+    } else {
+        const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
+        n1 = scale_i16(m1, alpha, beta);
+        n2 = scale_i16(m2, alpha, beta);
+    }
     // odd number of negative inputs (bit 15 of px ^ px >> 16): every message
     // flips sign; pm holds that bit in the sign bit of both halves
     const uint32_t pm = ((rs.px ^ (rs.px >> 16)) & 0x8000u) * 0x10001u;
@@ -756,7 +777,7 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
                                                         pm, lo, hi), ...);
 }
 
-template <int BG, int ZC, int R, bool kGeneral, bool kTable, int N>
+template <int BG, int ZC, int R, bool kGeneral, bool kTable, bool kSimple, int N>
 __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], uint32_t zx2, bool run,
                                            const DecParams &p, const uint32_t *ab) {
     if constexpr (R < Graph<BG>::kRows) {
@@ -772,15 +793,15 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
                 fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
                                                std::make_integer_sequence<int, deg>{});
             else
-                packed_row<BG, ZC, R>(lqb, msg, zx2, p, alpha, beta,
-                                      std::make_integer_sequence<int, (deg + 1) / 2>{});
+                packed_row<BG, ZC, R, kSimple>(lqb, msg, zx2, p, alpha, beta,
+                                               std::make_integer_sequence<int, (deg + 1) / 2>{});
         }
         if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
-        fixed_rows<BG, ZC, R + 1, kGeneral, kTable>(lqb, msg, zx2, run, p, ab);
+        fixed_rows<BG, ZC, R + 1, kGeneral, kTable, kSimple>(lqb, msg, zx2, run, p, ab);
     }
 }
 
-template <int BG, int ZC, bool kGeneral, bool kTable>
+template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple>
 __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
                              const uint32_t *tab, uint32_t *slots, int *status) {
     // exact rows: one message per edge in global edge order; fast rows:
@@ -796,8 +817,8 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        fixed_rows<BG, ZC, 0, kGeneral, kTable>(lqb, msg, zx2, zx < ZC, p,
-                                                scale_row<BG, kTable>(p, it));
+        fixed_rows<BG, ZC, 0, kGeneral, kTable, kSimple>(lqb, msg, zx2, zx < ZC, p,
+                                                         scale_row<BG, kTable>(p, it));
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -1078,7 +1099,7 @@ struct FixedLds {
     static constexpr int kBytes = kSlotsOff + (kMaxThreads / 64) * 4;
 };
 
-template <int BG, int ZC, bool kGeneral, bool kTable>
+template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple = false>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     using LdsL = FixedLds<BG, ZC>;
@@ -1098,7 +1119,8 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
 
     int status = 1, it = 0;
     if (p.max_it > 0) {
-        it = run_fixed_i16<BG, ZC, kGeneral, kTable>(lq, p, crc, crc_tab, slots, &status);
+        it = run_fixed_i16<BG, ZC, kGeneral, kTable, kSimple>(lq, p, crc, crc_tab, slots,
+                                                              &status);
         for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
     }
     if (t == 0) {
@@ -1236,14 +1258,17 @@ const void *i16_kernel(bool general, bool table) {
 // Lifting-size specialized kernels (ldpc_decode_fixed_kernel): BG1 at the
 // largest lifting size.
 template <int BG, int ZC>
-const void *fixed_kernel_of(bool general, bool table) {
-    if (table)
-        return general
-                   ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, true>)
-                   : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, true>);
-    return general
-               ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, false>)
-               : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false>);
+const void *fixed_kernel_of(bool general, bool table, bool simple) {
+    if (general)
+        return table ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, true>)
+                     : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, true, false>);
+    if (simple)
+        return table
+                   ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, true, true>)
+                   : reinterpret_cast<const void *>(
+                         &ldpc_decode_fixed_kernel<BG, ZC, false, false, true>);
+    return table ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, true>)
+                 : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false>);
 }
 
 constexpr uint32_t fixed_pair(int bg, int zc) { return ((uint32_t)bg << 16) | (uint32_t)zc; }
@@ -1253,17 +1278,19 @@ constexpr uint32_t kFixedPairs[] = {fixed_pair(1, 384)};
 // have no static LDS (their belief addressing assumes LDS address 0).
 bool g_fixed_ok = false;
 
-std::pair<const void *, int> fixed_kernel_any(bool general, bool table, uint32_t uniform) {
+std::pair<const void *, int> fixed_kernel_any(bool general, bool table, bool simple,
+                                              uint32_t uniform) {
     if (uniform == fixed_pair(1, 384))
-        return {fixed_kernel_of<1, 384>(general, table), FixedLds<1, 384>::kBytes};
+        return {fixed_kernel_of<1, 384>(general, table, simple), FixedLds<1, 384>::kBytes};
     return {nullptr, 0};
 }
 
 // (kernel, dynamic LDS bytes) of the specialized kernel for `uniform`, or
 // (nullptr, 0) when there is none.
-std::pair<const void *, int> fixed_kernel(bool general, bool table, uint32_t uniform) {
+std::pair<const void *, int> fixed_kernel(bool general, bool table, bool simple,
+                                          uint32_t uniform) {
     if (!g_fixed_ok) return {nullptr, 0};
-    return fixed_kernel_any(general, table, uniform);
+    return fixed_kernel_any(general, table, simple, uniform);
 }
 
 const void *tile_kernel(bool general, bool table) {
@@ -1300,7 +1327,7 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
             return hipErrorInvalidValue;
         fn = i16_kernel(pv.general != 0, algo != 1);
         if (uniform) {
-            const auto fk = fixed_kernel(pv.general != 0, algo != 1, uniform);
+            const auto fk = fixed_kernel(pv.general != 0, algo != 1, pv.simple != 0, uniform);
             if (fk.first) {
                 fn = fk.first;
                 lds = (size_t)fk.second;
@@ -1332,20 +1359,21 @@ extern "C" hipError_t nrldpc_configure_kernels() {
             e = hipFuncSetAttribute(i16_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     max_lds);
             for (uint32_t u : kFixedPairs)
-                if (e == hipSuccess)
-                    e = hipFuncSetAttribute(fixed_kernel_any(g, t, u).first,
+                for (int m = 0; m < 2 && e == hipSuccess; ++m)
+                    e = hipFuncSetAttribute(fixed_kernel_any(g, t, m, u).first,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         }
     if (e != hipSuccess) return e;
     bool fixed_ok = true;
     for (int g = 0; g < 2; ++g)
         for (int t = 0; t < 2; ++t)
-            for (uint32_t u : kFixedPairs) {
-                hipFuncAttributes fa;
-                e = hipFuncGetAttributes(&fa, fixed_kernel_any(g, t, u).first);
-                if (e != hipSuccess) return e;
-                if (fa.sharedSizeBytes != 0) fixed_ok = false;
-            }
+            for (uint32_t u : kFixedPairs)
+                for (int m = 0; m < 2; ++m) {
+                    hipFuncAttributes fa;
+                    e = hipFuncGetAttributes(&fa, fixed_kernel_any(g, t, m, u).first);
+                    if (e != hipSuccess) return e;
+                    if (fa.sharedSizeBytes != 0) fixed_ok = false;
+                }
     g_fixed_ok = fixed_ok;
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),

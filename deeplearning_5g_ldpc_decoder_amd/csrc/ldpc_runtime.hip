@@ -131,6 +131,7 @@ void bind_scale(DecParams &dp, const ScaleSet &s, const uint32_t *dev) {
     dp.scale[1] = dev + s.off[1];
     dp.scale_iters = s.iters;
     dp.general = s.fast ? 0 : 1;
+    dp.simple = (s.fast && s.simple) ? 1 : 0;
 }
 
 // Algo 0 is part of the ABI but its GPU kernel is compiled out of this build

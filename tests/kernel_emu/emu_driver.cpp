@@ -55,6 +55,7 @@ int make_batch(int n, const int32_t *cfgs, const uint32_t *llr_off, const int32_
         b.p.scale[1] = b.scale.words.data() + b.scale.off[1];
         b.p.scale_iters = b.scale.iters;
         b.p.general = b.scale.fast ? 0 : 1;
+        b.p.simple = (b.scale.fast && b.scale.simple) ? 1 : 0;
     }
     b.descs.resize(n);
     uint32_t h = 0, q = 0;
@@ -112,7 +113,13 @@ template <int BG, int ZC>
 void launch_fixed(unsigned n, unsigned threads, const DecOut &o, const DecParams &p, bool g,
                   bool t) {
     const size_t lds = (size_t)FixedLds<BG, ZC>::kBytes;
-    if (!g && !t)
+    if (!g && p.simple && !t)
+        emu_launch(n, threads, lds,
+                   [&] { ldpc_decode_fixed_kernel<BG, ZC, false, false, true>(o, p); });
+    else if (!g && p.simple && t)
+        emu_launch(n, threads, lds,
+                   [&] { ldpc_decode_fixed_kernel<BG, ZC, false, true, true>(o, p); });
+    else if (!g && !t)
         emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, false, false>(o, p); });
     else if (g && !t)
         emu_launch(n, threads, lds, [&] { ldpc_decode_fixed_kernel<BG, ZC, true, false>(o, p); });

@@ -346,23 +346,29 @@ def test_emu_synth_matches_oracle(oracle_mod, cfg):
 
 
 @pytest.mark.parametrize("pair", [(2, 26), (1, 15)])
-@pytest.mark.parametrize("row", [[1, 8, 22936, 0, 1280, 32767, -32767],
-                                 [1, 6, 22936, 40000, 21279, 32767, -32767],
-                                 [1, 6, 40000, 16, 1280, 300, 2000],
-                                 [2, 6, 0, 24, 1280, 32767, -32767],
-                                 [3, 6, 0, 40000, 1280, 32767, -32767]])
-def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, case_tables):
-    """The lifting-size specialized kernel (compile-time shifts, explicit
-    per-edge messages) on uniform batches: bit-exact with the generic kernel
-    and with the oracle, for algo 1 fast and exact row forms and algos 2/3
-    with the adversarial tables."""
+@pytest.mark.parametrize("row,tab", [
+    ([1, 8, 22936, 0, 1280, 32767, -32767], 2),      # packed, simple scale
+    ([1, 8, 26000, 32, -900, 32767, -32767], 2),     # packed, offset min-sum
+    ([1, 8, 22936, 0, 1280, 2000, -1500], 2),        # packed, tight clip
+    ([1, 6, 22936, 40000, 21279, 32767, -32767], 2),  # exact: message wrap
+    ([1, 6, 40000, 16, 1280, 300, 2000], 2),          # exact: alpha < 0, min > max
+    ([2, 6, 0, 24, 1280, 32767, -32767], 0),          # default tables: packed, simple
+    ([2, 6, 0, 24, 1280, 32767, -32767], 1),          # smooth tables: packed, beta != 0
+    ([2, 6, 0, 24, 1280, 32767, -32767], 2),          # adversarial tables: exact
+    ([3, 6, 0, 0, 1280, 32767, -32767], 1),           # degree alpha: packed, simple
+    ([3, 6, 0, 40000, 1280, 32767, -32767], 2)])      # exact
+def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, tab, case_tables):
+    """The lifting-size specialized kernel (compile-time shifts; packed
+    row-paired messages for fast rows, simple scale when beta = 0 and
+    max_llr = 32767; per-edge messages for exact rows) on uniform batches:
+    bit-exact with the generic kernel and with the oracle."""
     bgn, zc = pair
     rng = np.random.default_rng(1000 * bgn + zc + row[0])
     cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=4, zcs=[zc])
     cfgs, llrs = zip(*[(c, l) for c, l in zip(cfgs, llrs) if c[0] == bgn])
     z = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo2.npz"))
-    tabs = (z[f"alpha_bg{bgn}"][2], z[f"beta_bg{bgn}"][2])
-    dalpha = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo3.npz"))["degree_alpha"][2]
+    tabs = (z[f"alpha_bg{bgn}"][tab], z[f"beta_bg{bgn}"][tab])
+    dalpha = np.load(os.path.join(ROOT, "tests", "golden", "decode_algo3.npz"))["degree_alpha"][tab]
     case_tables(dict(cfg=(bgn,), tables=tabs, degree_alpha=dalpha))
     outs = {}
     try:
