@@ -177,21 +177,22 @@ __device__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, 
     }
 }
 
+template <int kBatch = kScatterBatch>
 __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
                                 int first, int stride) {
     const RecoverCtx c = recover_ctx(d);
     // floor(j / Qm) = umulhi(j, ceil(2^32 / Qm)), exact for j < 2^16 and Qm <= 8
     const uint32_t magic = c.Qm > 1 ? 0xFFFFFFFFu / (uint32_t)c.Qm + 1u : 0u;
     const int p0 = 2 * c.Zc;
-    for (int j0 = first; j0 < c.E; j0 += stride * kScatterBatch) {
-        int v[kScatterBatch];
+    for (int j0 = first; j0 < c.E; j0 += stride * kBatch) {
+        int v[kBatch];
 #pragma unroll
-        for (int u = 0; u < kScatterBatch; ++u) {
+        for (int u = 0; u < kBatch; ++u) {
             const int j = j0 + u * stride;
             v[u] = j < c.E ? (int)llr[j] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < kScatterBatch; ++u) {
+        for (int u = 0; u < kBatch; ++u) {
             const int j = j0 + u * stride;
             if (j < c.E) {
                 const int sym = c.Qm > 1 ? (int)__umulhi((uint32_t)j, magic) : j;
@@ -210,6 +211,7 @@ __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__r
 // (it holds two workgroup barriers); lanes with active = false only join
 // them.  The caller's next barrier orders the soft-buffer write-back before
 // decoding changes the beliefs.
+template <int kBatch = kScatterBatch>
 __device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, int16_t *soft,
                             int filler, int first, int stride, int n, bool active) {
     const bool scatter = active && d.E <= d.N - (d.K - d.B);
@@ -220,7 +222,7 @@ __device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, in
             recover_fill<int16_t>(lq, d, llr, soft, filler, first, stride, n);
     }
     __syncthreads();
-    if (scatter) recover_scatter(lq, d, llr, first, stride);
+    if (scatter) recover_scatter<kBatch>(lq, d, llr, first, stride);
     __syncthreads();
     if (scatter && soft)
         for (int p = first; p < n; p += stride) soft[p] = lq[p];
@@ -1113,7 +1115,8 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
+    // the fixed kernel has registers to spare before its message state is live
+    recover_i16<2 * kScatterBatch>(lq, d, llr, soft, p.filler, t, nt, n, true);
     const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
