@@ -158,28 +158,46 @@ __device__ void recover_fill(T *lq, const CbDesc &d, const int16_t *__restrict__
 constexpr int kScatterBatch = 16;
 
 // Initial beliefs: filler positions take the filler value, the others the
-// soft buffer (HARQ combining) or 0.
+// soft buffer (HARQ combining) or 0.  Lanes write 8 positions (16 bytes) at a
+// time: chunk c = first, first + stride, ... covers positions [8c, 8c + 8);
+// the beliefs and soft buffers are 16-byte aligned.
+__device__ __forceinline__ uint32_t put_half(uint32_t w, int hi, int v) {
+    return hi ? ((w & 0xFFFFu) | ((uint32_t)v << 16)) : ((w & 0xFFFF0000u) | ((uint32_t)v & 0xFFFFu));
+}
+
 __device__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, int filler,
                              int first, int stride, int n) {
     const bool harq = d.harq_on && soft;
-    for (int i0 = first; i0 < n; i0 += stride * kRecoverBatch) {
-        int v[kRecoverBatch];
-#pragma unroll
-        for (int u = 0; u < kRecoverBatch; ++u) {
-            const int p = i0 + u * stride;
-            v[u] = (harq && p < n) ? (int)soft[p] : 0;
+    const int chunks = n >> 3;
+    for (int c = first; c < chunks; c += stride) {
+        const int p0 = 8 * c;
+        uint4 v;
+        if (harq) {
+            v = *reinterpret_cast<const uint4 *>(soft + p0);
+        } else {
+            v.x = v.y = v.z = v.w = 0u;
         }
+        if (p0 + 8 > d.B && p0 < d.K) {   // the chunk meets the filler range
+            uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int u = 0; u < kRecoverBatch; ++u) {
-            const int p = i0 + u * stride;
-            if (p < n) lq[p] = (int16_t)((p >= d.B && p < d.K) ? filler : v[u]);
+            for (int u = 0; u < 8; ++u)
+                if (p0 + u >= d.B && p0 + u < d.K) w[u >> 1] = put_half(w[u >> 1], u & 1, filler);
+            v.x = w[0];
+            v.y = w[1];
+            v.z = w[2];
+            v.w = w[3];
         }
+        *reinterpret_cast<uint4 *>(lq + p0) = v;
     }
+    for (int p = 8 * chunks + first; p < n; p += stride)
+        lq[p] = (int16_t)((p >= d.B && p < d.K) ? filler : (harq ? (int)soft[p] : 0));
 }
 
+// harq: the initial beliefs hold soft-buffer values to add to; otherwise they
+// are 0 and each LLR is stored directly.
 template <int kBatch = kScatterBatch>
 __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
-                                int first, int stride) {
+                                int first, int stride, bool harq) {
     const RecoverCtx c = recover_ctx(d);
     // floor(j / Qm) = umulhi(j, ceil(2^32 / Qm)), exact for j < 2^16 and Qm <= 8
     const uint32_t magic = c.Qm > 1 ? 0xFFFFFFFFu / (uint32_t)c.Qm + 1u : 0u;
@@ -200,7 +218,7 @@ __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__r
                 int q = bit * c.S + sym + c.ck0;
                 if (q >= c.nc) q -= c.nc;
                 const int p = (q < c.lo ? q : q + c.F) + p0;
-                lq[p] = (int16_t)(lq[p] + v[u]);
+                lq[p] = (int16_t)(harq ? lq[p] + v[u] : v[u]);
             }
         }
     }
@@ -222,7 +240,7 @@ __device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, in
             recover_fill<int16_t>(lq, d, llr, soft, filler, first, stride, n);
     }
     __syncthreads();
-    if (scatter) recover_scatter<kBatch>(lq, d, llr, first, stride);
+    if (scatter) recover_scatter<kBatch>(lq, d, llr, first, stride, d.harq_on && soft);
     __syncthreads();
     if (scatter && soft)
         for (int p = first; p < n; p += stride) soft[p] = lq[p];
