@@ -771,7 +771,6 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     constexpr int pairs = (int)sizeof...(I);
     constexpr int w0 = packed_word0<BG>(R);
-    NRLDPC_OPAQUE_LANE(zx2);
     PackedRowState<pairs> rs;
     rs.k1 = kKeyMax;
     rs.k2 = kKeyMax;
@@ -832,11 +831,12 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
 #pragma unroll
     for (int i = 0; i < kWords; ++i) msg[i] = 0u;
     const int zx = threadIdx.x;
-    const uint32_t zx2 = 2u * (uint32_t)zx;
+    uint32_t zx2 = 2u * (uint32_t)zx;
     lds_char *lqb = fixed_lds_base(lq);
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
+        if constexpr (!kGeneral) NRLDPC_OPAQUE_LANE(zx2);   // packed rows: once per iteration
         fixed_rows<BG, ZC, 0, kGeneral, kTable, kSimple>(lqb, msg, zx2, zx < ZC, p,
                                                          scale_row<BG, kTable>(p, it));
         ++it;
@@ -1133,8 +1133,7 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    // the fixed kernel has registers to spare before its message state is live
-    recover_i16<2 * kScatterBatch>(lq, d, llr, soft, p.filler, t, nt, n, true);
+    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
