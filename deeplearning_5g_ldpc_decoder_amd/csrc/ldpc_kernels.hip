@@ -661,9 +661,21 @@ __device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t l
     const int y = min(max(sat16(pk_hi(a) + pk_hi(b)), pk_hi(lo)), pk_hi(hi));
     return pk_make(x, y);
 }
-// (half of a) * 32 + c
-__device__ __forceinline__ uint32_t key_lo(uint32_t a, uint32_t c) { return (a & 0xFFFFu) * 32u + c; }
-__device__ __forceinline__ uint32_t key_hi(uint32_t a, uint32_t c) { return (a >> 16) * 32u + c; }
+// per half: (a + b) mod 2^16
+__device__ __forceinline__ uint32_t pk_addw(uint32_t a, uint32_t b) {
+    return pk_make(pk_lo(a) + pk_lo(b), pk_hi(a) + pk_hi(b));
+}
+// per half: min / max (int16)
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return pk_make(min(pk_lo(a), pk_lo(b)), min(pk_hi(a), pk_hi(b)));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return pk_make(max(pk_lo(a), pk_lo(b)), max(pk_hi(a), pk_hi(b)));
+}
+// per half: max(a, lo) (a saturated sum a + b, then clipped below)
+__device__ __forceinline__ uint32_t pk_update_lo(uint32_t a, uint32_t b, uint32_t lo) {
+    return pk_make(max(sat16(pk_lo(a) + pk_lo(b)), pk_lo(lo)), max(sat16(pk_hi(a) + pk_hi(b)), pk_hi(lo)));
+}
 #else
 typedef short nr_s2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ nr_s2 as_s2(uint32_t v) { return __builtin_bit_cast(nr_s2, v); }
@@ -674,6 +686,9 @@ __device__ __forceinline__ uint32_t pk_make(int lo, int hi) {
 __device__ __forceinline__ uint32_t pk_subw(uint32_t a, uint32_t b) {
     return as_u32(as_s2(a) - as_s2(b));
 }
+__device__ __forceinline__ uint32_t pk_addw(uint32_t a, uint32_t b) {
+    return as_u32(as_s2(a) + as_s2(b));
+}
 __device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
     const nr_s2 v = as_s2(r);
     return as_u32(__builtin_elementwise_max(v, __builtin_elementwise_sub_sat((nr_s2)0, v)));
@@ -683,15 +698,14 @@ __device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t l
     const nr_s2 v = __builtin_elementwise_add_sat(as_s2(a), as_s2(b));
     return as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, as_s2(lo)), as_s2(hi)));
 }
-__device__ __forceinline__ uint32_t key_lo(uint32_t a, uint32_t c) {
-    uint32_t r;
-    asm("v_mad_u32_u16 %0, %1, 32, %2" : "=v"(r) : "v"(a), "v"(c));
-    return r;
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return as_u32(__builtin_elementwise_min(as_s2(a), as_s2(b)));
 }
-__device__ __forceinline__ uint32_t key_hi(uint32_t a, uint32_t c) {
-    uint32_t r;
-    asm("v_mad_u32_u16 %0, %1, 32, %2 op_sel:[1,0,0,0]" : "=v"(r) : "v"(a), "v"(c));
-    return r;
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return as_u32(__builtin_elementwise_max(as_s2(a), as_s2(b)));
+}
+__device__ __forceinline__ uint32_t pk_update_lo(uint32_t a, uint32_t b, uint32_t lo) {
+    return as_u32(__builtin_elementwise_max(__builtin_elementwise_add_sat(as_s2(a), as_s2(b)), as_s2(lo)));
 }
 #endif
 
@@ -707,55 +721,89 @@ template <int PAIRS>
 struct PackedRowState {
     uint32_t oa[PAIRS], ob[PAIRS];   // byte offsets of the pair's two variables
     uint32_t rel[PAIRS];             // extrinsic inputs, int16 halves
-    uint32_t k1, k2, px;
+    uint32_t mag[PAIRS];             // |rel| capped at 32767
+    uint32_t m1, m2;                 // per half: smallest and second smallest |rel|
+    uint32_t px;
 };
 
+// The first pass keeps the two smallest magnitudes per half; no edge index is
+// needed.  An edge's outgoing magnitude is the scaled minimum over the other
+// edges: f(m2) for an edge with |rel| = m1 (a tie makes m2 = m1), f(m1) for
+// every other edge, whose |rel| is then at least m2.  The fast-row scale f
+// (Q15 alpha in [0, 32767] after the offset and the max_llr cap) is
+// non-decreasing with f(m2) - f(m1) <= m2 - m1, so both cases are
+//     max((f(m2) + m1) - |rel|, f(m1)),
+// whose exact value lies in [-32767, 32767]: int16 wrap arithmetic is exact.
 template <int BG, int ZC, int E, int I, int DEG, int PAIRS>
 __device__ __forceinline__ void packed_pair_in(const lds_char *__restrict__ lqb,
                                                const uint32_t *msg, int w, uint32_t zx2,
-                                               PackedRowState<PAIRS> &rs) {
+                                               uint32_t zz, PackedRowState<PAIRS> &rs) {
     using F = FixedCode<BG, ZC>;
     constexpr bool has_b = 2 * I + 1 < DEG;
     constexpr uint32_t sha = F::shift_bytes(E), cba = F::col_bytes(E);
-    const uint32_t ca = zx2 + sha;
-    const uint32_t oa = min(ca, ca - 2u * (uint32_t)ZC);
-    rs.oa[I] = oa;
-    uint32_t q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
-    if constexpr (has_b) {
+    uint32_t q;
+    if constexpr (has_b && sha != 0 && F::shift_bytes(E + 1) != 0) {
+        // both offsets at once: zz holds 2*zx in each half, so t holds
+        // 2*zx + shift bytes per half (< 2 * 2Zc: no carry), u the same less
+        // 2Zc bytes per half, and each offset is the smaller of its halves
         constexpr uint32_t shb = F::shift_bytes(E + 1), cbb = F::col_bytes(E + 1);
-        const uint32_t cb = zx2 + shb;
-        const uint32_t ob = min(cb, cb - 2u * (uint32_t)ZC);
+        constexpr uint32_t wrap = (uint32_t)(uint16_t)(-2 * ZC) * 0x10001u;
+        const uint32_t t = zz + (sha | (shb << 16));
+        const uint32_t u = pk_addw(t, wrap);
+        const uint32_t oa = min(t & 0xFFFFu, u & 0xFFFFu), ob = min(t >> 16, u >> 16);
+        rs.oa[I] = oa;
         rs.ob[I] = ob;
+        q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
         q |= (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cbb + ob) << 16;
+    } else {
+        // a zero shift needs no arithmetic (zx2 < 2Zc): per-edge offsets
+        const uint32_t ca = zx2 + sha;
+        const uint32_t oa = min(ca, ca - 2u * (uint32_t)ZC);
+        rs.oa[I] = oa;
+        if constexpr (has_b) {
+            constexpr uint32_t shb = F::shift_bytes(E + 1), cbb = F::col_bytes(E + 1);
+            const uint32_t cb = zx2 + shb;
+            const uint32_t ob = min(cb, cb - 2u * (uint32_t)ZC);
+            rs.ob[I] = ob;
+            q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
+            q |= (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cbb + ob) << 16;
+        } else {
+            q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
+        }
     }
     const uint32_t rel = pk_subw(q, msg[w]);
     rs.rel[I] = rel;
     rs.px ^= rel;
-    const uint32_t a = pk_abs_sat(rel);
-    const uint32_t ka = key_lo(a, (uint32_t)(2 * I));
-    rs.k2 = med3_u32(rs.k1, ka, rs.k2);
-    rs.k1 = min(rs.k1, ka);
-    if constexpr (has_b) {
-        const uint32_t kb = key_hi(a, (uint32_t)(2 * I + 1));
-        rs.k2 = med3_u32(rs.k1, kb, rs.k2);
-        rs.k1 = min(rs.k1, kb);
+    uint32_t a = pk_abs_sat(rel);
+    // an odd last edge: the unused high half is at least every magnitude
+    if constexpr (!has_b) a |= 0x7FFF0000u;
+    rs.mag[I] = a;
+    if constexpr (I == 0) {
+        rs.m1 = a;
+    } else if constexpr (I == 1) {
+        rs.m2 = pk_max(rs.m1, a);
+        rs.m1 = pk_min(rs.m1, a);
+    } else {
+        rs.m2 = pk_min(rs.m2, pk_max(rs.m1, a));
+        rs.m1 = pk_min(rs.m1, a);
     }
 }
 
-template <int BG, int ZC, int E, int I, int DEG, int PAIRS, int N>
+template <int BG, int ZC, int E, int I, int DEG, int PAIRS, bool kSimple, int N>
 __device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
                                                 int w, const PackedRowState<PAIRS> &rs,
-                                                int ipair, uint32_t n1p, uint32_t n2p,
-                                                uint32_t pm, uint32_t lo, uint32_t hi) {
+                                                uint32_t cp, uint32_t n1p, uint32_t pm,
+                                                uint32_t lo, uint32_t hi) {
     using F = FixedCode<BG, ZC>;
     constexpr bool has_b = 2 * I + 1 < DEG;
-    uint32_t nv = (ipair == I) ? n2p : n1p;
+    uint32_t nv = pk_max(pk_subw(cp, rs.mag[I]), n1p);
     if constexpr (!has_b) nv &= 0xFFFFu;   // keeps the unused high half at 0
     const uint32_t rel = rs.rel[I];
     const uint32_t sm = pk_sign15(rel ^ pm);
     const uint32_t m = pk_subw(nv ^ sm, sm);
     msg[w] = m;
-    const uint32_t v = pk_update(rel, m, lo, hi);
+    // kSimple has max_llr = 32767: the saturated add already clips above
+    const uint32_t v = kSimple ? pk_update_lo(rel, m, lo) : pk_update(rel, m, lo, hi);
     *(lds_i16 *)(lqb + F::col_bytes(E) + rs.oa[I]) = (int16_t)(v & 0xFFFFu);
     if constexpr (has_b)
         *(lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]) = (int16_t)(v >> 16);
@@ -771,36 +819,41 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     constexpr int pairs = (int)sizeof...(I);
     constexpr int w0 = packed_word0<BG>(R);
+    static_assert(pairs >= 2, "packed rows need degree >= 3");
     PackedRowState<pairs> rs;
-    rs.k1 = kKeyMax;
-    rs.k2 = kKeyMax;
     rs.px = 0;
-    (packed_pair_in<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, zx2, rs), ...);
-    const int idx = (int)(rs.k1 & 31u);
+    const uint32_t zz = zx2 | (zx2 << 16);
+    (packed_pair_in<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, zx2, zz, rs), ...);
+    // merge the halves: m1 = min of the four, m2 = second smallest
+    const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
+    const uint32_t m2l = rs.m2 & 0xFFFFu, m2h = rs.m2 >> 16;
+    const int m1 = (int)min(m1l, m1h);
+    const int m2 = (int)min(max(m1l, m1h), min(m2l, m2h));
     int n1, n2;
     if constexpr (kSimple) {
-        n1 = scale_simple((int)(rs.k1 >> 5), alpha);
-        n2 = scale_simple((int)(rs.k2 >> 5), alpha);
+        n1 = scale_simple(m1, alpha);
+        n2 = scale_simple(m2, alpha);
     } else {
-        const int m1 = min((int)(rs.k1 >> 5), p.max_llr), m2 = min((int)(rs.k2 >> 5), p.max_llr);
-        n1 = scale_i16(m1, alpha, beta);
-        n2 = scale_i16(m2, alpha, beta);
+        n1 = scale_i16(min(m1, p.max_llr), alpha, beta);
+        n2 = scale_i16(min(m2, p.max_llr), alpha, beta);
     }
-    // odd number of negative inputs (bit 15 of px ^ px >> 16): every message
-    // flips sign; pm holds that bit in the sign bit of both halves
-    const uint32_t pm = ((rs.px ^ (rs.px >> 16)) & 0x8000u) * 0x10001u;
-    const uint32_t n1p = pk_make(n1, n1);
-    const uint32_t n2p = (idx & 1) ? pk_make(n1, n2) : pk_make(n2, n1);
+    // odd number of negative inputs: every message flips sign.  px ^ rot16(px)
+    // holds that parity in the sign bit of both halves (the other bits are
+    // never read: only the sign of rel ^ pm is)
+    const uint32_t pm = rs.px ^ ((rs.px >> 16) | (rs.px << 16));
+    const uint32_t n1p = (uint32_t)n1 * 0x10001u;   // n1 in [0, 32767]
+    const uint32_t c = (uint32_t)(n2 + m1) & 0xFFFFu;
+    const uint32_t cp = c | (c << 16);
     const uint32_t lo = pk_make(p.min_llr, p.min_llr), hi = pk_make(p.max_llr, p.max_llr);
-    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, rs, idx >> 1, n1p, n2p,
-                                                        pm, lo, hi), ...);
+    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, pairs, kSimple>(lqb, msg, w0 + I, rs, cp, n1p, pm,
+                                                                 lo, hi), ...);
 }
 
 template <int BG, int ZC, int R, bool kGeneral, bool kTable, bool kSimple, int N>
 __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], uint32_t zx2, bool run,
-                                           const DecParams &p, const uint32_t *ab) {
+                                           const DecParams &p, const uint32_t *ab, int alpha0) {
     if constexpr (R < Graph<BG>::kRows) {
-        int alpha = p.alpha, beta = p.beta;
+        int alpha = alpha0, beta = p.beta;
         if constexpr (kTable) {
             const uint32_t w = ab[R];
             alpha = (int)(int16_t)(w & 0xFFFFu);
@@ -816,7 +869,7 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
                                                std::make_integer_sequence<int, (deg + 1) / 2>{});
         }
         if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
-        fixed_rows<BG, ZC, R + 1, kGeneral, kTable, kSimple>(lqb, msg, zx2, run, p, ab);
+        fixed_rows<BG, ZC, R + 1, kGeneral, kTable, kSimple>(lqb, msg, zx2, run, p, ab, alpha0);
     }
 }
 
@@ -836,9 +889,15 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        if constexpr (!kGeneral) NRLDPC_OPAQUE_LANE(zx2);   // packed rows: once per iteration
+        // packed rows: lane values made opaque once per iteration (keeps the
+        // offsets from being hoisted into registers; alpha stays in one VGPR)
+        int alpha0 = p.alpha;
+        if constexpr (!kGeneral) {
+            NRLDPC_OPAQUE_LANE(zx2);
+            if constexpr (!kTable) NRLDPC_OPAQUE_LANE(alpha0);
+        }
         fixed_rows<BG, ZC, 0, kGeneral, kTable, kSimple>(lqb, msg, zx2, zx < ZC, p,
-                                                         scale_row<BG, kTable>(p, it));
+                                                         scale_row<BG, kTable>(p, it), alpha0);
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
