@@ -224,6 +224,57 @@ __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__r
     }
 }
 
+// recover_scatter for LLRs in 16-byte chunks (the codeblock's LLRs 16-byte
+// aligned, E a multiple of 8): kChunks loads of 8 LLRs in flight per lane, so
+// one round trip to memory covers E <= 8 * kChunks * stride.  Within a chunk
+// the rate-matched bit k = bit * S + sym advances by S per LLR, and by
+// 1 - (Qm - 1) * S when bit wraps to 0 (E = Qm * S).
+constexpr int kScatterChunks = 12;
+
+template <int kChunks = kScatterChunks>
+__device__ void recover_scatter_x8(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+                                   int first, int stride, bool harq) {
+    const RecoverCtx c = recover_ctx(d);
+    const uint4 *src = reinterpret_cast<const uint4 *>(llr);
+    const int chunks = c.E >> 3;
+    const int p0 = 2 * c.Zc;
+    const int back = 1 - (c.Qm - 1) * c.S;
+    const uint32_t magic = c.Qm > 1 ? 0xFFFFFFFFu / (uint32_t)c.Qm + 1u : 0u;   // as in recover_scatter
+    for (int c0 = first; c0 < chunks; c0 += stride * kChunks) {
+        uint4 v[kChunks];
+#pragma unroll
+        for (int u = 0; u < kChunks; ++u) {
+            const int ch = c0 + u * stride;
+            if (ch < chunks) v[u] = src[ch];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunks; ++u) {
+            const int ch = c0 + u * stride;
+            if (ch < chunks) {
+                const int j0 = 8 * ch;
+                const int sym = c.Qm > 1 ? (int)__umulhi((uint32_t)j0, magic) : j0;
+                int bit = j0 - sym * c.Qm;
+                int k = bit * c.S + sym;   // rate-matched bit of LLR j0
+                const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int val = (int)(int16_t)((e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xFFFFu));
+                    int q = k + c.ck0;   // k < E <= N - F: one wrap at most
+                    if (q >= c.nc) q -= c.nc;
+                    const int pp = (q < c.lo ? q : q + c.F) + p0;
+                    lq[pp] = (int16_t)(harq ? lq[pp] + val : val);
+                    if (++bit == c.Qm) {
+                        bit = 0;
+                        k += back;
+                    } else {
+                        k += c.S;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Rate recovery of one codeblock into int16 beliefs and its soft buffer by
 // lanes first, first + stride, ... .  Every lane of the workgroup calls it
 // (it holds two workgroup barriers); lanes with active = false only join
@@ -240,7 +291,13 @@ __device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, in
             recover_fill<int16_t>(lq, d, llr, soft, filler, first, stride, n);
     }
     __syncthreads();
-    if (scatter) recover_scatter<kBatch>(lq, d, llr, first, stride, d.harq_on && soft);
+    if (scatter) {
+        const bool harq = d.harq_on && soft;
+        if (((uintptr_t)llr & 15u) == 0 && (d.E & 7) == 0)
+            recover_scatter_x8(lq, d, llr, first, stride, harq);
+        else
+            recover_scatter<kBatch>(lq, d, llr, first, stride, harq);
+    }
     __syncthreads();
     if (scatter && soft)
         for (int p = first; p < n; p += stride) soft[p] = lq[p];
@@ -1192,8 +1249,10 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
+    // CRC tables first (their own LDS region): the loads overlap the recovery,
+    // whose barriers publish the table
     const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
+    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     __syncthreads();
 
     int status = 1, it = 0;

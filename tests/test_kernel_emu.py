@@ -387,6 +387,50 @@ def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, tab,
         assert np.array_equal(a["lq"], q), cfg
 
 
+def test_emu_scatter_chunked_and_unaligned(oracle_mod):
+    """Rate recovery without repetition reads 16-byte chunks of 8 LLRs when
+    the codeblock's LLRs are 16-byte aligned and E is a multiple of 8, and
+    single LLRs otherwise.  Both forms, in the per-codeblock and the packed
+    tile kernels, must give the oracle's buffer: Qm 1-8, every rv, filler
+    bits, HARQ combining into a soft buffer."""
+    rng = np.random.default_rng(11)
+    cfgs = [(1, 8424, 2, 25344, 0, 2, 0)]
+    # E a multiple of 8 and not, just inside N - F (no repetition)
+    for bgn, A, crc, rv, qm, harq, frac in [(1, 3816, 2, 2, 4, 1, 0.9), (2, 1000, 1, 3, 6, 0, 0.8),
+                                            (2, 296, 1, 1, 8, 1, 0.95), (1, 504, 1, 0, 1, 0, 0.7),
+                                            (2, 200, 1, 0, 2, 1, 0.9), (2, 200, 1, 2, 2, 0, 0.85)]:
+        c = oracle_mod.code(bgn, A, crc, 200 * qm, rv, qm)
+        e = int((c.N - (c.K - c.B)) * frac) // qm * qm
+        cfgs.append((bgn, A, crc, e, rv, qm, harq))
+    assert {c[3] % 8 == 0 for c in cfgs} == {True, False}
+    pad_cfg = (2, 40, 0, 150, 0, 2, 0)   # 150 LLRs: misaligns what follows
+    pad_llr = rng.integers(-100, 100, 150).astype(np.int16)
+    checked = 0
+    for cfg in cfgs:
+        c = oracle_mod.code(*cfg)
+        assert c.E <= c.N - (c.K - c.B), cfg   # no repetition: the scatter form
+        n = c.N + 2 * c.Zc
+        llr = rng.integers(-3000, 3000, c.E).astype(np.int16)
+        soft = rng.integers(-2000, 2000, n).astype(np.int16) if cfg[6] else None
+        want = oracle_mod.rate_recover(c, 77, llr, soft)
+        row = [1, 0, 22936, 0, 77, 32767, -32767]
+        for lead in (False, True):
+            for tile in (None, 0):
+                pool = np.zeros(2 * 68 * 384, np.int16)
+                if soft is not None:
+                    pool[68 * 384:68 * 384 + n] = soft
+                cs, ls, slots = [cfg], [llr], [1]
+                if lead:
+                    cs, ls, slots = [pad_cfg, cfg], [pad_llr, llr], [-1, 1]
+                out = emu.decode_batch(cs, row, ls, soft_pool=pool, soft_slots=slots,
+                                       tile_k=tile)[-1]
+                assert np.array_equal(out["lq"], want), (cfg, lead, tile)
+                if soft is not None:
+                    assert np.array_equal(pool[68 * 384:68 * 384 + n], want), (cfg, lead, tile)
+                checked += 1
+    assert checked == 4 * len(cfgs)
+
+
 def test_emu_rate_recovery_matches_reference():
     """The kernels' batched gather (recover_fill) on the reference's
     HARQ_memory fixtures: heavy repetition with int16 wrap, every rv, Qm 1-8,
