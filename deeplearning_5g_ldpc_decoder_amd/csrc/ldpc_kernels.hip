@@ -708,9 +708,13 @@ __device__ __forceinline__ uint32_t pk_subw(uint32_t a, uint32_t b) {
 __device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
     return pk_make(max(pk_lo(r), sat16(-pk_lo(r))), max(pk_hi(r), sat16(-pk_hi(r))));
 }
-// per half: r >> 15 (arithmetic): 0 or -1
-__device__ __forceinline__ uint32_t pk_sign15(uint32_t r) {
-    return pk_make(pk_lo(r) >> 15, pk_hi(r) >> 15);
+// per half: +1 for a clear sign bit, -1 for a set one
+__device__ __forceinline__ uint32_t pk_sgn(uint32_t r) {
+    return pk_make(pk_lo(r) < 0 ? -1 : 1, pk_hi(r) < 0 ? -1 : 1);
+}
+// per half: (a * b) mod 2^16
+__device__ __forceinline__ uint32_t pk_mullo(uint32_t a, uint32_t b) {
+    return pk_make(pk_lo(a) * pk_lo(b), pk_hi(a) * pk_hi(b));
 }
 // per half: min(max(sat(a + b), lo), hi)
 __device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t lo, uint32_t hi) {
@@ -750,7 +754,13 @@ __device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
     const nr_s2 v = as_s2(r);
     return as_u32(__builtin_elementwise_max(v, __builtin_elementwise_sub_sat((nr_s2)0, v)));
 }
-__device__ __forceinline__ uint32_t pk_sign15(uint32_t r) { return as_u32(as_s2(r) >> 15); }
+// bits 15 and 14 of each half after forcing bit 14: 0b01 -> +1, 0b11 -> -1
+__device__ __forceinline__ uint32_t pk_sgn(uint32_t r) {
+    return as_u32(as_s2(r | 0x40004000u) >> 14);
+}
+__device__ __forceinline__ uint32_t pk_mullo(uint32_t a, uint32_t b) {
+    return as_u32(as_s2(a) * as_s2(b));
+}
 __device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t lo, uint32_t hi) {
     const nr_s2 v = __builtin_elementwise_add_sat(as_s2(a), as_s2(b));
     return as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, as_s2(lo)), as_s2(hi)));
@@ -856,8 +866,8 @@ __device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint
     uint32_t nv = pk_max(pk_subw(cp, rs.mag[I]), n1p);
     if constexpr (!has_b) nv &= 0xFFFFu;   // keeps the unused high half at 0
     const uint32_t rel = rs.rel[I];
-    const uint32_t sm = pk_sign15(rel ^ pm);
-    const uint32_t m = pk_subw(nv ^ sm, sm);
+    // message = magnitude * (+-1): the sign of rel, flipped for odd row parity
+    const uint32_t m = pk_mullo(nv, pk_sgn(rel ^ pm));
     msg[w] = m;
     // kSimple has max_llr = 32767: the saturated add already clips above
     const uint32_t v = kSimple ? pk_update_lo(rel, m, lo) : pk_update(rel, m, lo, hi);
