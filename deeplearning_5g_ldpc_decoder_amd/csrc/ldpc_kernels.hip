@@ -726,6 +726,10 @@ __device__ __forceinline__ uint32_t pk_update(uint32_t a, uint32_t b, uint32_t l
 __device__ __forceinline__ uint32_t pk_addw(uint32_t a, uint32_t b) {
     return pk_make(pk_lo(a) + pk_lo(b), pk_hi(a) + pk_hi(b));
 }
+// low halves of lo and hi as one word
+__device__ __forceinline__ uint32_t pk_join(uint32_t lo, uint32_t hi) {
+    return (lo & 0xFFFFu) | (hi << 16);
+}
 // per half: min / max (int16)
 __device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
     return pk_make(min(pk_lo(a), pk_lo(b)), min(pk_hi(a), pk_hi(b)));
@@ -749,6 +753,10 @@ __device__ __forceinline__ uint32_t pk_subw(uint32_t a, uint32_t b) {
 }
 __device__ __forceinline__ uint32_t pk_addw(uint32_t a, uint32_t b) {
     return as_u32(as_s2(a) + as_s2(b));
+}
+// one v_perm_b32: bytes 0-1 of lo, then bytes 0-1 of hi
+__device__ __forceinline__ uint32_t pk_join(uint32_t lo, uint32_t hi) {
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 __device__ __forceinline__ uint32_t pk_abs_sat(uint32_t r) {
     const nr_s2 v = as_s2(r);
@@ -784,14 +792,107 @@ __host__ __device__ constexpr int packed_word0(int r) {
     return w;
 }
 
+// Barrier groups as index ranges: group g holds rows first[g] .. first[g+1]-1.
+template <int BG>
+struct GroupPlan {
+    int group_of[Graph<BG>::kRows];
+    int first[Graph<BG>::kRows + 1];
+    int groups, max_rows, max_pairs;
+    constexpr GroupPlan() : group_of(), first(), groups(0), max_rows(0), max_pairs(0) {
+        int g = 0;
+        first[0] = 0;
+        for (int r = 0; r < Graph<BG>::kRows; ++r) {
+            group_of[r] = g;
+            const int pr = (Graph<BG>::start(r + 1) - Graph<BG>::start(r) + 1) / 2;
+            max_pairs = pr > max_pairs ? pr : max_pairs;
+            if (kRowGroups<BG>.barrier_after[r]) {
+                ++g;
+                first[g] = r + 1;
+                max_rows = first[g] - first[g - 1] > max_rows ? first[g] - first[g - 1] : max_rows;
+            }
+        }
+        groups = g;
+    }
+};
+template <int BG> inline constexpr GroupPlan<BG> kGroupPlan{};
+
+// Edge j of row r reads a column that no row of the previous barrier group
+// writes (groups after the first of an iteration): its belief is final once
+// that group's barrier is reached, so it may be loaded before the barrier.
+template <int BG>
+__host__ __device__ constexpr bool edge_early(int r, int j) {
+    const int g = kGroupPlan<BG>.group_of[r];
+    if (g == 0) return false;
+    const int c = Graph<BG>::col(Graph<BG>::start(r) + j);
+    for (int x = kGroupPlan<BG>.first[g - 1]; x < kGroupPlan<BG>.first[g]; ++x)
+        for (int f = Graph<BG>::start(x); f < Graph<BG>::start(x + 1); ++f)
+            if (Graph<BG>::col(f) == c) return false;
+    return true;
+}
+// Late (not early) edges of pair i of row r: 0, 1 or 2.
+template <int BG>
+__host__ __device__ constexpr int pair_late(int r, int i) {
+    const int deg = Graph<BG>::start(r + 1) - Graph<BG>::start(r);
+    return (edge_early<BG>(r, 2 * i) ? 0 : 1) + (2 * i + 1 < deg && !edge_early<BG>(r, 2 * i + 1) ? 1 : 0);
+}
+// k-th pair of row r in first-pass order: pairs with fewer late edges first.
+template <int BG>
+__host__ __device__ constexpr int pair_order(int r, int k) {
+    const int pairs = (Graph<BG>::start(r + 1) - Graph<BG>::start(r) + 1) / 2;
+    for (int late = 0; late <= 2; ++late)
+        for (int i = 0; i < pairs; ++i)
+            if (pair_late<BG>(r, i) == late && k-- == 0) return i;
+    return -1;
+}
+
 template <int PAIRS>
 struct PackedRowState {
     uint32_t oa[PAIRS], ob[PAIRS];   // byte offsets of the pair's two variables
+    uint32_t qa[PAIRS], qb[PAIRS];   // the pair's two beliefs as loaded (uint16)
     uint32_t rel[PAIRS];             // extrinsic inputs, int16 halves
     uint32_t mag[PAIRS];             // |rel| capped at 32767
     uint32_t m1, m2;                 // per half: smallest and second smallest |rel|
     uint32_t px;
 };
+
+// Offsets of pair I (edges E, E + 1) of a row.
+template <int BG, int ZC, int E, int I, int DEG, int PAIRS>
+__device__ __forceinline__ void packed_pair_addr(uint32_t zx2, uint32_t zz, PackedRowState<PAIRS> &rs) {
+    using F = FixedCode<BG, ZC>;
+    constexpr bool has_b = 2 * I + 1 < DEG;
+    constexpr uint32_t sha = F::shift_bytes(E);
+    if constexpr (has_b && sha != 0 && F::shift_bytes(E + 1) != 0) {
+        // both offsets at once: zz holds 2*zx in each half, so t holds
+        // 2*zx + shift bytes per half (< 2 * 2Zc: no carry), u the same less
+        // 2Zc bytes per half, and each offset is the smaller of its halves
+        constexpr uint32_t shb = F::shift_bytes(E + 1);
+        constexpr uint32_t wrap = (uint32_t)(uint16_t)(-2 * ZC) * 0x10001u;
+        const uint32_t t = zz + (sha | (shb << 16));
+        const uint32_t u = pk_addw(t, wrap);
+        rs.oa[I] = min(t & 0xFFFFu, u & 0xFFFFu);
+        rs.ob[I] = min(t >> 16, u >> 16);
+    } else {
+        // per-edge offsets; a zero shift needs none (running lanes have zx < Zc)
+        rs.oa[I] = sha == 0 ? zx2 : min(zx2 + sha, zx2 + sha - 2u * (uint32_t)ZC);
+        if constexpr (has_b) {
+            constexpr uint32_t shb = F::shift_bytes(E + 1);
+            rs.ob[I] = shb == 0 ? zx2 : min(zx2 + shb, zx2 + shb - 2u * (uint32_t)ZC);
+        }
+    }
+}
+
+// Loads of pair I's edges whose early flag equals kEarly.
+template <int BG, int ZC, int R, int E, int I, int DEG, int PAIRS, bool kEarly>
+__device__ __forceinline__ void packed_pair_load(const lds_char *__restrict__ lqb,
+                                                 PackedRowState<PAIRS> &rs) {
+    using F = FixedCode<BG, ZC>;
+    if constexpr (edge_early<BG>(R, 2 * I) == kEarly)
+        rs.qa[I] = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + F::col_bytes(E) + rs.oa[I]);
+    if constexpr (2 * I + 1 < DEG) {
+        if constexpr (edge_early<BG>(R, 2 * I + 1) == kEarly)
+            rs.qb[I] = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]);
+    }
+}
 
 // The first pass keeps the two smallest magnitudes per half; no edge index is
 // needed.  An edge's outgoing magnitude is the scaled minimum over the other
@@ -801,53 +902,21 @@ struct PackedRowState {
 // non-decreasing with f(m2) - f(m1) <= m2 - m1, so both cases are
 //     max((f(m2) + m1) - |rel|, f(m1)),
 // whose exact value lies in [-32767, 32767]: int16 wrap arithmetic is exact.
-template <int BG, int ZC, int E, int I, int DEG, int PAIRS>
-__device__ __forceinline__ void packed_pair_in(const lds_char *__restrict__ lqb,
-                                               const uint32_t *msg, int w, uint32_t zx2,
-                                               uint32_t zz, PackedRowState<PAIRS> &rs) {
-    using F = FixedCode<BG, ZC>;
+// K is the pair's place in the first pass (min and max commute).
+template <int K, int I, int DEG, int PAIRS>
+__device__ __forceinline__ void packed_pair_in(const uint32_t *msg, int w, PackedRowState<PAIRS> &rs) {
     constexpr bool has_b = 2 * I + 1 < DEG;
-    constexpr uint32_t sha = F::shift_bytes(E), cba = F::col_bytes(E);
-    uint32_t q;
-    if constexpr (has_b && sha != 0 && F::shift_bytes(E + 1) != 0) {
-        // both offsets at once: zz holds 2*zx in each half, so t holds
-        // 2*zx + shift bytes per half (< 2 * 2Zc: no carry), u the same less
-        // 2Zc bytes per half, and each offset is the smaller of its halves
-        constexpr uint32_t shb = F::shift_bytes(E + 1), cbb = F::col_bytes(E + 1);
-        constexpr uint32_t wrap = (uint32_t)(uint16_t)(-2 * ZC) * 0x10001u;
-        const uint32_t t = zz + (sha | (shb << 16));
-        const uint32_t u = pk_addw(t, wrap);
-        const uint32_t oa = min(t & 0xFFFFu, u & 0xFFFFu), ob = min(t >> 16, u >> 16);
-        rs.oa[I] = oa;
-        rs.ob[I] = ob;
-        q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
-        q |= (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cbb + ob) << 16;
-    } else {
-        // a zero shift needs no arithmetic (zx2 < 2Zc): per-edge offsets
-        const uint32_t ca = zx2 + sha;
-        const uint32_t oa = min(ca, ca - 2u * (uint32_t)ZC);
-        rs.oa[I] = oa;
-        if constexpr (has_b) {
-            constexpr uint32_t shb = F::shift_bytes(E + 1), cbb = F::col_bytes(E + 1);
-            const uint32_t cb = zx2 + shb;
-            const uint32_t ob = min(cb, cb - 2u * (uint32_t)ZC);
-            rs.ob[I] = ob;
-            q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
-            q |= (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cbb + ob) << 16;
-        } else {
-            q = (uint32_t)(uint16_t)*(const lds_i16 *)(lqb + cba + oa);
-        }
-    }
+    const uint32_t q = has_b ? pk_join(rs.qa[I], rs.qb[I]) : rs.qa[I];
     const uint32_t rel = pk_subw(q, msg[w]);
     rs.rel[I] = rel;
-    rs.px ^= rel;
+    rs.px = K == 0 ? rel : (rs.px ^ rel);
     uint32_t a = pk_abs_sat(rel);
     // an odd last edge: the unused high half is at least every magnitude
     if constexpr (!has_b) a |= 0x7FFF0000u;
     rs.mag[I] = a;
-    if constexpr (I == 0) {
+    if constexpr (K == 0) {
         rs.m1 = a;
-    } else if constexpr (I == 1) {
+    } else if constexpr (K == 1) {
         rs.m2 = pk_max(rs.m1, a);
         rs.m1 = pk_min(rs.m1, a);
     } else {
@@ -876,21 +945,29 @@ __device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint
         *(lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]) = (int16_t)(v >> 16);
 }
 
-// kSimple: DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps
-// every minimum within [0, 32767]).
-template <int BG, int ZC, int R, bool kSimple, int N, int... I>
-__device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
-                                           uint32_t zx2, const DecParams &p, int alpha, int beta,
-                                           std::integer_sequence<int, I...>) {
+// Before row R's barrier: every offset of the row and its early loads.
+template <int BG, int ZC, int R, int PAIRS, int... I>
+__device__ __forceinline__ void packed_pre(const lds_char *__restrict__ lqb, uint32_t zx2, uint32_t zz,
+                                           PackedRowState<PAIRS> &rs, std::integer_sequence<int, I...>) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
-    constexpr int pairs = (int)sizeof...(I);
+    (packed_pair_addr<BG, ZC, e0 + 2 * I, I, deg, PAIRS>(zx2, zz, rs), ...);
+    (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, true>(lqb, rs), ...);
+}
+
+// After the barrier: the late loads, then the row.  kSimple:
+// DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps every
+// minimum within [0, 32767]).
+template <int BG, int ZC, int R, bool kSimple, int PAIRS, int N, int... I>
+__device__ __forceinline__ void packed_main(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                            PackedRowState<PAIRS> &rs, const DecParams &p,
+                                            int alpha, int beta, std::integer_sequence<int, I...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
     constexpr int w0 = packed_word0<BG>(R);
-    static_assert(pairs >= 2, "packed rows need degree >= 3");
-    PackedRowState<pairs> rs;
-    rs.px = 0;
-    const uint32_t zz = zx2 | (zx2 << 16);
-    (packed_pair_in<BG, ZC, e0 + 2 * I, I, deg, pairs>(lqb, msg, w0 + I, zx2, zz, rs), ...);
+    static_assert(sizeof...(I) >= 2, "packed rows need degree >= 3");
+    (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, false>(lqb, rs), ...);
+    (packed_pair_in<I, pair_order<BG>(R, I), deg, PAIRS>(msg, w0 + pair_order<BG>(R, I), rs), ...);
     // merge the halves: m1 = min of the four, m2 = second smallest
     const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
     const uint32_t m2l = rs.m2 & 0xFFFFu, m2h = rs.m2 >> 16;
@@ -912,8 +989,60 @@ __device__ __forceinline__ void packed_row(lds_char *__restrict__ lqb, uint32_t 
     const uint32_t c = (uint32_t)(n2 + m1) & 0xFFFFu;
     const uint32_t cp = c | (c << 16);
     const uint32_t lo = pk_make(p.min_llr, p.min_llr), hi = pk_make(p.max_llr, p.max_llr);
-    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, pairs, kSimple>(lqb, msg, w0 + I, rs, cp, n1p, pm,
+    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, PAIRS, kSimple>(lqb, msg, w0 + I, rs, cp, n1p, pm,
                                                                  lo, hi), ...);
+}
+
+template <int BG>
+using GroupRowState = PackedRowState<kGroupPlan<BG>.max_pairs>;
+
+template <int BG, int ZC, int R, int REnd>
+__device__ __forceinline__ void group_pre(const lds_char *lqb, uint32_t zx2, uint32_t zz,
+                                          GroupRowState<BG> *rows) {
+    if constexpr (R < REnd) {
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        packed_pre<BG, ZC, R>(lqb, zx2, zz, *rows, std::make_integer_sequence<int, (deg + 1) / 2>{});
+        group_pre<BG, ZC, R + 1, REnd>(lqb, zx2, zz, rows + 1);
+    }
+}
+
+template <int BG, int ZC, int R, int REnd, bool kTable, bool kSimple, int N>
+__device__ __forceinline__ void group_main(lds_char *lqb, uint32_t (&msg)[N], const DecParams &p,
+                                           const uint32_t *ab, int alpha0, GroupRowState<BG> *rows) {
+    if constexpr (R < REnd) {
+        int alpha = alpha0, beta = p.beta;
+        if constexpr (kTable) {
+            const uint32_t w = ab[R];
+            alpha = (int)(int16_t)(w & 0xFFFFu);
+            beta = (int)(int16_t)(w >> 16);
+        }
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        packed_main<BG, ZC, R, kSimple>(lqb, msg, *rows, p, alpha, beta,
+                                        std::make_integer_sequence<int, (deg + 1) / 2>{});
+        group_main<BG, ZC, R + 1, REnd, kTable, kSimple>(lqb, msg, p, ab, alpha0, rows + 1);
+    }
+}
+
+// Packed rows of one iteration, group by group.  rows holds group G's row
+// states (offsets and early loads made before the barrier that opens G);
+// before the barrier that closes G, the next group's are made the same way.
+template <int BG, int ZC, int G, bool kTable, bool kSimple, int N>
+__device__ __forceinline__ void packed_groups(lds_char *lqb, uint32_t (&msg)[N], uint32_t zx2,
+                                              uint32_t zz, bool run, const DecParams &p,
+                                              const uint32_t *ab, int alpha0,
+                                              GroupRowState<BG> *rows) {
+    constexpr auto &gp = kGroupPlan<BG>;
+    if (run)
+        group_main<BG, ZC, gp.first[G], gp.first[G + 1], kTable, kSimple>(lqb, msg, p, ab, alpha0,
+                                                                          rows);
+    if constexpr (G + 1 < gp.groups) {
+        GroupRowState<BG> next[gp.max_rows];
+        if (run) group_pre<BG, ZC, gp.first[G + 1], gp.first[G + 2]>(lqb, zx2, zz, next);
+        __syncthreads();
+        packed_groups<BG, ZC, G + 1, kTable, kSimple>(lqb, msg, zx2, zz, run, p, ab, alpha0, next);
+    } else {
+        __syncthreads();
+    }
 }
 
 template <int BG, int ZC, int R, bool kGeneral, bool kTable, bool kSimple, int N>
@@ -927,14 +1056,9 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
             beta = (int)(int16_t)(w >> 16);
         }
         constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
-        if (run) {
-            if constexpr (kGeneral)
-                fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
-                                               std::make_integer_sequence<int, deg>{});
-            else
-                packed_row<BG, ZC, R, kSimple>(lqb, msg, zx2, p, alpha, beta,
-                                               std::make_integer_sequence<int, (deg + 1) / 2>{});
-        }
+        if (run)
+            fixed_row<BG, ZC, R, kGeneral>(lqb, msg, zx2, p, alpha, beta,
+                                           std::make_integer_sequence<int, deg>{});
         if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         fixed_rows<BG, ZC, R + 1, kGeneral, kTable, kSimple>(lqb, msg, zx2, run, p, ab, alpha0);
     }
@@ -963,8 +1087,18 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
             NRLDPC_OPAQUE_LANE(zx2);
             if constexpr (!kTable) NRLDPC_OPAQUE_LANE(alpha0);
         }
-        fixed_rows<BG, ZC, 0, kGeneral, kTable, kSimple>(lqb, msg, zx2, zx < ZC, p,
-                                                         scale_row<BG, kTable>(p, it), alpha0);
+        const bool run = zx < ZC;
+        if constexpr (kGeneral) {
+            fixed_rows<BG, ZC, 0, kGeneral, kTable, kSimple>(lqb, msg, zx2, run, p,
+                                                             scale_row<BG, kTable>(p, it), alpha0);
+        } else {
+            const uint32_t zz = zx2 | (zx2 << 16);
+            GroupRowState<BG> rows[kGroupPlan<BG>.max_rows];
+            if (run)
+                group_pre<BG, ZC, 0, kGroupPlan<BG>.first[1]>(lqb, zx2, zz, rows);
+            packed_groups<BG, ZC, 0, kTable, kSimple>(lqb, msg, zx2, zz, run, p,
+                                                      scale_row<BG, kTable>(p, it), alpha0, rows);
+        }
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
