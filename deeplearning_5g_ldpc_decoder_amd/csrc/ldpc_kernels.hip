@@ -955,7 +955,17 @@ __device__ __forceinline__ void packed_pre(const lds_char *__restrict__ lqb, uin
     (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, true>(lqb, rs), ...);
 }
 
-// After the barrier: the late loads, then the row.  kSimple:
+// After the barrier: the late loads of row R (for every row of the group
+// before any row's update: the rows of a group touch disjoint columns).
+template <int BG, int ZC, int R, int PAIRS, int... I>
+__device__ __forceinline__ void packed_late(const lds_char *__restrict__ lqb, PackedRowState<PAIRS> &rs,
+                                            std::integer_sequence<int, I...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, false>(lqb, rs), ...);
+}
+
+// Then the row itself.  kSimple:
 // DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps every
 // minimum within [0, 32767]).
 template <int BG, int ZC, int R, bool kSimple, int PAIRS, int N, int... I>
@@ -966,7 +976,6 @@ __device__ __forceinline__ void packed_main(lds_char *__restrict__ lqb, uint32_t
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     constexpr int w0 = packed_word0<BG>(R);
     static_assert(sizeof...(I) >= 2, "packed rows need degree >= 3");
-    (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, false>(lqb, rs), ...);
     (packed_pair_in<I, pair_order<BG>(R, I), deg, PAIRS>(msg, w0 + pair_order<BG>(R, I), rs), ...);
     // merge the halves: m1 = min of the four, m2 = second smallest
     const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
@@ -1006,6 +1015,15 @@ __device__ __forceinline__ void group_pre(const lds_char *lqb, uint32_t zx2, uin
     }
 }
 
+template <int BG, int ZC, int R, int REnd>
+__device__ __forceinline__ void group_late(const lds_char *lqb, GroupRowState<BG> *rows) {
+    if constexpr (R < REnd) {
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        packed_late<BG, ZC, R>(lqb, *rows, std::make_integer_sequence<int, (deg + 1) / 2>{});
+        group_late<BG, ZC, R + 1, REnd>(lqb, rows + 1);
+    }
+}
+
 template <int BG, int ZC, int R, int REnd, bool kTable, bool kSimple, int N>
 __device__ __forceinline__ void group_main(lds_char *lqb, uint32_t (&msg)[N], const DecParams &p,
                                            const uint32_t *ab, int alpha0, GroupRowState<BG> *rows) {
@@ -1032,9 +1050,11 @@ __device__ __forceinline__ void packed_groups(lds_char *lqb, uint32_t (&msg)[N],
                                               const uint32_t *ab, int alpha0,
                                               GroupRowState<BG> *rows) {
     constexpr auto &gp = kGroupPlan<BG>;
-    if (run)
+    if (run) {
+        group_late<BG, ZC, gp.first[G], gp.first[G + 1]>(lqb, rows);
         group_main<BG, ZC, gp.first[G], gp.first[G + 1], kTable, kSimple>(lqb, msg, p, ab, alpha0,
                                                                           rows);
+    }
     if constexpr (G + 1 < gp.groups) {
         GroupRowState<BG> next[gp.max_rows];
         if (run) group_pre<BG, ZC, gp.first[G + 1], gp.first[G + 2]>(lqb, zx2, zz, next);
