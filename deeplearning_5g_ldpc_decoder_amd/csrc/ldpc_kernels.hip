@@ -51,8 +51,10 @@
 // iteration loop, which would hold hundreds of registers across it.
 #ifdef NRLDPC_CPU_EMU
 #define NRLDPC_OPAQUE_LANE(x) (void)(x)
+#define NRLDPC_OPAQUE_UNIFORM(x) (void)(x)
 #else
 #define NRLDPC_OPAQUE_LANE(x) asm volatile("" : "+v"(x))
+#define NRLDPC_OPAQUE_UNIFORM(x) asm volatile("" : "+s"(x))
 #endif
 
 namespace nrldpc {
@@ -504,26 +506,6 @@ __device__ __forceinline__ void init_state(StateI16<BG> &st) {
     }
 }
 
-template <int BG, bool kGeneral, bool kTable>
-__device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
-                       const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
-    StateI16<BG> st;
-    init_state<BG>(st);
-    const int zx = threadIdx.x;
-    int it = 0;
-    *status = 1;
-    while (it < p.max_it) {
-        rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, d.Zc, zx < d.Zc, p,
-                                          scale_row<BG, kTable>(p, it));
-        ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
-            *status = 0;
-            break;
-        }
-    }
-    return it;
-}
-
 // ---------------------------------------------------------------------------
 // Lifting-size specialized int16 decode: every codeblock of the launch has
 // base graph BG and lifting size ZC, so each edge's circulant shift and
@@ -925,23 +907,70 @@ __device__ __forceinline__ void packed_pair_in(const uint32_t *msg, int w, Packe
     }
 }
 
-template <int BG, int ZC, int E, int I, int DEG, int PAIRS, bool kSimple, int N>
-__device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
-                                                int w, const PackedRowState<PAIRS> &rs,
-                                                uint32_t cp, uint32_t n1p, uint32_t pm,
-                                                uint32_t lo, uint32_t hi) {
-    using F = FixedCode<BG, ZC>;
+// A row's scaled minima and parity, packed for the second pass.
+struct PackedScale {
+    uint32_t cp;    // f(m2) + m1 in both halves
+    uint32_t n1p;   // f(m1) in both halves
+    uint32_t pm;    // row parity in the sign bit of both halves
+    uint32_t lo, hi;
+};
+
+// kSimple: DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps
+// every minimum within [0, 32767]); otherwise the general fast-row scale.
+template <bool kSimple, int PAIRS>
+__device__ __forceinline__ PackedScale packed_finalize(const PackedRowState<PAIRS> &rs,
+                                                       const DecParams &p, int alpha, int beta) {
+    // merge the halves: m1 = min of the four, m2 = second smallest
+    const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
+    const uint32_t m2l = rs.m2 & 0xFFFFu, m2h = rs.m2 >> 16;
+    const int m1 = (int)min(m1l, m1h);
+    const int m2 = (int)min(max(m1l, m1h), min(m2l, m2h));
+    int n1, n2;
+    if constexpr (kSimple) {
+        n1 = scale_simple(m1, alpha);
+        n2 = scale_simple(m2, alpha);
+    } else {
+        n1 = scale_i16(min(m1, p.max_llr), alpha, beta);
+        n2 = scale_i16(min(m2, p.max_llr), alpha, beta);
+    }
+    PackedScale f;
+    // odd number of negative inputs: every message flips sign.  px ^ rot16(px)
+    // holds that parity in the sign bit of both halves (the other bits are
+    // never read: only the sign of rel ^ pm is)
+    f.pm = rs.px ^ ((rs.px >> 16) | (rs.px << 16));
+    f.n1p = (uint32_t)n1 * 0x10001u;   // n1 in [0, 32767]
+    const uint32_t c = (uint32_t)(n2 + m1) & 0xFFFFu;
+    f.cp = c | (c << 16);
+    f.lo = pk_make(p.min_llr, p.min_llr);
+    f.hi = pk_make(p.max_llr, p.max_llr);
+    return f;
+}
+
+// Second pass of pair I: its new messages (stored) and its two new beliefs
+// (returned as int16 halves).
+template <int I, int DEG, int PAIRS, bool kSimple, int N>
+__device__ __forceinline__ uint32_t packed_pair_update(uint32_t (&msg)[N], int w,
+                                                       const PackedRowState<PAIRS> &rs,
+                                                       const PackedScale &f) {
     constexpr bool has_b = 2 * I + 1 < DEG;
-    uint32_t nv = pk_max(pk_subw(cp, rs.mag[I]), n1p);
+    uint32_t nv = pk_max(pk_subw(f.cp, rs.mag[I]), f.n1p);
     if constexpr (!has_b) nv &= 0xFFFFu;   // keeps the unused high half at 0
     const uint32_t rel = rs.rel[I];
     // message = magnitude * (+-1): the sign of rel, flipped for odd row parity
-    const uint32_t m = pk_mullo(nv, pk_sgn(rel ^ pm));
+    const uint32_t m = pk_mullo(nv, pk_sgn(rel ^ f.pm));
     msg[w] = m;
     // kSimple has max_llr = 32767: the saturated add already clips above
-    const uint32_t v = kSimple ? pk_update_lo(rel, m, lo) : pk_update(rel, m, lo, hi);
+    return kSimple ? pk_update_lo(rel, m, f.lo) : pk_update(rel, m, f.lo, f.hi);
+}
+
+template <int BG, int ZC, int E, int I, int DEG, int PAIRS, bool kSimple, int N>
+__device__ __forceinline__ void packed_pair_out(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
+                                                int w, const PackedRowState<PAIRS> &rs,
+                                                const PackedScale &f) {
+    using F = FixedCode<BG, ZC>;
+    const uint32_t v = packed_pair_update<I, DEG, PAIRS, kSimple>(msg, w, rs, f);
     *(lds_i16 *)(lqb + F::col_bytes(E) + rs.oa[I]) = (int16_t)(v & 0xFFFFu);
-    if constexpr (has_b)
+    if constexpr (2 * I + 1 < DEG)
         *(lds_i16 *)(lqb + F::col_bytes(E + 1) + rs.ob[I]) = (int16_t)(v >> 16);
 }
 
@@ -965,9 +994,7 @@ __device__ __forceinline__ void packed_late(const lds_char *__restrict__ lqb, Pa
     (packed_pair_load<BG, ZC, R, e0 + 2 * I, I, deg, PAIRS, false>(lqb, rs), ...);
 }
 
-// Then the row itself.  kSimple:
-// DecParams::simple (beta 0, max_llr 32767; the capped |rel| keeps every
-// minimum within [0, 32767]).
+// Then the row itself (kSimple as in packed_finalize).
 template <int BG, int ZC, int R, bool kSimple, int PAIRS, int N, int... I>
 __device__ __forceinline__ void packed_main(lds_char *__restrict__ lqb, uint32_t (&msg)[N],
                                             PackedRowState<PAIRS> &rs, const DecParams &p,
@@ -977,29 +1004,8 @@ __device__ __forceinline__ void packed_main(lds_char *__restrict__ lqb, uint32_t
     constexpr int w0 = packed_word0<BG>(R);
     static_assert(sizeof...(I) >= 2, "packed rows need degree >= 3");
     (packed_pair_in<I, pair_order<BG>(R, I), deg, PAIRS>(msg, w0 + pair_order<BG>(R, I), rs), ...);
-    // merge the halves: m1 = min of the four, m2 = second smallest
-    const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
-    const uint32_t m2l = rs.m2 & 0xFFFFu, m2h = rs.m2 >> 16;
-    const int m1 = (int)min(m1l, m1h);
-    const int m2 = (int)min(max(m1l, m1h), min(m2l, m2h));
-    int n1, n2;
-    if constexpr (kSimple) {
-        n1 = scale_simple(m1, alpha);
-        n2 = scale_simple(m2, alpha);
-    } else {
-        n1 = scale_i16(min(m1, p.max_llr), alpha, beta);
-        n2 = scale_i16(min(m2, p.max_llr), alpha, beta);
-    }
-    // odd number of negative inputs: every message flips sign.  px ^ rot16(px)
-    // holds that parity in the sign bit of both halves (the other bits are
-    // never read: only the sign of rel ^ pm is)
-    const uint32_t pm = rs.px ^ ((rs.px >> 16) | (rs.px << 16));
-    const uint32_t n1p = (uint32_t)n1 * 0x10001u;   // n1 in [0, 32767]
-    const uint32_t c = (uint32_t)(n2 + m1) & 0xFFFFu;
-    const uint32_t cp = c | (c << 16);
-    const uint32_t lo = pk_make(p.min_llr, p.min_llr), hi = pk_make(p.max_llr, p.max_llr);
-    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, PAIRS, kSimple>(lqb, msg, w0 + I, rs, cp, n1p, pm,
-                                                                 lo, hi), ...);
+    const PackedScale f = packed_finalize<kSimple>(rs, p, alpha, beta);
+    (packed_pair_out<BG, ZC, e0 + 2 * I, I, deg, PAIRS, kSimple>(lqb, msg, w0 + I, rs, f), ...);
 }
 
 template <int BG>
@@ -1119,6 +1125,143 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
             packed_groups<BG, ZC, 0, kTable, kSimple>(lqb, msg, zx2, zz, run, p,
                                                       scale_row<BG, kTable>(p, it), alpha0, rows);
         }
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Packed rows with run-time shifts: the generic int16 kernel (one codeblock
+// of any lifting size per workgroup) and the packed tiles, fast rows only.
+// The arithmetic is the specialized kernel's (packed_pair_in,
+// packed_finalize with the general scale, packed_pair_update); offsets come
+// from the edge table, etab[e] = column * Zc | shift << 16 (one scalar load
+// per edge).  lqb is the lane's codeblock beliefs in LDS, zx2 = 2 * zx,
+// zz = zx2 in both halves, z2 = 2 * Zc and wrap = -2 * Zc in both int16
+// halves.
+// ---------------------------------------------------------------------------
+struct GenLane {
+    char *lqb;
+    uint32_t zx2, zz, z2, wrap;
+};
+
+template <int E, int I, int DEG, int PAIRS>
+__device__ __forceinline__ void gpair_load(const GenLane &g, const uint32_t *__restrict__ etab,
+                                           PackedRowState<PAIRS> &rs) {
+    const uint32_t ta = etab[E];
+    const uint32_t sha = (ta >> 16) * 2u, cola = (ta & 0xFFFFu) * 2u;
+    if constexpr (2 * I + 1 < DEG) {
+        // both offsets at once (see packed_pair_addr): halves below 4 * Zc
+        const uint32_t tb = etab[E + 1];
+        const uint32_t shb = (tb >> 16) * 2u, colb = (tb & 0xFFFFu) * 2u;
+        const uint32_t t = g.zz + (sha | (shb << 16));
+        const uint32_t u = pk_addw(t, g.wrap);
+        rs.oa[I] = cola + min(t & 0xFFFFu, u & 0xFFFFu);
+        rs.ob[I] = colb + min(t >> 16, u >> 16);
+        rs.qa[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.oa[I]);
+        rs.qb[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.ob[I]);
+    } else {
+        const uint32_t c = g.zx2 + sha;
+        rs.oa[I] = cola + min(c, c - g.z2);
+        rs.qa[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.oa[I]);
+    }
+}
+
+template <int I, int DEG, int PAIRS>
+__device__ __forceinline__ void gpair_store(const GenLane &g, const PackedRowState<PAIRS> &rs,
+                                            uint32_t v) {
+    *(int16_t *)(g.lqb + rs.oa[I]) = (int16_t)(v & 0xFFFFu);
+    if constexpr (2 * I + 1 < DEG) *(int16_t *)(g.lqb + rs.ob[I]) = (int16_t)(v >> 16);
+}
+
+template <int BG, int R, int N, int... I>
+__device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N],
+                                            const uint32_t *etab, const DecParams &p, int alpha,
+                                            int beta, std::integer_sequence<int, I...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    constexpr int w0 = packed_word0<BG>(R);
+    constexpr int pairs = (int)sizeof...(I);
+    static_assert(pairs >= 2, "packed rows need degree >= 3");
+    PackedRowState<pairs> rs;
+    (gpair_load<e0 + 2 * I, I, deg, pairs>(g, etab, rs), ...);
+    (packed_pair_in<I, I, deg, pairs>(msg, w0 + I, rs), ...);
+    const PackedScale f = packed_finalize<false>(rs, p, alpha, beta);
+    (gpair_store<I, deg, pairs>(g, rs, packed_pair_update<I, deg, pairs, false>(msg, w0 + I, rs, f)),
+     ...);
+}
+
+template <int BG, int R, bool kTable, int N>
+__device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], const uint32_t *etab,
+                                       bool run, const DecParams &p, const uint32_t *ab) {
+    if constexpr (R < Graph<BG>::kRows) {
+        int alpha = p.alpha, beta = p.beta;
+        if constexpr (kTable) {
+            const uint32_t w = ab[R];
+            alpha = (int)(int16_t)(w & 0xFFFFu);
+            beta = (int)(int16_t)(w >> 16);
+        }
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        if (run)
+            gpacked_row<BG, R>(g, msg, etab, p, alpha, beta,
+                               std::make_integer_sequence<int, (deg + 1) / 2>{});
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
+        gprows<BG, R + 1, kTable>(g, msg, etab, run, p, ab);
+    }
+}
+
+// Lane setup of the run-time packed rows.
+__device__ __forceinline__ GenLane gen_lane(int16_t *lq, int zx, int Zc) {
+    GenLane g;
+    g.lqb = reinterpret_cast<char *>(lq);
+    g.zx2 = 2u * (uint32_t)zx;
+    g.zz = g.zx2 | (g.zx2 << 16);
+    g.z2 = 2u * (uint32_t)Zc;
+    g.wrap = (uint32_t)(uint16_t)(-2 * Zc) * 0x10001u;
+    return g;
+}
+
+// One iteration's rows, with the lane's offsets and the edge table made
+// opaque first: otherwise their loop-invariant offsets would be hoisted out
+// of the iteration loop into hundreds of registers.
+template <int BG, bool kTable, int N>
+__device__ __forceinline__ void gprows_iteration(GenLane g, uint32_t (&msg)[N], const uint32_t *etab,
+                                                 bool run, const DecParams &p, const uint32_t *ab) {
+    NRLDPC_OPAQUE_LANE(g.zx2);
+    NRLDPC_OPAQUE_LANE(g.zz);
+    NRLDPC_OPAQUE_UNIFORM(etab);
+    gprows<BG, 0, kTable>(g, msg, etab, run, p, ab);
+}
+
+template <int BG>
+constexpr int kPackedWords = packed_word0<BG>(Graph<BG>::kRows);
+
+template <int BG, bool kGeneral, bool kTable>
+__device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
+                       const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
+    const int zx = threadIdx.x;
+    int it = 0;
+    *status = 1;
+    // exact rows: compressed messages; fast rows: row-paired packed messages
+    StateI16<BG> st;
+    uint32_t msg[kGeneral ? 1 : kPackedWords<BG>];
+    if constexpr (kGeneral) {
+        init_state<BG>(st);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kPackedWords<BG>; ++i) msg[i] = 0u;
+    }
+    const GenLane g = gen_lane(lq, zx, d.Zc);
+    while (it < p.max_it) {
+        if constexpr (kGeneral)
+            rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, d.Zc, zx < d.Zc, p,
+                                              scale_row<BG, kTable>(p, it));
+        else
+            gprows_iteration<BG, kTable>(g, msg, etab, zx < d.Zc, p, scale_row<BG, kTable>(p, it));
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -1449,13 +1592,23 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
                              int *status, int *iters) {
     StateI16<BG> st;
-    init_state<BG>(st);
+    uint32_t msg[kGeneral ? 1 : kPackedWords<BG>];
+    if constexpr (kGeneral) {
+        init_state<BG>(st);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kPackedWords<BG>; ++i) msg[i] = 0u;
+    }
+    const GenLane g = gen_lane(lq, zx, Zc);
     bool done = !member;
     bool tile_active = __syncthreads_or(member);
     const Crc &c = crc.spec;
     for (int it = 0; it < p.max_it && tile_active; ++it) {
-        rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, Zc, !done, p,
-                                          scale_row<BG, kTable>(p, it));
+        if constexpr (kGeneral)
+            rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, Zc, !done, p,
+                                              scale_row<BG, kTable>(p, it));
+        else
+            gprows_iteration<BG, kTable>(g, msg, etab, !done, p, scale_row<BG, kTable>(p, it));
         const int b = it & 1;
         if (!done) {
             uint32_t r = 0;
