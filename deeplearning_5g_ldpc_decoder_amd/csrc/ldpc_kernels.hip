@@ -1149,8 +1149,16 @@ struct GenLane {
     uint32_t zx2, zz, z2, wrap;
 };
 
+// The edge table through the constant address space: uniform indices become
+// scalar loads and the column / shift decoding scalar ops.
+#ifdef NRLDPC_CPU_EMU
+typedef const uint32_t *etab_cptr;
+#else
+typedef const __attribute__((address_space(4))) uint32_t *etab_cptr;
+#endif
+
 template <int E, int I, int DEG, int PAIRS>
-__device__ __forceinline__ void gpair_load(const GenLane &g, const uint32_t *__restrict__ etab,
+__device__ __forceinline__ void gpair_load(const GenLane &g, etab_cptr etab,
                                            PackedRowState<PAIRS> &rs) {
     const uint32_t ta = etab[E];
     const uint32_t sha = (ta >> 16) * 2u, cola = (ta & 0xFFFFu) * 2u;
@@ -1180,7 +1188,7 @@ __device__ __forceinline__ void gpair_store(const GenLane &g, const PackedRowSta
 
 template <int BG, int R, int N, int... I>
 __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N],
-                                            const uint32_t *etab, const DecParams &p, int alpha,
+                                            etab_cptr etab, const DecParams &p, int alpha,
                                             int beta, std::integer_sequence<int, I...>) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
@@ -1196,7 +1204,7 @@ __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N]
 }
 
 template <int BG, int R, bool kTable, int N>
-__device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], const uint32_t *etab,
+__device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], etab_cptr etab,
                                        bool run, const DecParams &p, const uint32_t *ab) {
     if constexpr (R < Graph<BG>::kRows) {
         int alpha = p.alpha, beta = p.beta;
@@ -1233,8 +1241,9 @@ __device__ __forceinline__ void gprows_iteration(GenLane g, uint32_t (&msg)[N], 
                                                  bool run, const DecParams &p, const uint32_t *ab) {
     NRLDPC_OPAQUE_LANE(g.zx2);
     NRLDPC_OPAQUE_LANE(g.zz);
-    NRLDPC_OPAQUE_UNIFORM(etab);
-    gprows<BG, 0, kTable>(g, msg, etab, run, p, ab);
+    etab_cptr ce = (etab_cptr)etab;
+    NRLDPC_OPAQUE_UNIFORM(ce);
+    gprows<BG, 0, kTable>(g, msg, ce, run, p, ab);
 }
 
 template <int BG>
