@@ -26,6 +26,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_unrolled_backward",
     "ldpc_mi355_batch_create", "ldpc_mi355_batch_decode", "ldpc_mi355_batch_layout",
     "ldpc_mi355_batch_query", "ldpc_mi355_batch_wait", "ldpc_mi355_batch_destroy",
+    "ldpc_mi355_copy_spans",
     "ldpc_mi355_set_trained_tables", "ldpc_mi355_get_trained_tables",
     "ldpc_mi355_set_degree_alpha", "ldpc_mi355_get_degree_alpha",
 )
@@ -176,6 +177,8 @@ def load():
     L.ldpc_mi355_batch_wait.restype = ctypes.c_int32
     L.ldpc_mi355_batch_destroy.argtypes = [P]
     L.ldpc_mi355_batch_destroy.restype = None
+    L.ldpc_mi355_copy_spans.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32, P]
+    L.ldpc_mi355_copy_spans.restype = ctypes.c_int32
     L.ldpc_mi355_set_trained_tables.argtypes = [ctypes.c_int32, P, P, ctypes.c_int32]
     L.ldpc_mi355_set_trained_tables.restype = ctypes.c_int32
     L.ldpc_mi355_get_trained_tables.argtypes = [ctypes.c_int32, P, P, ctypes.c_int32]

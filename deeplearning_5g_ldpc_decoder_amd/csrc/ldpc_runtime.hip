@@ -666,6 +666,19 @@ extern "C" void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *b) {
     delete b;
 }
 
+// Many small asynchronous copies in one call (the served loop moves one
+// pinned span per request in and one result span out per batch).
+extern "C" int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src,
+                                         const uint64_t *bytes, int32_t n, int32_t kind,
+                                         void *stream) {
+    if (n < 0 || (n > 0 && (!dst || !src || !bytes)) || (kind != 1 && kind != 2)) return -1;
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    for (int32_t i = 0; i < n; ++i)
+        if (bytes[i] && hipMemcpyAsync(dst[i], src[i], bytes[i], k, s) != hipSuccess) return -1;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Device input synthesis (one config, n codeblocks, one launch)
 // ---------------------------------------------------------------------------

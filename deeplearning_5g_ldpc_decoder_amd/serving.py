@@ -262,12 +262,18 @@ class _Context:
             raise RuntimeError(f"ldpc_mi355_batch_create failed ({st})")
         self.stream = torch.cuda.Stream(device=device)    # default priority
         self.cap = cap_cbs
-        self.status = torch.empty(cap_cbs, dtype=torch.int32, device=device)
-        self.iters = torch.empty(cap_cbs, dtype=torch.int32, device=device)
-        self.hard = torch.empty(cap_cbs * 1056, dtype=torch.uint8, device=device)
-        self.h_status = torch.empty(cap_cbs, dtype=torch.int32).pin_memory()
-        self.h_iters = torch.empty(cap_cbs, dtype=torch.int32).pin_memory()
-        self.h_hard = torch.empty(cap_cbs * 1056, dtype=torch.uint8).pin_memory()
+        # results in one span (status, iterations, hard bytes), so that one
+        # copy brings a batch's results back
+        nbytes = 8 * cap_cbs + cap_cbs * 1056
+        self.out_dev = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        self.status = self.out_dev[:4 * cap_cbs].view(torch.int32)
+        self.iters = self.out_dev[4 * cap_cbs:8 * cap_cbs].view(torch.int32)
+        self.hard = self.out_dev[8 * cap_cbs:]
+        self.out_host = torch.empty(nbytes, dtype=torch.uint8).pin_memory()
+        h = self.out_host.numpy()
+        self.h_status = h[:4 * cap_cbs].view(np.int32)
+        self.h_iters = h[4 * cap_cbs:8 * cap_cbs].view(np.int32)
+        self.h_hard = h[8 * cap_cbs:]
         self.descs = (CbDescriptor * cap_cbs)()
         self.done_event = torch.cuda.Event()
         self.busy = False
@@ -309,41 +315,53 @@ class GpuExecutor:
     def launch(self, ctx, reqs, region):
         """Copy the batch's LLRs into its pool region, decode, copy results
         back -- all queued on the context's stream."""
-        torch = self.torch
+        L = _lib.load()
         off, need = region
-        parts = []
-        pos = 0
+        nreq = len(reqs)
         stream = ctx.stream
-        dev = self.llr_pool.tensor
-        with torch.cuda.stream(stream):
-            for r in reqs:
-                # one host->device copy per request, straight from its pinned
-                # LLRs into the batch's pool region
-                a, b = self.pool.span(r.template, r.rv_index)
-                dev[off + pos:off + pos + b - a].copy_(self.pool.pinned[a:b], non_blocking=True)
-                d = self.pool.desc[(r.template, r.rv_index)].copy()
-                d["llr_offset"] += off + pos
-                d["soft_slot"] += self.harq.slot(r.ue, r.proc, 0)
-                parts.append(d)
-                pos += b - a
-        descs = np.concatenate(parts) if len(parts) > 1 else parts[0]
+        sh = ctypes.c_void_p(stream.cuda_stream)
+        # one host->device span per request, straight from its pinned LLRs
+        # into the batch's pool region, all in one call
+        spans = np.empty((3, nreq), np.uint64)
+        parts, llr_base, slot_base = [], np.empty(nreq, np.int64), np.empty(nreq, np.int64)
+        dev0, host0 = self.llr_pool.tensor.data_ptr(), self.pool.pinned.data_ptr()
+        pos = 0
+        for i, r in enumerate(reqs):
+            a, b = self.pool.span(r.template, r.rv_index)
+            spans[0, i] = dev0 + 2 * (off + pos)
+            spans[1, i] = host0 + 2 * a
+            spans[2, i] = 2 * (b - a)
+            parts.append(self.pool.desc[(r.template, r.rv_index)])
+            llr_base[i] = off + pos
+            slot_base[i] = self.harq.slot(r.ue, r.proc, 0)
+            pos += b - a
+        assert pos <= need
+        if L.ldpc_mi355_copy_spans(spans[0].ctypes.data, spans[1].ctypes.data,
+                                   spans[2].ctypes.data, nreq, 1, sh) != 0:
+            raise RuntimeError("ldpc_mi355_copy_spans (LLRs) failed")
+        descs = np.concatenate(parts)
+        counts = [len(d) for d in parts]
+        descs["llr_offset"] += np.repeat(llr_base, counts).astype(descs["llr_offset"].dtype)
+        descs["soft_slot"] += np.repeat(slot_base, counts).astype(descs["soft_slot"].dtype)
         n = len(descs)
-        assert n <= ctx.cap and pos <= need
+        assert n <= ctx.cap
         ctx.desc_buf = descs
         bad = ctypes.c_int32(-1)
-        st = _lib.load().ldpc_mi355_batch_decode(
+        st = L.ldpc_mi355_batch_decode(
             ctx.handle, descs.ctypes.data_as(ctypes.POINTER(CbDescriptor)), n,
             ptr(self.llr_pool.tensor), ptr(self.harq.tensor), ptr(ctx.hard), ptr(ctx.status),
-            ptr(ctx.iters), None, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(bad))
+            ptr(ctx.iters), None, sh, ctypes.byref(bad))
         if st != 0:
             raise RuntimeError(f"ldpc_mi355_batch_decode failed ({st}), codeblock {bad.value}")
-        # codeblocks' hard bytes are laid out back to back in batch order
+        # codeblocks' hard bytes are laid out back to back in batch order:
+        # status, iterations and those bytes come back as one span
         hb = int(sum(int(self.pool.cb_hard[r.template].sum()) for r in reqs))
-        with torch.cuda.stream(stream):
-            ctx.h_status[:n].copy_(ctx.status[:n], non_blocking=True)
-            ctx.h_iters[:n].copy_(ctx.iters[:n], non_blocking=True)
-            ctx.h_hard[:hb].copy_(ctx.hard[:hb], non_blocking=True)
-            ctx.done_event.record(stream)
+        out = np.array([[ctx.out_host.data_ptr()], [ctx.out_dev.data_ptr()],
+                        [8 * ctx.cap + hb]], np.uint64)
+        if L.ldpc_mi355_copy_spans(out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data,
+                                   1, 2, sh) != 0:
+            raise RuntimeError("ldpc_mi355_copy_spans (results) failed")
+        ctx.done_event.record(stream)
         ctx.busy = True
         ctx.requests = reqs
         ctx.region = region
@@ -360,9 +378,7 @@ class GpuExecutor:
 
     def collect(self, ctx):
         """Per-request results of a finished context; frees it."""
-        st = ctx.h_status.numpy()
-        its = ctx.h_iters.numpy()
-        hard = ctx.h_hard.numpy()
+        st, its, hard = ctx.h_status, ctx.h_iters, ctx.h_hard
         k = 0
         h = 0
         for r in ctx.requests:

@@ -290,6 +290,15 @@ int32_t ldpc_mi355_batch_query(ldpc_mi355_batch_t *batch);
 int32_t ldpc_mi355_batch_wait(ldpc_mi355_batch_t *batch);
 void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *batch);
 
+/* n asynchronous copies on stream, span i: bytes[i] from src[i] to dst[i];
+ * kind 1 = host to device, 2 = device to host (pinned host memory for the
+ * copies to be asynchronous).  No counterpart in the reference: it serves
+ * callers that move many small spans per launch, such as a served loop
+ * copying each request's LLRs into its batch, in one call.  Returns 0, or
+ * -1 on invalid arguments or a HIP error. */
+int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src, const uint64_t *bytes,
+                              int32_t n, int32_t kind, void *stream);
+
 /* 1 when this build runs decoder algo `algo` on the GPU, else 0 (algo 0,
  * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls
  * then return -1 with a message instead of decoding). */
