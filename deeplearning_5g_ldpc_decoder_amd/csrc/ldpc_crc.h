@@ -43,6 +43,27 @@ __device__ inline uint32_t crc_mulmod(uint32_t a, uint32_t b, const Crc &c) {
     return r;
 }
 
+// crc_mulmod unrolled for a width W known at compile time, 5 ops per bit of
+// b (Horner from its top bit): r * x is r << 1 with g folded in when bit
+// W - 1 of r was set (g | x^W also clears the bit shifted out), and a is
+// added under bit i of b, both selected by sign-extended bit masks.
+template <int W>
+__device__ __forceinline__ uint32_t crc_mulmod_w(uint32_t a, uint32_t b, uint32_t poly) {
+    const uint32_t g = poly | (1u << W);
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = W - 1; i >= 0; --i) {
+        const uint32_t top = (uint32_t)((int32_t)(r << (31 - (W - 1))) >> 31);
+        r = (r << 1) ^ (g & top);
+        r ^= a & (uint32_t)((int32_t)(b << (31 - i)) >> 31);
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t crc_mulmod_fast(uint32_t a, uint32_t b, const Crc &c) {
+    return c.w == 24 ? crc_mulmod_w<24>(a, b, c.poly) : crc_mulmod_w<16>(a, b, c.poly);
+}
+
 // x^(8m) mod g(x)
 __device__ inline uint32_t crc_x8pow(int m, const Crc &c) {
     uint32_t res = 1u, base = 0x100u;
@@ -58,6 +79,25 @@ __device__ inline uint32_t wave_xor(uint32_t v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v ^= __shfl_xor(v, o, 64);
     return v;
+}
+
+// XOR of v over the wave, in every lane.  On the GPU: DPP row shifts give
+// each row of 16 lanes its XOR in its last lane, two row broadcasts carry
+// rows 0-2 into lane 63, and one readlane returns it (the wave must be
+// whole: DPP reads inactive lanes' registers too).
+__device__ inline uint32_t wave_xor_full(uint32_t v) {
+#ifdef NRLDPC_CPU_EMU
+    return wave_xor(v);
+#else
+    int x = (int)v;
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, true);   // row_bcast:15
+    x ^= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, true);   // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
+#endif
 }
 
 // Hard-decision byte from 8 consecutive beliefs (bit 7 = first belief).
@@ -113,12 +153,18 @@ __device__ inline uint32_t block_crc(const T *lq, const CrcCtx &ctx,
                                      const uint32_t *tab, uint32_t *slots) {
     const Crc &c = ctx.spec;
     uint32_t r = 0;
-    for (int b = ctx.b_lo; b < ctx.b_hi; ++b) {
-        const uint32_t byte = hard_byte(lq, b);
-        r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
+    // up to 4 bytes' beliefs loaded together, then the dependent table steps
+    for (int b0 = ctx.b_lo; b0 < ctx.b_hi; b0 += 4) {
+        uint32_t bytes[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bytes[u] = b0 + u < ctx.b_hi ? hard_byte(lq, b0 + u) : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b0 + u < ctx.b_hi)
+                r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ bytes[u]) & 0xFFu]) & c.mask;
     }
-    uint32_t part = r ? crc_mulmod(r, ctx.mult, c) : 0u;
-    part = wave_xor(part);
+    uint32_t part = crc_mulmod_fast(r, ctx.mult, c);
+    part = wave_xor_full(part);   // block sizes are whole waves
     const int wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) slots[wave] = part;
     __syncthreads();

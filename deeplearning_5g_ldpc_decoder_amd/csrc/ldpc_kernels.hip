@@ -1625,7 +1625,7 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
                 const uint32_t byte = hard_byte(lq, k);
                 r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
             }
-            if (r) atomicXor(&bank[b][cbl], crc_mulmod(r, crc.mult, c));
+            if (r) atomicXor(&bank[b][cbl], crc_mulmod_fast(r, crc.mult, c));
         }
         __syncthreads();
         if (!done) {
