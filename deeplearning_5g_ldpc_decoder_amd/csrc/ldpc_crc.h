@@ -103,6 +103,7 @@ __device__ inline uint32_t wave_xor_full(uint32_t v) {
 // Hard-decision byte from 8 consecutive beliefs (bit 7 = first belief).
 __device__ inline uint32_t hard_byte(const int16_t *lq, int b) {
     const uint4 v = *reinterpret_cast<const uint4 *>(lq + 8 * b);
+#ifdef NRLDPC_CPU_EMU
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
     uint32_t out = 0;
 #pragma unroll
@@ -111,6 +112,15 @@ __device__ inline uint32_t hard_byte(const int16_t *lq, int b) {
         out = (out << 1) | ((w[i] >> 31) & 1u);
     }
     return out;
+#else
+    // v_perm_b32 selectors 8-11 give 0x00 / 0xFF bytes from the sign bits of
+    // the low and high halves of its two words (beliefs 0-3, then 4-7); one
+    // bit per byte survives the mask, and v_sad_u8 against 0 adds the bytes,
+    // which ORs the distinct bits into the hard byte.
+    const uint32_t s01 = __builtin_amdgcn_perm(v.y, v.x, 0x0B0A0908u) & 0x10204080u;
+    const uint32_t s23 = __builtin_amdgcn_perm(v.w, v.z, 0x0B0A0908u) & 0x01020408u;
+    return __builtin_amdgcn_sad_u8(s01, 0u, __builtin_amdgcn_sad_u8(s23, 0u, 0u));
+#endif
 }
 
 __device__ inline uint32_t hard_byte(const float *lq, int b) {
