@@ -429,8 +429,8 @@ def main():
     if args.profile:
         args.no_cpu = True
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(codeblocks shard by index across GPUs with no exchange, see DESIGN.md)",
+        print(f"bench.py: the decoder runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a multi-GPU run would shard codeblocks by index, with no exchange)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -106,3 +106,34 @@ def test_hip_graph_capture_replay(torch_dev, oracle_mod):
         for k, v in out.items():
             assert np.array_equal(v.cpu().numpy(), ref[k]), (rep, k)
     plan.close()
+
+
+def test_copy_spans_round_trip(torch_dev):
+    """ldpc_mi355_copy_spans: ragged pinned spans (one empty) into one device
+    buffer at their own offsets, then back out in one span, on one stream."""
+    import ctypes
+    torch = torch_dev
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(4)
+    sizes = [1, 0, 4097, 77, 65536]
+    src = [torch.from_numpy(rng.integers(0, 256, max(s, 1)).astype(np.uint8)).pin_memory()
+           for s in sizes]
+    dev = torch.zeros(sum(sizes) + 64, dtype=torch.uint8, device="cuda")
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    spans = np.array([[dev.data_ptr() + int(o) for o in offs], [t.data_ptr() for t in src], sizes],
+                     np.uint64)
+    torch.cuda.synchronize()   # the zero fill ran on the default stream
+    stream = torch.cuda.Stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    assert L.ldpc_mi355_copy_spans(spans[0].ctypes.data, spans[1].ctypes.data,
+                                   spans[2].ctypes.data, len(sizes), 1, sh) == 0
+    back = torch.zeros(dev.numel(), dtype=torch.uint8).pin_memory()
+    out = np.array([[back.data_ptr()], [dev.data_ptr()], [dev.numel()]], np.uint64)
+    assert L.ldpc_mi355_copy_spans(out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data,
+                                   1, 2, sh) == 0
+    stream.synchronize()
+    b = back.numpy()
+    for o, s, t in zip(offs, sizes, src):
+        assert np.array_equal(b[o:o + s], t.numpy()[:s])
+    assert not b[sum(sizes):].any()

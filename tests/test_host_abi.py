@@ -227,3 +227,13 @@ def test_scale_table_setters_round_trip(L):
         tables.set_trained_tables(2)
         tables.set_degree_alpha()
     assert tables.get_trained_tables(2)[0].shape == (32, 42)
+
+
+def test_copy_spans_argument_checks(L):
+    """ldpc_mi355_copy_spans refuses bad arguments before any HIP call, and
+    a call with no spans does nothing."""
+    assert L.ldpc_mi355_copy_spans(None, None, None, 0, 1, None) == 0
+    assert L.ldpc_mi355_copy_spans(None, None, None, 0, 2, None) == 0
+    assert L.ldpc_mi355_copy_spans(None, None, None, -1, 1, None) == -1
+    assert L.ldpc_mi355_copy_spans(None, None, None, 0, 3, None) == -1   # unknown kind
+    assert L.ldpc_mi355_copy_spans(None, None, None, 2, 1, None) == -1   # no span arrays
