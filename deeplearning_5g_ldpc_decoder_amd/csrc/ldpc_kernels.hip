@@ -1293,9 +1293,8 @@ struct StateF32 {
 };
 
 template <int BG, int R>
-__device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st,
-                                          const uint32_t *__restrict__ etab, int zx,
-                                          int Zc, const DecParams &p) {
+__device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st, etab_cptr etab,
+                                          int zx, int Zc, const DecParams &p) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     const float o1 = st.n1[R], o2 = st.n2[R];
@@ -1350,12 +1349,15 @@ __device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &
     st.sgn[R] = bits | ((uint32_t)idx << 24);
 }
 
+// Rows with the barrier after each group of column-disjoint rows only
+// (RowGroups): such rows touch disjoint beliefs, so the result is the
+// row-serial one.
 template <int BG, int R>
-__device__ __forceinline__ void rows_f32ms(float *lq, StateF32<BG> &st, const uint32_t *etab,
+__device__ __forceinline__ void rows_f32ms(float *lq, StateF32<BG> &st, etab_cptr etab,
                                            int zx, int Zc, const DecParams &p) {
     if constexpr (R < Graph<BG>::kRows) {
         if (zx < Zc) row_f32ms<BG, R>(lq, st, etab, zx, Zc, p);
-        __syncthreads();
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         rows_f32ms<BG, R + 1>(lq, st, etab, zx, Zc, p);
     }
 }
@@ -1374,7 +1376,10 @@ __device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const
     int it = 0;
     *status = 1;
     while (it < p.max_it) {
-        rows_f32ms<BG, 0>(lq, st, etab, zx, d.Zc, p);
+        // the edge table read by scalar loads, reloaded each iteration
+        etab_cptr ce = (etab_cptr)etab;
+        NRLDPC_OPAQUE_UNIFORM(ce);
+        rows_f32ms<BG, 0>(lq, st, ce, zx, d.Zc, p);
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
