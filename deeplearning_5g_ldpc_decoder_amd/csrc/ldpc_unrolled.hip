@@ -182,7 +182,8 @@ __device__ __forceinline__ void urows_fwd(float *lq, UnrState<BG> &st, const Unr
                                           bool active, uint32_t *rec) {
     if constexpr (R < Graph<BG>::kRows) {
         if (active) urow_fwd<BG, R>(lq, st, a, etab, ab, zx, rec);
-        __syncthreads();
+        // column-disjoint rows commute (RowGroups): a barrier between groups only
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
         urows_fwd<BG, R + 1>(lq, st, a, etab, ab, zx, active, rec);
     }
 }
@@ -282,7 +283,11 @@ __device__ __forceinline__ void urows_bwd(float *glq, const UnrolledArgs &a, con
                                           const uint32_t *rec, float *glr) {
     if constexpr (R >= 0) {
         urow_bwd<BG, R>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec, glr);
-        __syncthreads();
+        // in reverse: a barrier where row R - 1 starts another group
+        if constexpr (R == 0)
+            __syncthreads();
+        else if constexpr (kRowGroups<BG>.barrier_after[R - 1])
+            __syncthreads();
         urows_bwd<BG, R - 1>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec,
                              glr);
     }
