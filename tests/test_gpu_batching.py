@@ -137,3 +137,32 @@ def test_copy_spans_round_trip(torch_dev):
     for o, s, t in zip(offs, sizes, src):
         assert np.array_equal(b[o:o + s], t.numpy()[:s])
     assert not b[sum(sizes):].any()
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_overlapping_launches_of_one_plan(torch_dev, oracle_mod, uniform):
+    """Launches of one plan in flight on two streams, each with its own
+    outputs (as the throughput bench runs them): every output set equals the
+    single-stream decode.  uniform: BG1 Zc=384 codeblocks only (the
+    specialized kernel); otherwise a mixed batch (packed tiles)."""
+    torch = torch_dev
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    rng = np.random.default_rng(21)
+    if uniform:
+        cfgs = [(1, 8424, 2, 25344, 0, 2, 0)] * 48
+        llrs = [rng.integers(-900, 1100, 25344).astype(np.int16) for _ in cfgs]
+    else:
+        cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=ALL_ZC[1::3])
+    d_llr = torch.from_numpy(np.concatenate(llrs)).cuda()
+    plan = BatchPlan(cfgs, ROW1, want_lq=True)
+    ref = {k: v.cpu().numpy() for k, v in plan.decode(d_llr).items()}
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    outs = [plan.alloc_outputs("cuda") for _ in range(2)]
+    torch.cuda.synchronize()
+    for k in range(8):
+        plan.launch(d_llr, outs[k % 2], stream=streams[k % 2].cuda_stream)
+    torch.cuda.synchronize()
+    for o in outs:
+        for k, v in o.items():
+            assert np.array_equal(v.cpu().numpy(), ref[k]), k
+    plan.close()
