@@ -270,7 +270,8 @@ def bench_serve(args):
     waterfall) arrive as a Poisson process from 16 UEs x 4 HARQ processes,
     each with a 0.5 ms deadline; failed transmissions come back after 1 ms with
     the next redundancy version (0, 2, 3, 1) and soft-combine into device
-    HARQ buffers.  Continuous batching over 3 launch contexts, admission by a
+    HARQ buffers.  Continuous batching over --serve-contexts launch contexts
+    (default 3), admission by a
     256 MB device LLR pool.  First the capacity (requests/s with every request
     queued at once), then offered loads of 0.2-1.0 x capacity."""
     from deeplearning_5g_ldpc_decoder_amd.serving import (DevicePool, GpuExecutor, HarqSlots,
@@ -279,7 +280,8 @@ def bench_serve(args):
     pool = RequestPool(128, seed=11, max_cbs=64, threads=min(16, os.cpu_count() or 8))
     harq = HarqSlots(16, 4, 64)
     llr_pool = DevicePool(256 << 20)
-    ex = GpuExecutor(pool, harq, llr_pool, n_contexts=3, max_batch_codeblocks=args.serve_batch)
+    ex = GpuExecutor(pool, harq, llr_pool, n_contexts=args.serve_contexts,
+                     max_batch_codeblocks=args.serve_batch)
     srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=1e-3)
     rng = np.random.default_rng(5)
 
@@ -331,7 +333,7 @@ def bench_serve(args):
             "ms_per_step": mid["latency_ms"]["mean"], "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
             "config": {"workload": "serve_mixed_1to64cb_harq16x4_poisson",
-                       "decoder_row": ROW, "contexts": 3,
+                       "decoder_row": ROW, "contexts": args.serve_contexts,
                        "max_batch_codeblocks": args.serve_batch,
                        "llr_pool_MB": 256, "harq_slots": 16 * 4 * 64,
                        "parallelism": "single-gpu"},
@@ -423,6 +425,8 @@ def main():
                          "FP32 BG2 BLER sweep")
     ap.add_argument("--serve-batch", type=int, default=256,
                     help="serve workload: codeblocks per batch at most")
+    ap.add_argument("--serve-contexts", type=int, default=3,
+                    help="serve workload: launch contexts (one stream each) in flight at most")
     ap.add_argument("--no-reference", action="store_true",
                     help="train workload: skip the torch-autograd reference-style step")
     args = ap.parse_args()
