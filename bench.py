@@ -294,7 +294,7 @@ def bench_serve(args):
     each with a 0.5 ms deadline; failed transmissions come back after 1 ms with
     the next redundancy version (0, 2, 3, 1) and soft-combine into device
     HARQ buffers.  Continuous batching over --serve-contexts launch contexts
-    (default 3), admission by a
+    (default 2), admission by a
     256 MB device LLR pool.  First the capacity (requests/s with every request
     queued at once), then offered loads of 0.2-1.0 x capacity."""
     from deeplearning_5g_ldpc_decoder_amd.serving import (DevicePool, GpuExecutor, HarqSlots,
@@ -448,7 +448,7 @@ def main():
                          "FP32 BG2 BLER sweep")
     ap.add_argument("--serve-batch", type=int, default=256,
                     help="serve workload: codeblocks per batch at most")
-    ap.add_argument("--serve-contexts", type=int, default=3,
+    ap.add_argument("--serve-contexts", type=int, default=2,
                     help="serve workload: launch contexts (one stream each) in flight at most")
     ap.add_argument("--no-reference", action="store_true",
                     help="train workload: skip the torch-autograd reference-style step")
