@@ -1292,6 +1292,12 @@ struct StateF32 {
     uint32_t sgn[Graph<BG>::kRows];   // sign bit per edge | argmin << 24
 };
 
+// v negated when bit j of bits is set: IEEE negation is a flip of the sign
+// bit, so one shift and one bitwise op give exactly -v or v.
+__device__ __forceinline__ float flip_sign(float v, uint32_t bits, int j) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) ^ ((bits << (31 - j)) & 0x80000000u));
+}
+
 template <int BG, int R>
 __device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st, etab_cptr etab,
                                           int zx, int Zc, const DecParams &p) {
@@ -1309,8 +1315,7 @@ __device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &
         c = (c >= Zc) ? c - Zc : c;
         addr[j] = (int)(t & 0xFFFFu) + c;
         const float ov = (j == oidx) ? o2 : o1;
-        const float lr = ((sb >> j) & 1u) ? -ov : ov;
-        rel[j] = lq[addr[j]] - lr;
+        rel[j] = lq[addr[j]] - flip_sign(ov, sb, j);
     }
     float m1 = 3.402823466e+38f, m2 = 3.402823466e+38f;
     int idx = 0;
@@ -1338,8 +1343,7 @@ __device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &
 #pragma unroll
     for (int j = 0; j < deg; ++j) {
         const float nv = (j == idx) ? n2 : n1;
-        const float lr = ((bits >> j) & 1u) ? -nv : nv;
-        float v = rel[j] + lr;
+        float v = rel[j] + flip_sign(nv, bits, j);
         if (v > p.fmax) v = p.fmax;
         if (v < p.fmin) v = p.fmin;
         lq[addr[j]] = v;
