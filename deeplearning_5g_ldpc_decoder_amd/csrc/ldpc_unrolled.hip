@@ -95,6 +95,12 @@ __device__ __forceinline__ float fmed3(float a, float b, float c) {
 }
 __device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
+// v negated when bit j of bits is set: IEEE negation flips the sign bit, so
+// a shift and one bitwise op give exactly -v or v.
+__device__ __forceinline__ float uflip_sign(float v, uint32_t bits, int j) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) ^ ((bits << (31 - j)) & 0x80000000u));
+}
+
 template <int BG>
 struct UnrState {
     float g1[Graph<BG>::kRows];   // alpha * max(m1 - beta, 0)
@@ -126,7 +132,7 @@ __device__ __forceinline__ void urow_fwd(float *__restrict__ lq, UnrState<BG> &s
         c = (c >= Zc) ? c - Zc : c;
         addr[j] = (int)(e & 0xFFFFu) + c;
         const float ov = (j == oidx) ? o2 : o1;
-        const float lr = ((sb >> j) & 1u) ? -ov : ov;
+        const float lr = uflip_sign(ov, sb, j);
         rel[j] = lq[addr[j]] - lr;
     }
     // two smallest magnitudes (with multiplicity), then the first index of
@@ -155,7 +161,7 @@ __device__ __forceinline__ void urow_fwd(float *__restrict__ lq, UnrState<BG> &s
 #pragma unroll
     for (int j = 0; j < deg; ++j) {
         const float mag = (j == i1) ? g2 : g1;
-        const float nl = ((bits >> j) & 1u) ? -mag : mag;
+        const float nl = uflip_sign(mag, bits, j);
         float q = rel[j] + nl;
         if (a.has_clip) {
             pass |= (uint32_t)(q >= -a.clip && q <= a.clip) << j;
@@ -246,7 +252,7 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
 #pragma unroll
         for (int j = 0; j < deg; ++j) {
             const float gn = lr_in ? gq[j] + gin[j] : gq[j];
-            const float gmag = ((bits >> j) & 1u) ? -gn : gn;
+            const float gmag = uflip_sign(gn, bits, j);
             const bool first = (j == i1);
             ga += gmag * (first ? c2 : c1);
             const float gd = (first ? p2 : p1) ? gmag * al : 0.0f;
