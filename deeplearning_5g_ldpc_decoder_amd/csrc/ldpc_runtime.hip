@@ -134,12 +134,12 @@ void bind_scale(DecParams &dp, const ScaleSet &s, const uint32_t *dev) {
     dp.simple = (s.fast && s.simple) ? 1 : 0;
 }
 
-// Algo 0 is part of the ABI but its GPU kernel is compiled out of this build
+// Algo 0 is part of the ABI but its GPU kernel is compiled out of the library
 // (see ldpc_kernels.hip); calls fail loudly rather than falling back.
 bool algo_runnable(int algo) { return algo != 0 || nrldpc_bp_kernel_enabled(); }
 
 void report_bp_disabled() {
-    fprintf(stderr, "libnrldpc_mi355: algo 0 (FP32 BP) GPU kernel is disabled in this build\n");
+    fprintf(stderr, "libnrldpc_mi355: algo 0 (FP32 BP) GPU kernel is disabled in this library\n");
 }
 
 DecParams make_params(const ldpc_decoder_cfg_t &c) { return decoder_params(c); }

@@ -299,7 +299,7 @@ void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *batch);
 int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src, const uint64_t *bytes,
                               int32_t n, int32_t kind, void *stream);
 
-/* 1 when this build runs decoder algo `algo` on the GPU, else 0 (algo 0,
+/* 1 when the library runs decoder algo `algo` on the GPU, else 0 (algo 0,
  * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls
  * then return -1 with a message instead of decoding). */
 int32_t ldpc_mi355_algo_available(int32_t algo);

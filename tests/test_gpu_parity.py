@@ -115,7 +115,7 @@ def test_algo0_disabled_fails_loudly(L):
     """While the BP kernel is compiled out, algo 0 returns -1 (no fallback),
     after rate recovery, like an unsupported algorithm."""
     if L.ldpc_mi355_algo_available(0):
-        pytest.skip("BP kernel enabled in this build")
+        pytest.skip("BP kernel enabled in the library")
     rng = np.random.default_rng(2)
     llr = rng.integers(-500, 500, 512).astype(np.int16)
     hard = np.full(25, 0x5A, np.uint8)
