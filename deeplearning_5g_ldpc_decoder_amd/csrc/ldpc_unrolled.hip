@@ -95,6 +95,14 @@ __device__ __forceinline__ float fmed3(float a, float b, float c) {
 }
 __device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
+// The edge table through the constant address space: uniform indices become
+// scalar loads and the column / shift decoding scalar ops.
+#ifdef NRLDPC_CPU_EMU
+typedef const uint32_t *uetab_ptr;
+#else
+typedef const __attribute__((address_space(4))) uint32_t *uetab_ptr;
+#endif
+
 // v negated when bit j of bits is set: IEEE negation flips the sign bit, so
 // a shift and one bitwise op give exactly -v or v.
 __device__ __forceinline__ float uflip_sign(float v, uint32_t bits, int j) {
@@ -111,7 +119,7 @@ struct UnrState {
 // One lifted check of row R, forward.
 template <int BG, int R>
 __device__ __forceinline__ void urow_fwd(float *__restrict__ lq, UnrState<BG> &st,
-                                         const UnrolledArgs &a, const uint32_t *etab,
+                                         const UnrolledArgs &a, uetab_ptr etab,
                                          const float *ab, int zx, uint32_t *__restrict__ rec) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
@@ -119,6 +127,7 @@ __device__ __forceinline__ void urow_fwd(float *__restrict__ lq, UnrState<BG> &s
     int Zc = a.Zc;
     NRLDPC_OPAQUE_S(Zc);
     NRLDPC_OPAQUE_V(zx);
+    NRLDPC_OPAQUE_S(etab);
     const float al = ab[R], be = ab[nR + R];
     const float o1 = st.g1[R], o2 = st.g2[R];
     const uint32_t sb = st.sgn[R];
@@ -184,7 +193,7 @@ __device__ __forceinline__ void urow_fwd(float *__restrict__ lq, UnrState<BG> &s
 
 template <int BG, int R>
 __device__ __forceinline__ void urows_fwd(float *lq, UnrState<BG> &st, const UnrolledArgs &a,
-                                          const uint32_t *etab, const float *ab, int zx,
+                                          uetab_ptr etab, const float *ab, int zx,
                                           bool active, uint32_t *rec) {
     if constexpr (R < Graph<BG>::kRows) {
         if (active) urow_fwd<BG, R>(lq, st, a, etab, ab, zx, rec);
@@ -206,7 +215,7 @@ __device__ __forceinline__ float seg_sum(float v, int seg) {
 // and written (not in the first).
 template <int BG, int R>
 __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const UnrolledArgs &a,
-                                         const uint32_t *etab, const float *ab, int zx, int b,
+                                         uetab_ptr etab, const float *ab, int zx, int b,
                                          int t,
                                          bool active, bool member, bool lr_in, bool lr_out,
                                          const uint32_t *__restrict__ rec,
@@ -217,6 +226,7 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
     int Zc = a.Zc;
     NRLDPC_OPAQUE_S(Zc);
     NRLDPC_OPAQUE_V(zx);
+    NRLDPC_OPAQUE_S(etab);
     float ga = 0.0f, gb = 0.0f;
     if (active) {
         const float al = ab[R], be = ab[nR + R];
@@ -283,7 +293,7 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
 }
 
 template <int BG, int R>
-__device__ __forceinline__ void urows_bwd(float *glq, const UnrolledArgs &a, const uint32_t *etab,
+__device__ __forceinline__ void urows_bwd(float *glq, const UnrolledArgs &a, uetab_ptr etab,
                                           const float *ab, int zx, int b, int t, bool active,
                                           bool member, bool lr_in, bool lr_out,
                                           const uint32_t *rec, float *glr) {
@@ -299,12 +309,6 @@ __device__ __forceinline__ void urows_bwd(float *glq, const UnrolledArgs &a, con
     }
 }
 
-// Edge table of the code into LDS (uniform reads in the row loops).
-template <int BG>
-__device__ __forceinline__ void load_etab(uint32_t *dst, const uint32_t *src) {
-    for (int i = threadIdx.x; i < Graph<BG>::kEdges; i += blockDim.x) dst[i] = src[i];
-}
-
 // alpha[t][*] and beta[t][*] into LDS for the rows of iteration t.  The
 // caller has a barrier between the previous iteration's rows and this.
 template <int BG>
@@ -318,10 +322,9 @@ __device__ __forceinline__ void load_ab(float *ab, const UnrolledArgs &a, int t)
 template <int BG>
 __global__ void __launch_bounds__(kMaxThreads) unrolled_fwd_kernel(UnrolledArgs a) {
     NRLDPC_DYNAMIC_LDS(smem);
-    __shared__ uint32_t etab[Graph<BG>::kEdges];
     __shared__ float ab[2 * Graph<BG>::kRows];
     constexpr int nR = Graph<BG>::kRows;
-    load_etab<BG>(etab, a.etab);
+    const uetab_ptr etab = (uetab_ptr)a.etab;
     const int lane = threadIdx.x, cbl = lane / a.lpc, zx = lane - cbl * a.lpc;
     const int b = blockIdx.x * a.cbs + cbl;
     const bool member = b < a.batch;
@@ -370,10 +373,9 @@ __global__ void __launch_bounds__(kMaxThreads) unrolled_fwd_kernel(UnrolledArgs 
 template <int BG>
 __global__ void __launch_bounds__(kMaxThreads) unrolled_bwd_kernel(UnrolledArgs a) {
     NRLDPC_DYNAMIC_LDS(smem);
-    __shared__ uint32_t etab[Graph<BG>::kEdges];
     __shared__ float ab[2 * Graph<BG>::kRows];
     constexpr int nR = Graph<BG>::kRows;
-    load_etab<BG>(etab, a.etab);
+    const uetab_ptr etab = (uetab_ptr)a.etab;
     constexpr int nE = Graph<BG>::kEdges;
     const int lane = threadIdx.x, cbl = lane / a.lpc, zx = lane - cbl * a.lpc;
     const int b = blockIdx.x * a.cbs + cbl;
