@@ -13,9 +13,8 @@
 //   * min-sum check-to-variable messages: VGPRs, compressed per lifted check
 //     to (min1', min2', argmin, one sign bit per edge) -- the reference's
 //     message of edge j is exactly sign_j * (j == argmin ? min2' : min1'),
-//     so the compressed form re-expands bit-exactly;
-//   * BP messages (no compressed form): a float array in HBM, edge-major so
-//     one row edge of all lanes is one coalesced 4*Zc-byte access.
+//     so the compressed form re-expands bit-exactly.
+// (The FP32 BP decoder, algo 0, is in ldpc_bp.hip.)
 // Rows are unrolled at compile time per base graph so the per-row state sits
 // at fixed register indices.
 //
@@ -38,13 +37,7 @@
 #include "ldpc_device.h"
 #include "ldpc_graph.h"
 #include "ldpc_crc.h"
-
-// The FP32 BP kernel (algo 0) did not return on its first MI355X run and no
-// defect has been found in its source, so it is compiled out until one is:
-// algo 0 then fails with an explicit error instead of running.
-#ifndef NRLDPC_BP_KERNEL
-#define NRLDPC_BP_KERNEL 0
-#endif
+#include "ldpc_recover.h"
 
 // Opaque per-row redefinition of a loop-invariant value: keeps the compiler
 // from hoisting every row's (constant-offset) address arithmetic out of the
@@ -58,99 +51,6 @@
 #endif
 
 namespace nrldpc {
-
-// ---------------------------------------------------------------------------
-// Rate recovery + HARQ combine fused into the LDS fill
-// (c_lib/ldpc_rate_recovery.c:24-65).  Gather form: buffer position p takes
-// the sum of every LLR k with compact index q(p) = (ck0 + k) mod (N - F),
-// k < E; the sum wraps mod 2^16 exactly like the reference's int16 +=.
-//
-// recover_fill writes positions first, first + stride, ... < n of one
-// codeblock into lq (as T) and, when the codeblock has a soft buffer, into
-// it.  A lane works on kRecoverBatch positions at a time and issues their
-// first LLR loads back to back, so the gather runs at memory throughput
-// instead of one dependent global load per position; further contributions
-// (E > N - F, repetition) are added in a second, usually empty, loop.
-// ---------------------------------------------------------------------------
-constexpr int kRecoverBatch = 8;
-
-// Belief as the decoder stores it: int16 as is, float / 128 (algos 0 and 4,
-// c_lib/floatpoint_ldpc_decoder.c:325-327).
-template <typename T> __device__ inline T from_soft(int16_t v);
-template <> __device__ inline int16_t from_soft<int16_t>(int16_t v) { return v; }
-template <> __device__ inline float from_soft<float>(int16_t v) { return (float)v / 128.0f; }
-
-struct RecoverCtx {
-    int Zc, B, K, F, lo, nc, ck0, S, E, Qm;
-};
-
-__device__ __forceinline__ RecoverCtx recover_ctx(const CbDesc &d) {
-    RecoverCtx c;
-    c.Zc = d.Zc;
-    c.B = d.B;
-    c.K = d.K;
-    c.F = d.K - d.B;
-    c.lo = d.B - 2 * d.Zc;
-    const int hi = d.K - 2 * d.Zc;
-    c.nc = d.N - c.F;
-    c.ck0 = d.k0 < c.lo ? d.k0 : (d.k0 < hi ? c.lo : d.k0 - c.F);
-    c.E = d.E;
-    c.Qm = d.Qm;
-    c.S = d.E / d.Qm;
-    return c;
-}
-
-// LLR index of rate-matched bit k (bit-interleaved: symbol k mod S, bit k / S).
-__device__ __forceinline__ int llr_index(int k, const RecoverCtx &c) {
-    int sym = k, bit = 0;
-    while (sym >= c.S) {
-        sym -= c.S;
-        ++bit;
-    }
-    return sym * c.Qm + bit;
-}
-
-template <typename T>
-__device__ void recover_fill(T *lq, const CbDesc &d, const int16_t *__restrict__ llr,
-                             int16_t *soft, int filler, int first, int stride, int n) {
-    const RecoverCtx c = recover_ctx(d);
-    const bool harq = d.harq_on && soft;
-    for (int i0 = first; i0 < n; i0 += stride * kRecoverBatch) {
-        int acc[kRecoverBatch], k[kRecoverBatch];
-#pragma unroll
-        for (int u = 0; u < kRecoverBatch; ++u) {
-            const int p = i0 + u * stride;
-            acc[u] = 0;
-            k[u] = c.E;   // no contribution
-            if (p < n) {
-                if (harq) acc[u] = soft[p];
-                if (p >= c.B && p < c.K) {
-                    acc[u] = filler;
-                } else if (p >= 2 * c.Zc) {
-                    const int pos = p - 2 * c.Zc;
-                    int kk = (pos < c.lo ? pos : pos - c.F) - c.ck0;
-                    if (kk < 0) kk += c.nc;
-                    k[u] = kk;
-                }
-            }
-        }
-        int add[kRecoverBatch];
-#pragma unroll
-        for (int u = 0; u < kRecoverBatch; ++u)
-            add[u] = k[u] < c.E ? (int)llr[llr_index(k[u], c)] : 0;
-#pragma unroll
-        for (int u = 0; u < kRecoverBatch; ++u) {
-            const int p = i0 + u * stride;
-            int v = acc[u] + add[u];
-            for (int kk = k[u] + c.nc; kk < c.E; kk += c.nc) v += llr[llr_index(kk, c)];
-            if (p < n) {
-                const int16_t w = (int16_t)v;   // the reference's int16 += wraps mod 2^16
-                if (soft) soft[p] = w;
-                lq[p] = from_soft<T>(w);
-            }
-        }
-    }
-}
 
 // Scatter form of the same recovery, for codeblocks without repetition
 // (E <= N - F: every buffer position receives at most one LLR).  The LLRs are
@@ -1393,102 +1293,11 @@ __device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const
     return it;
 }
 
-#if NRLDPC_BP_KERNEL
-// ---------------------------------------------------------------------------
-// Algo 0: FP32 layered BP (c_lib/floatpoint_ldpc_decoder.c:53-151).  Exact
-// zeros are special-cased like the reference; messages saturate at
-// +-2*19.07 when the extrinsic product reaches +-1; LQ is not clipped.
-// Device tanhf/atanhf: results track the reference within tolerance.
-// ---------------------------------------------------------------------------
-__device__ inline float bp_msg(float prod) {
-    const float sat = 2.0f * 19.07f;
-    if (prod >= 1.0f) return sat;
-    if (prod <= -1.0f) return -sat;
-    return 2.0f * atanhf(prod);
-}
-
-template <int BG, int R>
-__device__ __forceinline__ void row_bp(float *__restrict__ lq, float *__restrict__ lr_mem,
-                                       const uint32_t *__restrict__ etab, int zx, int Zc,
-                                       bool first) {
-    constexpr int e0 = Graph<BG>::start(R);
-    constexpr int deg = Graph<BG>::start(R + 1) - e0;
-    int addr[deg];
-    float rel[deg], t[deg];
-#pragma unroll
-    for (int j = 0; j < deg; ++j) {
-        const uint32_t tt = etab[e0 + j];
-        int c = zx + (int)(tt >> 16);
-        c = (c >= Zc) ? c - Zc : c;
-        addr[j] = (int)(tt & 0xFFFFu) + c;
-        const float old = first ? 0.0f : lr_mem[(e0 + j) * Zc + zx];
-        rel[j] = lq[addr[j]] - old;
-    }
-    int nz = 0, zidx = -1;
-    float prod = 1.0f;
-#pragma unroll
-    for (int j = 0; j < deg; ++j) {
-        const bool z = rel[j] == 0.0f;
-        t[j] = z ? 0.0f : tanhf(rel[j] / 2.0f);
-        if (z) { ++nz; zidx = j; }
-        else prod *= t[j];
-    }
-    float msg[deg];
-    if (nz >= 1) {
-        const float m = (nz == 1) ? bp_msg(prod) : 0.0f;
-#pragma unroll
-        for (int j = 0; j < deg; ++j) msg[j] = (j == zidx) ? m : 0.0f;
-    } else {
-        float pre[deg + 1], suf[deg + 1];
-        pre[0] = 1.0f;
-#pragma unroll
-        for (int j = 0; j < deg; ++j) pre[j + 1] = pre[j] * t[j];
-        suf[deg] = 1.0f;
-#pragma unroll
-        for (int j = deg - 1; j >= 0; --j) suf[j] = suf[j + 1] * t[j];
-#pragma unroll
-        for (int j = 0; j < deg; ++j) msg[j] = bp_msg(pre[j] * suf[j + 1]);
-    }
-#pragma unroll
-    for (int j = 0; j < deg; ++j) {
-        lr_mem[(e0 + j) * Zc + zx] = msg[j];
-        lq[addr[j]] = rel[j] + msg[j];
-    }
-}
-
-template <int BG, int R>
-__device__ __forceinline__ void rows_bp(float *lq, float *lr_mem, const uint32_t *etab, int zx,
-                                        int Zc, bool first) {
-    if constexpr (R < Graph<BG>::kRows) {
-        if (zx < Zc) row_bp<BG, R>(lq, lr_mem, etab, zx, Zc, first);
-        __syncthreads();
-        rows_bp<BG, R + 1>(lq, lr_mem, etab, zx, Zc, first);
-    }
-}
-
-template <int BG>
-__device__ int run_bp(float *lq, float *lr_mem, const uint32_t *etab, const CbDesc &d,
-                      const DecParams &p, const CrcCtx &crc, const uint32_t *tab,
-                      uint32_t *slots, int *status) {
-    const int zx = threadIdx.x;
-    int it = 0;
-    *status = 1;
-    while (it < p.max_it) {
-        rows_bp<BG, 0>(lq, lr_mem, etab, zx, d.Zc, it == 0);
-        ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
-            *status = 0;
-            break;
-        }
-    }
-    return it;
-}
-#endif  // NRLDPC_BP_KERNEL
 
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4, kBpF32 = 0 };
+enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4 };
 
 
 // kGeneral selects the exact-for-every-row int16 path (see row_i16) and
@@ -1526,12 +1335,6 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
         } else if constexpr (A == Algo::kMinSumF32) {
             it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
-        } else {
-#if NRLDPC_BP_KERNEL
-            float *lr_mem = o.work + d.work_off;
-            it = (d.bgn == 1) ? run_bp<1>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status)
-                              : run_bp<2>(lq, lr_mem, etab, d, p, crc, crc_tab, slots, &status);
-#endif
         }
         // hard decisions of the final iteration (left untouched when no
         // iteration ran, like the reference)
@@ -1777,6 +1580,10 @@ const void *tile_kernel(bool general, bool table) {
 
 // Host-callable launchers (C linkage inside the library only).  Algos 1-3
 // share the int16 kernels (2 and 3 with per-row scale words).
+extern "C" hipError_t nrldpc_launch_bp(const nrldpc::DecOut *o, const nrldpc::DecParams *p,
+                                       int ncb, int threads, int max_soft, hipStream_t stream);
+extern "C" hipError_t nrldpc_configure_bp();
+
 // uniform: (bgn << 16) | Zc when every codeblock of the launch shares one base
 // graph and lifting size, else 0; a specialized int16 kernel runs when one
 // exists for that pair.
@@ -1810,11 +1617,8 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
     case 4:
         fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>);
         break;
-#if NRLDPC_BP_KERNEL
     case 0:
-        fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>);
-        break;
-#endif
+        return nrldpc_launch_bp(o, p, ncb, threads, max_soft, stream);
     default:
         return hipErrorInvalidValue;
     }
@@ -1851,12 +1655,8 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-#if NRLDPC_BP_KERNEL
     if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kBpF32, float>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-#endif
+    e = nrldpc_configure_bp();
     if (e != hipSuccess) return e;
     // tiles hold at most 384 lanes' worth of beliefs (~56 KB) next to 5.6 KB
     // of static LDS
@@ -1867,8 +1667,6 @@ extern "C" hipError_t nrldpc_configure_kernels() {
                                     tile_lds);
     return e;
 }
-
-extern "C" int nrldpc_bp_kernel_enabled() { return NRLDPC_BP_KERNEL; }
 
 // Packed-tile launch (algos 1-3): ntiles workgroups of kMaxThreads lanes.
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,

@@ -18,7 +18,6 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
                                            int ncb, int threads, int max_soft, uint32_t uniform,
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
-extern "C" int nrldpc_bp_kernel_enabled();
 extern "C" hipError_t nrldpc_launch_synth(int bgn, const SynthArgs *a, hipStream_t s);
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const DecOut *o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams *p, int ntiles,
@@ -134,14 +133,6 @@ void bind_scale(DecParams &dp, const ScaleSet &s, const uint32_t *dev) {
     dp.simple = (s.fast && s.simple) ? 1 : 0;
 }
 
-// Algo 0 is part of the ABI but its GPU kernel is compiled out of the library
-// (see ldpc_kernels.hip); calls fail loudly rather than falling back.
-bool algo_runnable(int algo) { return algo != 0 || nrldpc_bp_kernel_enabled(); }
-
-void report_bp_disabled() {
-    fprintf(stderr, "libnrldpc_mi355: algo 0 (FP32 BP) GPU kernel is disabled in this library\n");
-}
-
 DecParams make_params(const ldpc_decoder_cfg_t &c) { return decoder_params(c); }
 
 CbDesc make_desc(const Params &p, const Runtime &r) {
@@ -172,8 +163,7 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     std::lock_guard<std::mutex> guard(r.mu);
 
     const bool valid_algo = algo_supported(decoder_config->algo);
-    const bool runnable = valid_algo && algo_runnable(decoder_config->algo);
-    const int algo = runnable ? decoder_config->algo : 1;
+    const int algo = valid_algo ? decoder_config->algo : 1;
     ScaleSet scale;
     if (table_algo(algo)) {
         make_scale_set(*decoder_config, scale);
@@ -184,14 +174,14 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
         }
     }
     DecParams dp = make_params(*decoder_config);
-    if (!runnable) dp.max_it = 0;   // rate recovery only, like the reference
+    if (!valid_algo) dp.max_it = 0;   // rate recovery only, like the reference
     const int n = p.N + 2 * p.Zc;
 
     CbDesc d = make_desc(p, r);
     d.llr_off = 0;
     d.hard_off = 0;
     d.soft_off = 0;
-    d.lq_off = (posteriors && runnable) ? 0u : kNone;
+    d.lq_off = (posteriors && valid_algo) ? 0u : kNone;
     d.work_off = 0;
     memcpy(r.h_in + InLayout::desc, &d, sizeof d);
     memcpy(r.h_in + InLayout::llr, LLr, (size_t)p.E * 2);
@@ -237,10 +227,6 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     if (!valid_algo) {
         fprintf(stderr, "Invalid decoding algorithm selection: %u\n", decoder_config->algo);
         return -3;
-    }
-    if (!runnable) {
-        report_bp_disabled();
-        return -1;
     }
     if (its > 0) memcpy(hardbits, r.h_out + OutLayout::hard, (size_t)p.B / 8);
     if (iterations) *iterations = its;
@@ -371,10 +357,6 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     if (bad_index) *bad_index = -1;
     if (n <= 0) return -2;
     if (!algo_supported(dec->algo)) return -3;
-    if (!algo_runnable(dec->algo)) {
-        report_bp_disabled();
-        return -1;
-    }
     Runtime &r = rt();
     if (r.ensure() != 0) return -1;
     auto *pl = new ldpc_mi355_plan;
@@ -757,7 +739,7 @@ extern "C" int32_t ldpc_mi355_sync(void) {
 extern "C" int32_t ldpc_mi355_soft_slot_elems(void) { return kMaxSoft; }
 
 extern "C" int32_t ldpc_mi355_algo_available(int32_t algo) {
-    return algo_supported(algo) && algo_runnable(algo) ? 1 : 0;
+    return algo_supported(algo) ? 1 : 0;
 }
 
 extern "C" const char *ldpc_mi355_version(void) { return "nrldpc-mi355 0.1 (gfx950)"; }

@@ -3,6 +3,7 @@
 // infrastructure: lets CPU tests check the kernel source logic against the
 // oracle, and shows whether a kernel's control flow terminates.
 #include "ldpc_kernels.hip"
+#include "ldpc_bp.hip"
 
 #include "ldpc_host.h"
 
@@ -192,7 +193,7 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
         return 0;
     case 0:
         emu_launch(n, threads, (size_t)b.max_soft * 4,
-                   [&] { ldpc_decode_kernel<Algo::kBpF32, float>(o, p); });
+                   [&] { ldpc_bp_kernel(o, p); });
         return 0;
     default:
         return -3;

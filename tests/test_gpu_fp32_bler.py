@@ -1,7 +1,10 @@
-"""FP32 min-sum (algo 4) on the BG2 workload of the FP32 configuration: Zc 104 and
-208 (B = 1040 / 2080, CRC24B), R = 0.2 and 0.5, Qm 2 and 4, decoder row
-[4, 20, 22936, 0, 1280, 32767, -32767] (20 iterations, CRC early stop):
+"""The FP32 decoders on the BG2 workload of the FP32 configuration: all 8
+codes of Zc 104 and 208 (A = 1016 / 2056, CRC24B), R = 0.2 and 0.5, Qm 2 and
+4, 20 iterations with CRC early stop, for FP32 min-sum (algo 4, row
+[4, 20, 22936, 0, 1280, 32767, -32767]) and FP32 BP (algo 0, row
+[0, 20, 1, 0, 1280, 20, -32768]).
 
+Algo 4:
 * GPU posteriors equal, bit for bit, to the reference's FP32 decoder (the
   oracle's restatement of c_lib's FP32 min-sum) on and off the waterfall;
 * GPU posteriors within 1e-3 * max|LLR| of the FP64 oracle (the FP64 twin of
@@ -9,11 +12,21 @@
   converges.  On the waterfall itself, where codeblocks run 15-20 iterations,
   min-sum's argmin decisions amplify rounding differences: the reference's own
   FP32 decoder then differs from FP64 by up to the size of the posteriors, and
-  the GPU's FP32 result is that same FP32 result;
-* GPU BLER inside the Monte Carlo confidence interval of the reference
-  decoder (the oracle's FP32 c_lib restatement) run on independent noise.
+  the GPU's FP32 result is that same FP32 result.
+
+Algo 0 (device tanhf/atanhf, which differ from glibc's in the last ulp):
+* status, iterations and hard decisions equal to the reference's FP32 BP,
+  posteriors within 1e-3 * max|LLR| of it but for at most 2 beliefs per
+  codeblock, and no farther from FP64 than the reference's own FP32 BP
+  (test_gpu_parity._bp_matches_reference states why FP32 BP and FP64 part
+  once beliefs pass the FP32 tanh saturation).
+
+Both: GPU BLER inside the Monte Carlo confidence interval of the reference
+decoder (the oracle's FP32 c_lib restatement) run on independent noise, at a
+point on each code's waterfall found by a GPU sweep.
 """
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -22,8 +35,10 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 ROW4 = [4, 20, 22936, 0, 1280, 32767, -32767]
-CONFIGS = [(2, 1016, 2, 5200, 0, 2, 0), (2, 1016, 2, 2080, 0, 4, 0),
-           (2, 2056, 2, 10400, 0, 4, 0), (2, 2056, 2, 4160, 0, 2, 0)]
+ROW0 = [0, 20, 1, 0, 1280, 20, -32768]
+CONFIGS = [(2, A, 2, E, 0, Qm, 0)
+           for A, Es in ((1016, (5200, 2080)), (2056, (10400, 4160)))
+           for E in Es for Qm in (2, 4)]
 
 
 def _say(*a):
@@ -37,29 +52,28 @@ def _waterfall_noise_db(cfg, offset_db):
     return -(waterfall_esn0_db(B / cfg[3], cfg[5], B) + offset_db)
 
 
-def _gpu_decode(cfg, llr, want_lq=True):
+def _gpu_decode(cfg, row, llr, want_lq=True):
     from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
     n = llr.shape[0]
-    plan = BatchPlan([cfg] * n, ROW4, want_lq=want_lq)
+    plan = BatchPlan([cfg] * n, row, want_lq=want_lq)
     out = plan.decode(torch.from_numpy(llr.reshape(-1)).cuda())
     res = {k: v.cpu().numpy() for k, v in out.items()}
     lq = plan.split_lq(res["lq"]) if want_lq else None
+    hard = plan.split_hard(res["hard"])
     plan.close()
-    return res, lq
+    return res, lq, hard
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
-def test_fp32_posteriors_match_reference_and_fp64(oracle_mod, cfg):
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+def test_fp32_minsum_posteriors_match_reference_and_fp64(oracle_mod, cfg):
     from deeplearning_5g_ldpc_decoder_amd.batch import synth_batch
     O = oracle_mod
-    _say(f"fp32 posteriors cfg={cfg}")
+    _say(f"algo 4 posteriors cfg={cfg}")
     c = O.code(*cfg)
-    n = 48
+    n = 32
     for k, off in enumerate((-0.5, 0.5, 1.5)):
         llr, _ = synth_batch(cfg, n, _waterfall_noise_db(cfg, off), seed=100 + k)
-        res, lq = _gpu_decode(cfg, llr)
+        res, lq, _ = _gpu_decode(cfg, ROW4, llr)
         worst = 0.0
         for i in range(n):
             soft = O.rate_recover(c, ROW4[4], llr[i])
@@ -76,34 +90,71 @@ def test_fp32_posteriors_match_reference_and_fp64(oracle_mod, cfg):
             assert worst <= 1e-3
 
 
-@pytest.mark.parametrize("cfg,offset_db", [(CONFIGS[0], -0.45), (CONFIGS[1], -0.75)])
-def test_fp32_bler_inside_reference_ci(oracle_mod, cfg, offset_db):
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_bp_posteriors_match_reference(oracle_mod, cfg):
+    from test_gpu_parity import _bp_matches_reference
+    from deeplearning_5g_ldpc_decoder_amd.batch import synth_batch
+    O = oracle_mod
+    _say(f"algo 0 posteriors cfg={cfg}")
+    c = O.code(*cfg)
+    n = 16
+    for k, off in enumerate((-1.0, 0.0, 1.5)):
+        llr, _ = synth_batch(cfg, n, _waterfall_noise_db(cfg, off), seed=200 + k)
+        res, lq, hard = _gpu_decode(cfg, ROW0, llr)
+        for i in range(n):
+            soft = O.rate_recover(c, ROW0[4], llr[i])
+            out = dict(status=int(res["status"][i]), iters=int(res["iters"][i]), hard=hard[i],
+                       lq=lq[i])
+            _bp_matches_reference(O, out, cfg, ROW0, soft)
+
+
+def _waterfall_point(cfg, row, target=0.3):
+    """Noise power on the waterfall of this decoder: the point of a GPU sweep
+    (512 codeblocks per point, 0.25 dB steps) whose BLER is closest to
+    target."""
+    from deeplearning_5g_ldpc_decoder_amd.batch import synth_batch
+    best = None
+    for off in np.arange(-2.0, 1.01, 0.25):
+        noise = _waterfall_noise_db(cfg, float(off))
+        llr, _ = synth_batch(cfg, 512, noise, seed=int(1000 + 100 * off), want_payload=False)
+        res, _, _ = _gpu_decode(cfg, row, llr, want_lq=False)
+        bler = float(np.count_nonzero(res["status"] != 0)) / 512
+        if best is None or abs(bler - target) < abs(best[1] - target):
+            best = (noise, bler)
+    return best[0]
+
+
+@pytest.mark.parametrize("algo", [4, 0])
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_fp32_bler_inside_reference_ci(oracle_mod, cfg, algo):
     from deeplearning_5g_ldpc_decoder_amd.batch import synth_batch
     from deeplearning_5g_ldpc_decoder_amd.ldpc_sim import bler_confidence_interval
     O = oracle_mod
+    row = ROW4 if algo == 4 else ROW0
     c = O.code(*cfg)
-    noise = _waterfall_noise_db(cfg, offset_db)     # on the waterfall of this decoder
+    noise = _waterfall_point(cfg, row)
     n_gpu, n_ref = 4096, 384
-    _say(f"fp32 BLER cfg={cfg} noise={noise:.2f} dB: {n_gpu} GPU / {n_ref} reference codeblocks")
-    llr, _ = synth_batch(cfg, n_gpu, noise, seed=7)
-    res, _ = _gpu_decode(cfg, llr, want_lq=False)
+    _say(f"algo {algo} BLER cfg={cfg} noise={noise:.2f} dB: {n_gpu} GPU / {n_ref} reference "
+         "codeblocks")
+    llr, _ = synth_batch(cfg, n_gpu, noise, seed=7, want_payload=False)
+    res, _, _ = _gpu_decode(cfg, row, llr, want_lq=False)
     e_gpu = int(np.count_nonzero(res["status"] != 0))
-    llr2, _ = synth_batch(cfg, n_ref, noise, seed=8)       # independent noise
-    e_ref = 0
-    for i in range(n_ref):
-        soft = O.rate_recover(c, ROW4[4], llr2[i])
-        st, _, _, _ = O.decode_soft(c, ROW4, soft, precision="f32")
-        e_ref += int(st != 0)
+    llr2, _ = synth_batch(cfg, n_ref, noise, seed=8, want_payload=False)   # independent noise
+
+    def ref_status(i):
+        soft = O.rate_recover(c, row[4], llr2[i])
+        return O.decode_soft(c, row, soft, precision="f32")[0]
+
+    with ThreadPoolExecutor(8) as ex:
+        e_ref = sum(int(s != 0) for s in ex.map(ref_status, range(n_ref)))
     lo_g, hi_g = bler_confidence_interval(e_gpu, n_gpu, z=3.29)
     lo_r, hi_r = bler_confidence_interval(e_ref, n_ref, z=3.29)
     _say(f"  GPU {e_gpu / n_gpu:.4f} [{lo_g:.4f}, {hi_g:.4f}]  reference {e_ref / n_ref:.4f} "
          f"[{lo_r:.4f}, {hi_r:.4f}]")
     assert 0.02 < e_gpu / n_gpu < 0.98          # the point is on the waterfall
     assert lo_r <= hi_g and lo_g <= hi_r       # 99.9 % intervals overlap
-    # and on identical inputs the decisions are the reference's exactly
+    # and on identical inputs the decisions are the reference's
     for i in range(16):
-        soft = O.rate_recover(c, ROW4[4], llr[i])
-        st, hard, _, it = O.decode_soft(c, ROW4, soft, precision="f32")
+        soft = O.rate_recover(c, row[4], llr[i])
+        st, _, _, it = O.decode_soft(c, row, soft, precision="f32")
         assert st == res["status"][i] and it == res["iters"][i]

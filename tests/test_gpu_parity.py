@@ -3,7 +3,9 @@ oracle.  Every call goes through the C ABI of libnrldpc_mi355.so.
 
 Bars: algo 1 (int16 min-sum) and algo 4 (FP32 min-sum) bit-exact (hard
 bits, status, iterations, posteriors, HARQ buffer); algo 0 (FP32 BP, device
-tanhf/atanhf) posteriors within 1e-3 * max|LLR| of the FP64 oracle.
+tanhf/atanhf) status, iterations and hard decisions equal to the
+reference's FP32 BP and posteriors within 1e-3 * max|LLR| of it except on
+FP32 saturation flips (see _bp_matches_reference).
 """
 import ctypes
 
@@ -78,55 +80,87 @@ def test_algo4_abi_matches_reference(L, case):
     _check_exact(L, case)
 
 
+def _bp_matches_reference(oracle_mod, out, cfg, row, soft, ref32=None):
+    """GPU FP32 BP (device tanhf/atanhf) vs the reference's FP32 BP (the
+    oracle's restatement, bit-exact with the reference fixtures; ref32 =
+    (status, iterations, hard, lq) when already known) and the FP64 oracle:
+
+    * status, iterations and hard decisions equal to the reference's;
+    * posteriors within 1e-3 * max|LLR| of the reference's, except the
+      edges of at most one check per codeblock: device and glibc tanhf differ
+      in the last ulp, and where the FP32 tanh of an edge rounds to exactly 1
+      in one and not the other, that check's extrinsic products change and
+      its messages jump between 2 * 19.07 (saturation) and
+      2 * atanh(1 - 2^-24) = 17.33.  Such beliefs keep the reference's sign
+      and stay within one jump, 38.14 - 17.33 < 21, of its value;
+    * no farther from FP64 than the reference is: the reference's own FP32
+      BP saturates wherever tanh(|x| / 2) rounds to 1 in FP32 (|x| > 18.02),
+      FP64 only where the exact product reaches 1, so once beliefs grow the
+      reference's FP32 posteriors sit up to about half of max|LLR| from FP64
+      (measured on its own fixtures); the GPU's distance to FP64 is bounded
+      by the reference's plus 1e-3 * max|LLR|."""
+    c = oracle_mod.code(*cfg)
+    n = len(soft)
+    if ref32 is None:
+        s32, h32, q32, k32 = oracle_mod.decode_soft(c, row, soft, precision="f32")
+    else:
+        s32, k32, h32, q32 = ref32
+    assert out["status"] == s32 and out["iters"] == k32, (cfg, out["status"], out["iters"], s32, k32)
+    if k32 == 0:
+        return
+    assert np.array_equal(out["hard"], h32), cfg
+    lq = out["lq"][:n].astype(np.float64)
+    q32 = np.asarray(q32, np.float64)
+    m = np.abs(q32).max()
+    off = np.flatnonzero(np.abs(lq - q32) > 1e-3 * m)
+    assert len(off) <= 19, (cfg, off, lq[off], q32[off])
+    assert np.all(np.sign(lq[off]) == np.sign(q32[off])), (cfg, off, lq[off], q32[off])
+    assert np.all(np.abs(lq[off] - q32[off]) < 21.0), (cfg, off, lq[off], q32[off])
+    r64 = list(row)
+    r64[1] = int(out["iters"])
+    _, _, lq64, _ = oracle_mod.decode_soft(c, r64, soft, precision="f64")
+    assert np.abs(lq - lq64).max() <= np.abs(q32 - lq64).max() + 1e-3 * m, cfg
+
+
 @pytest.mark.parametrize("case", ALGO0, ids=case_id)
-def test_algo0_abi_tracks_fp64_oracle(L, oracle_mod, case):
-    """FP32 BP on the GPU vs the FP64 oracle, same iteration count forced:
-    posteriors within 1e-3 * max|LLR| (tolerance of the float path)."""
-    if not L.ldpc_mi355_algo_available(0):
-        pytest.skip("FP32 BP kernel disabled: its first GPU run did not return and no "
-                    "defect has been found in its source yet")
+def test_algo0_abi_matches_reference(L, oracle_mod, case):
+    """FP32 BP on the GPU vs the reference fixtures: HARQ buffer exact;
+    status, iterations and hard decisions equal; posteriors as
+    _bp_matches_reference states."""
     row = list(case["row"])
     out = abi_decode(L, case["llr"], case["cfg"], row, case["harq_in"])
     n = len(case["soft"])
     assert np.array_equal(out["soft"][:n], case["soft"])
-    assert out["status"] == case["status"]
     if row[1] == 0:
+        assert out["status"] == case["status"] and out["iters"] == 0
         return
-    # FP64 oracle run with exactly the iterations the GPU executed
-    c = oracle_mod.code(*case["cfg"])
-    r64 = list(row)
-    r64[1] = out["iters"]
-    st64, hard64, lq64, it64 = oracle_mod.decode_soft(c, r64, case["soft"], precision="f64")
-    lq = out["lq"][:n].astype(np.float64)
-    tol = 1e-3 * np.abs(lq64).max()
-    if case["status"] == 0:
-        assert np.array_equal(out["hard"], case["hard"])
-        assert np.max(np.abs(lq - lq64)) <= tol
-    else:
-        # non-converged: compare the first iteration only (BP is chaotic)
-        r1 = list(row)
-        r1[1] = 1
-        o1 = abi_decode(L, case["llr"], case["cfg"], r1, case["harq_in"])
-        _, _, lq64_1, _ = oracle_mod.decode_soft(c, r1, case["soft"], precision="f64")
-        assert np.max(np.abs(o1["lq"][:n] - lq64_1)) <= 1e-3 * np.abs(lq64_1).max()
+    _bp_matches_reference(oracle_mod, out, case["cfg"], row, case["soft"],
+                          ref32=(case["status"], case["iters"], case["hard"], case["lq"][:n]))
 
 
-def test_algo0_disabled_fails_loudly(L):
-    """While the BP kernel is compiled out, algo 0 returns -1 (no fallback),
-    after rate recovery, like an unsupported algorithm."""
-    if L.ldpc_mi355_algo_available(0):
-        pytest.skip("BP kernel enabled in the library")
+def test_algo0_random_llrs_bg2_zc26(L, oracle_mod):
+    """BG2 Zc=26 (A=176, CRC24A, E=512, QPSK) with random LLRs, decoder row
+    [0, 8, 1, 0, 1280, 20, -32768]: no CRC pass and all 8 iterations, as in
+    the reference; the soft buffer is the reference's rate recovery.  With
+    |LLR| <= 3.9 no tanh saturates in the first iteration, and there the GPU
+    posteriors are within 1e-3 * max|LLR| of FP64 outright."""
+    from oracle import oracle as O
     rng = np.random.default_rng(2)
     llr = rng.integers(-500, 500, 512).astype(np.int16)
-    hard = np.full(25, 0x5A, np.uint8)
     cfg = (2, 176, 1, 512, 0, 2, 0)
-    st = L.c_ldpc_5g_decode(ptr(llr), ptr(hard), ctypes.byref(LdpcInConfig(*cfg)),
-                            ctypes.byref(DecoderConfig(0, 8, 1, 0, 1280, 20, -32768)))
-    assert st == -1
-    assert np.all(hard == 0x5A)
-    from oracle import oracle as O
+    row = [0, 8, 1, 0, 1280, 20, -32768]
+    print(f"algo 0 cfg={cfg} row={row}", flush=True)
+    out = abi_decode(L, llr, cfg, row)
     want = O.rate_recover(O.code(*cfg), 1280, llr)
-    assert np.array_equal(harq_view(L)[:len(want)], want)
+    assert np.array_equal(out["soft"][:len(want)], want)
+    assert out["status"] == 1 and out["iters"] == 8
+    r1 = list(row)
+    r1[1] = 1
+    o1 = abi_decode(L, llr, cfg, r1)
+    _bp_matches_reference(oracle_mod, o1, cfg, r1, want)
+    _, _, lq64, _ = oracle_mod.decode_soft(O.code(*cfg), r1, want, precision="f64")
+    n = len(want)
+    assert np.abs(o1["lq"][:n] - lq64).max() <= 1e-3 * np.abs(lq64).max()
 
 
 def test_invalid_config_and_algo(L):
@@ -190,6 +224,29 @@ def test_batch_plan_matches_oracle(L, oracle_mod, row):
             assert np.array_equal(got, q), (i, cfg)
         else:
             assert np.array_equal(got.view(np.uint32), q.view(np.uint32)), (i, cfg)
+    plan.close()
+
+
+def test_batch_plan_bp_tracks_fp64(L, oracle_mod):
+    """Algo 0 (FP32 BP) in one mixed-size launch (BG1/BG2, Zc 2..384, plan
+    message store): every codeblock as _bp_matches_reference states."""
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    row = [0, 12, 1, 0, 1280, 20, -32768]
+    rng = np.random.default_rng(300)
+    cfgs, llrs = _random_batch(rng, 24, MIXED, noise_span=(-1.0, 2.0))
+    print("algo 0 mixed plan:", cfgs, flush=True)
+    plan = BatchPlan(cfgs, row, want_lq=True)
+    out = plan.decode(torch.from_numpy(np.concatenate(llrs)).cuda())
+    hard = plan.split_hard(out["hard"].cpu().numpy())
+    lq = plan.split_lq(out["lq"].cpu().numpy())
+    st = out["status"].cpu().numpy()
+    its = out["iters"].cpu().numpy()
+    for i, cfg in enumerate(cfgs):
+        c = oracle_mod.code(*cfg)
+        soft = oracle_mod.rate_recover(c, row[4], llrs[i])
+        o = dict(status=int(st[i]), iters=int(its[i]), hard=hard[i], lq=lq[i])
+        _bp_matches_reference(oracle_mod, o, cfg, row, soft)
     plan.close()
 
 

@@ -3,7 +3,7 @@
 fixtures and the oracle.  This checks the kernels' indexing, schedule, CRC
 reduction and fused rate recovery without a GPU; the GPU tests check the
 compiled gfx950 code.  With glibc tanhf/atanhf on the host, the BP kernel's
-source is bit-exact with the reference's FP32 BP here too.
+source (ldpc_bp.hip) is bit-exact with the reference's FP32 BP here too.
 """
 import os
 import sys
@@ -64,6 +64,26 @@ def test_emu_algo4_matches_reference(case):
 @pytest.mark.parametrize("case", [c for c in ALGO0 if _small(c)], ids=case_id)
 def test_emu_algo0_matches_reference(case):
     _run_case(case)
+
+
+def test_emu_algo0_full_size_case():
+    case = next(c for c in ALGO0 if c["cfg"][1] == 8424)
+    _run_case(case)
+
+
+def test_emu_algo0_random_llrs_bg2_zc26(oracle_mod):
+    """BG2 Zc=26 (A=176, CRC24A, E=512) with random LLRs, 8 BP iterations
+    without a CRC pass: the kernel source equals the oracle's FP32 BP."""
+    O = oracle_mod
+    rng = np.random.default_rng(2)
+    llr = rng.integers(-500, 500, 512).astype(np.int16)
+    cfg = (2, 176, 1, 512, 0, 2, 0)
+    row = [0, 8, 1, 0, 1280, 20, -32768]
+    out = emu.decode_batch([cfg], row, [llr])[0]
+    ref = O.decode_path(llr, cfg, row)
+    assert out["status"] == ref["status"] == 1 and out["iters"] == ref["iters"] == 8
+    assert np.array_equal(out["hard"], ref["hard"])
+    assert np.array_equal(out["lq"].view(np.uint32), ref["lq"].view(np.uint32))
 
 
 @pytest.fixture
