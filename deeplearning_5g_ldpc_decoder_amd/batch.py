@@ -83,6 +83,7 @@ class BatchPlan:
             off += c[3]
             arr[i].soft_slot = -1 if soft_slots is None else int(soft_slots[i])
         self.llr_elems = max(o + c[3] for o, c in zip(self.llr_offsets, self.cfgs))
+        self.max_slot = -1 if soft_slots is None else max(int(v) for v in soft_slots)
         self._handle = ctypes.c_void_p()
         bad = ctypes.c_int32(-1)
         dec = _decoder(self.row)
@@ -134,6 +135,14 @@ class BatchPlan:
             raise ValueError("llr must be int16 with every codeblock's E values")
         if not llr.is_cuda:
             raise ValueError("llr must be a device tensor")
+        if self.max_slot >= 0:
+            need = (self.max_slot + 1) * soft_slot_elems()
+            if soft_pool is None or not soft_pool.is_cuda or soft_pool.dtype.itemsize != 2:
+                raise ValueError("codeblocks use soft-buffer slots: soft_pool must be an int16 "
+                                 "device tensor")
+            if soft_pool.numel() < need:
+                raise ValueError(f"soft_pool holds {soft_pool.numel()} elements; slot "
+                                 f"{self.max_slot} needs {need}")
         if stream is None:
             stream = torch.cuda.current_stream(llr.device).cuda_stream
         st = _lib.load().ldpc_mi355_plan_decode(

@@ -135,6 +135,12 @@ void bind_scale(DecParams &dp, const ScaleSet &s, const uint32_t *dev) {
 
 DecParams make_params(const ldpc_decoder_cfg_t &c) { return decoder_params(c); }
 
+void report_missing_soft_pool(int max_slot) {
+    fprintf(stderr,
+            "libnrldpc_mi355: codeblocks use soft-buffer slots up to %d but soft_pool is NULL\n",
+            max_slot);
+}
+
 CbDesc make_desc(const Params &p, const Runtime &r) {
     return codeblock_desc(p, r.etab_off[p.bgn - 1][p.Zc]);
 }
@@ -244,6 +250,7 @@ extern "C" int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_con
 // ---------------------------------------------------------------------------
 struct ldpc_mi355_plan {
     int n = 0, algo = 1, threads = 64, max_soft = 0;
+    int max_slot = -1;             // largest soft-buffer slot of its codeblocks, -1 for none
     uint32_t uniform = 0;          // (bgn << 16) | Zc of a single-size batch, else 0
     DecParams dp;
     CbDesc *d_desc = nullptr;
@@ -294,6 +301,7 @@ struct BatchLayout {
     std::vector<uint32_t> hard_off, lq_off;
     uint64_t hard_bytes = 0, lq_elems = 0, work = 0;
     int max_zc = 2, max_soft = 0;
+    int max_slot = -1;      // largest soft-buffer slot a codeblock uses, -1 for none
     uint32_t uniform = 0;   // (bgn << 16) | Zc when every codeblock shares them, else 0
 };
 
@@ -305,6 +313,7 @@ int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, cons
     L.hard_bytes = L.lq_elems = L.work = 0;
     L.max_zc = 2;
     L.max_soft = 0;
+    L.max_slot = -1;
     for (int i = 0; i < n; ++i) {
         Params p;
         if (const char *why = derive(cbs[i].cfg, p)) {
@@ -329,6 +338,7 @@ int build_layout(const ldpc_mi355_cb_t *cbs, int n, int algo, bool want_lq, cons
         L.hard_bytes += (uint64_t)p.B / 8;
         L.lq_elems += (uint64_t)(p.N + 2 * p.Zc);
         if (algo == 0) L.work += (uint64_t)p.edges * p.Zc;
+        if (cbs[i].soft_slot > L.max_slot) L.max_slot = cbs[i].soft_slot;
         if (p.Zc > L.max_zc) L.max_zc = p.Zc;
         if (p.N + 2 * p.Zc > L.max_soft) L.max_soft = p.N + 2 * p.Zc;
         const uint32_t u = uniform_pair(p.bgn, p.Zc);
@@ -378,6 +388,7 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
     pl->hard_bytes = L.hard_bytes;
     pl->lq_elems = L.lq_elems;
     pl->max_soft = L.max_soft;
+    pl->max_slot = L.max_slot;
     pl->threads = threads_for(L.max_zc);
     pl->uniform = (flags & 8u) ? 0u : L.uniform;
     if (hipMalloc(&pl->d_desc, sizeof(CbDesc) * (size_t)n) != hipSuccess ||
@@ -436,6 +447,10 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
                                           int32_t *iterations, void *lq, void *stream) {
     if (!plan) return -1;
     if (plan->want_lq && !lq) return -1;
+    if (plan->max_slot >= 0 && !soft_pool) {
+        report_missing_soft_pool(plan->max_slot);
+        return -4;
+    }
     Runtime &r = rt();
     if (r.ensure() != 0) return -1;
     DecOut o;
@@ -563,6 +578,10 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
     if (int st = build_layout(cbs, n, b->algo, b->want_lq, r,
                               table_algo(b->algo) ? &b->scale : nullptr, b->max_it, L, bad_index))
         return st;
+    if (L.max_slot >= 0 && !soft_pool) {
+        report_missing_soft_pool(L.max_slot);
+        return -4;
+    }
     const size_t doff = 0;
     const size_t toff = ((sizeof(CbDesc) * (size_t)b->capacity) + 255) & ~(size_t)255;
     const size_t ooff = toff + (((sizeof(TileDesc) * (size_t)b->capacity) + 255) & ~(size_t)255);

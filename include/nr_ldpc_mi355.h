@@ -246,7 +246,10 @@ int32_t ldpc_mi355_plan_layout(const ldpc_mi355_plan_t *plan, uint64_t *hard_byt
  * stream).  All pointers are device pointers: llr (int16), soft_pool (int16,
  * slots of 68*384 elements; may be NULL if no codeblock uses a slot),
  * hard (hard_bytes), status and iterations (n int32 each), lq (optional).
- * Asynchronous and allocation-free (capturable in a hipGraph). */
+ * Asynchronous and allocation-free (capturable in a hipGraph).  Returns 0;
+ * -4 (nothing launched) when a codeblock has a soft slot and soft_pool is
+ * NULL; -1 when posteriors were asked for and lq is NULL, or on a GPU error.
+ * soft_pool must hold (largest slot + 1) * 68*384 elements. */
 int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const int16_t *llr,
                                int16_t *soft_pool, uint8_t *hard, int32_t *status,
                                int32_t *iterations, void *lq, void *stream);
@@ -273,8 +276,8 @@ int32_t ldpc_mi355_set_tile_codeblocks(int32_t bgn, int32_t zc, int32_t k);
  * blocks.  flags as for plans (bit 0 posteriors, bit 1 pack, bit 2 never
  * pack).  Algos 1-4; algos 2/3 keep the scale tables current at creation.
  * Returns 0, -2 invalid codeblock config or algo-2 iterations beyond the
- * table (*bad_index), -3 unsupported algo, -4 invalid argument, -1 GPU
- * error. */
+ * table (*bad_index), -3 unsupported algo, -4 invalid argument (including a
+ * codeblock with a soft slot while soft_pool is NULL), -1 GPU error. */
 typedef struct ldpc_mi355_batch ldpc_mi355_batch_t;
 int32_t ldpc_mi355_batch_create(ldpc_mi355_batch_t **batch, int32_t capacity,
                                 const ldpc_decoder_cfg_t *decoder_config, uint32_t flags);
@@ -299,9 +302,7 @@ void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *batch);
 int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src, const uint64_t *bytes,
                               int32_t n, int32_t kind, void *stream);
 
-/* 1 when the library runs decoder algo `algo` on the GPU, else 0 (algo 0,
- * FP32 BP, is compiled out of builds whose BP kernel is disabled; its calls
- * then return -1 with a message instead of decoding). */
+/* 1 when the library runs decoder algo `algo` (0-4) on the GPU, else 0. */
 int32_t ldpc_mi355_algo_available(int32_t algo);
 
 /* Blocks until the library stream is idle.  0 or -1. */

@@ -314,3 +314,37 @@ def test_rate_recovery_abi_matches_reference(L):
                                 ctypes.byref(DecoderConfig(1, 0, 22936, 0, filler, 32767, -32767)))
         assert st == 1
         assert np.array_equal(hv[:len(want)], want), cfg
+
+
+def test_missing_soft_pool_refused_before_launch(L):
+    """A plan or batch context whose codeblocks use soft-buffer slots refuses a
+    decode without a soft pool (-4 from the C ABI, ValueError from BatchPlan)
+    on the host: nothing is launched."""
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan, CbDescriptor
+    cfg = (2, 1016, 2, 3120, 0, 2, 0)
+    plan = BatchPlan([cfg, cfg], ROW1, soft_slots=[-1, 3])
+    llr = torch.zeros(2 * 3120, dtype=torch.int16, device="cuda")
+    out = plan.alloc_outputs()
+    rc = L.ldpc_mi355_plan_decode(plan._handle, ptr(llr), None, ptr(out["hard"]),
+                                  ptr(out["status"]), ptr(out["iters"]), None, None)
+    assert rc == -4
+    with pytest.raises(ValueError):
+        plan.launch(llr, out)
+    small = torch.zeros(3 * 68 * 384, dtype=torch.int16, device="cuda")   # slots 0..2 only
+    with pytest.raises(ValueError):
+        plan.launch(llr, out, soft_pool=small)
+    plan.close()
+    b = ctypes.c_void_p()
+    assert L.ldpc_mi355_batch_create(ctypes.byref(b), 4, ctypes.byref(DecoderConfig(*ROW1)),
+                                     0) == 0
+    arr = (CbDescriptor * 1)()
+    arr[0].cfg = LdpcInConfig(*cfg)
+    arr[0].llr_offset = 0
+    arr[0].soft_slot = 0
+    bad = ctypes.c_int32(-1)
+    rc = L.ldpc_mi355_batch_decode(b, arr, 1, ptr(llr), None, ptr(out["hard"]),
+                                   ptr(out["status"]), ptr(out["iters"]), None, None,
+                                   ctypes.byref(bad))
+    assert rc == -4
+    L.ldpc_mi355_batch_destroy(b)
