@@ -29,6 +29,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_copy_spans",
     "ldpc_mi355_set_trained_tables", "ldpc_mi355_get_trained_tables",
     "ldpc_mi355_set_degree_alpha", "ldpc_mi355_get_degree_alpha",
+    "ldpc_mi355_bp_decode_noalloc",
 )
 EXPORTED_DATA = (
     "BG1_HBG_offset_array", "BG1_HBG_array", "BG1_HBG_Vij_array",
@@ -135,6 +136,10 @@ def load():
     L.ldpc_mi355_set_tile_codeblocks.restype = ctypes.c_int32
     L.ldpc_mi355_algo_available.argtypes = [ctypes.c_int32]
     L.ldpc_mi355_algo_available.restype = ctypes.c_int32
+    L.ldpc_mi355_bp_decode_noalloc.argtypes = [ctypes.POINTER(LdpcInConfig),
+                                               ctypes.POINTER(DecoderConfig), ctypes.c_int32,
+                                               P, P, P, P, P, P, P]
+    L.ldpc_mi355_bp_decode_noalloc.restype = ctypes.c_int32
     L.ldpc_mi355_version.argtypes = []
     L.ldpc_mi355_version.restype = ctypes.c_char_p
     L.ldpc_mi355_synth_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32,

@@ -19,7 +19,7 @@
 //     compiler outlined into 240-370 KB device functions; in the BG2 one its
 //     long-branch expansion reused the return-address registers s[30:31], so
 //     the function's return jumped back into its own exit block and never
-//     left it.  tests/test_bp_codegen.py checks the compiled kernel for calls,
+//     left it.  tests/test_codegen.py checks the compiled kernels for calls,
 //     long branches and scratch.)
 //
 // BP messages have no compressed form, so each codeblock keeps its nnz * Zc
@@ -164,14 +164,9 @@ __device__ __forceinline__ void bp_row(int deg, float *lq, float *lr, bp_etab_pt
 // One codeblock per workgroup (blockDim: Zc rounded up to whole waves).
 // Fused rate recovery + HARQ combine into float beliefs (/128,
 // c_lib/floatpoint_ldpc_decoder.c:251-254), then iterations of all rows with
-// the parallel CRC after each (:286-294).
-__global__ void __launch_bounds__(kMaxThreads) ldpc_bp_kernel(DecOut o, DecParams p) {
-    NRLDPC_DYNAMIC_LDS(smem);
-    __shared__ uint32_t crc_tab[256];
-    __shared__ uint32_t slots[kMaxThreads / 64];
-    float *lq = reinterpret_cast<float *>(smem);
-
-    const CbDesc d = o.desc[blockIdx.x];
+// the parallel CRC after each (:286-294).  Inlined into both kernels below.
+__device__ __forceinline__ void bp_decode(const CbDesc &d, const DecOut &o, const DecParams &p,
+                                          float *lq, uint32_t *crc_tab, uint32_t *slots) {
     const int n = d.N + 2 * d.Zc;
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
@@ -214,6 +209,34 @@ __global__ void __launch_bounds__(kMaxThreads) ldpc_bp_kernel(DecOut o, DecParam
     }
 }
 
+// Codeblock blockIdx.x of a batch, its descriptor in o.desc.
+__global__ void __launch_bounds__(kMaxThreads) ldpc_bp_kernel(DecOut o, DecParams p) {
+    NRLDPC_DYNAMIC_LDS(smem);
+    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t slots[kMaxThreads / 64];
+    const CbDesc d = o.desc[blockIdx.x];
+    bp_decode(d, o, p, reinterpret_cast<float *>(smem), crc_tab, slots);
+}
+
+// Codeblock i = blockIdx.x of a batch of one config: the descriptor is
+// `base` with its buffers at codeblock i's place -- LLRs at i * E, hard bytes
+// at i * B / 8, soft slot i (when base has one), posteriors at i * (N + 2Zc),
+// messages at i * nnz * Zc.
+__global__ void __launch_bounds__(kMaxThreads)
+ldpc_bp_uniform_kernel(DecOut o, DecParams p, CbDesc base, uint32_t edges) {
+    NRLDPC_DYNAMIC_LDS(smem);
+    __shared__ uint32_t crc_tab[256];
+    __shared__ uint32_t slots[kMaxThreads / 64];
+    const uint32_t i = blockIdx.x;
+    CbDesc d = base;
+    d.llr_off = i * d.E;
+    d.hard_off = i * (d.B / 8u);
+    if (base.soft_off != kNone) d.soft_off = i * (uint32_t)kMaxSoft;
+    if (base.lq_off != kNone) d.lq_off = i * (uint32_t)(d.N + 2 * d.Zc);
+    d.work_off = i * edges * d.Zc;
+    bp_decode(d, o, p, reinterpret_cast<float *>(smem), crc_tab, slots);
+}
+
 }  // namespace nrldpc
 
 #ifndef NRLDPC_CPU_EMU
@@ -237,5 +260,176 @@ extern "C" hipError_t nrldpc_launch_bp(const nrldpc::DecOut *o, const nrldpc::De
                                          dim3(ncb), dim3(threads), args, lds, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free BP decode (ldpc_mi355_bp_decode_noalloc).  The messages
+// live in `scratch`, a file-scope device array (one per GPU, loaded with the
+// code object and held until the process ends), and the edge and CRC tables
+// in `noalloc_tables`.  In host code these names are host-side stand-ins,
+// not device addresses: every host use goes through the per-device pointers
+// from hipGetSymbolAddress.
+// ---------------------------------------------------------------------------
+#include <stdio.h>
+
+#include <mutex>
+#include <vector>
+
+#include "ldpc_host.h"
+
+namespace nrldpc {
+constexpr size_t kScratchFloats = (size_t)8 << 20;   // 32 MiB of float messages
+constexpr int kNumLiftingSizes = 51;
+constexpr size_t kNoAllocTableWords =
+    (size_t)kNumLiftingSizes * (NR_BG1_EDGES + NR_BG2_EDGES) + 4 * (size_t)kCrcTabWords;
+}  // namespace nrldpc
+
+static __device__ __align__(256) float scratch[nrldpc::kScratchFloats];
+static __device__ __align__(256) uint32_t noalloc_tables[nrldpc::kNoAllocTableWords];
+
+namespace {
+using namespace nrldpc;
+
+// Host image of noalloc_tables (the same on every GPU).
+struct NoAllocTables {
+    std::vector<uint32_t> words;
+    uint32_t etab_off[2][kMaxZc + 1] = {};
+    uint32_t crc_off = 0;
+    NoAllocTables() {
+        int nz = 0;
+        const uint16_t *zs = all_lifting_sizes(&nz);
+        for (int bg = 1; bg <= 2; ++bg) {
+            const int ne = bg == 1 ? NR_BG1_EDGES : NR_BG2_EDGES;
+            for (int i = 0; i < nz; ++i) {
+                etab_off[bg - 1][zs[i]] = (uint32_t)words.size();
+                words.resize(words.size() + ne);
+                edge_table(bg, zs[i], lifting_set(zs[i]), words.data() + etab_off[bg - 1][zs[i]]);
+            }
+        }
+        crc_off = (uint32_t)words.size();
+        words.resize(words.size() + 4 * (size_t)kCrcTabWords);
+        crc_device_tables(words.data() + crc_off);
+    }
+};
+
+const NoAllocTables &noalloc_host_tables() {
+    static const NoAllocTables t;
+    return t;
+}
+
+// Device addresses of scratch and noalloc_tables on one GPU.
+struct NoAllocDevice {
+    bool ready = false;
+    float *scratch = nullptr;
+    uint32_t *tables = nullptr;
+};
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+#define NOALLOC_HIP(expr)                                                              \
+    do {                                                                               \
+        const hipError_t _e = (expr);                                                  \
+        if (_e != hipSuccess) {                                                        \
+            fprintf(stderr, "libnrldpc_mi355: %s failed: %s\n", #expr,                \
+                    hipGetErrorString(_e));                                            \
+            return nullptr;                                                            \
+        }                                                                              \
+    } while (0)
+
+// Once per GPU, with that GPU current: the symbol addresses, the tables and
+// the kernel's LDS limit.
+const NoAllocDevice *noalloc_device(int dev) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> guard(g_noalloc_mu);
+    NoAllocDevice &d = g_noalloc[dev];
+    if (d.ready) return &d;
+    const NoAllocTables &t = noalloc_host_tables();
+    if (t.words.size() != kNoAllocTableWords) {
+        fprintf(stderr, "libnrldpc_mi355: table layout mismatch\n");
+        return nullptr;
+    }
+    void *ps = nullptr, *pt = nullptr;
+    NOALLOC_HIP(hipGetSymbolAddress(&ps, HIP_SYMBOL(scratch)));
+    NOALLOC_HIP(hipGetSymbolAddress(&pt, HIP_SYMBOL(noalloc_tables)));
+    NOALLOC_HIP(hipMemcpy(pt, t.words.data(), t.words.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    NOALLOC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&ldpc_bp_uniform_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024 - 2048));
+    d.scratch = static_cast<float *>(ps);
+    d.tables = static_cast<uint32_t *>(pt);
+    d.ready = true;
+    return &d;
+}
+}  // namespace
+
+extern "C" int32_t ldpc_mi355_bp_decode_noalloc(const ldpc_in_config_t *cfg,
+                                                const ldpc_decoder_cfg_t *decoder_config,
+                                                int32_t n, const int16_t *llr,
+                                                int16_t *soft_pool, uint8_t *hard,
+                                                int32_t *status, int32_t *iterations, float *lq,
+                                                void *stream) {
+    using namespace nrldpc;
+    if (!cfg || !decoder_config || n <= 0 || !llr || !hard || !status || !iterations) return -4;
+    if (decoder_config->algo != 0) {
+        fprintf(stderr, "libnrldpc_mi355: the allocation-free decode runs algo 0 (FP32 BP) only, "
+                        "not algo %u\n", decoder_config->algo);
+        return -3;
+    }
+    Params p;
+    if (const char *why = derive(*cfg, p)) {
+        print_invalid(*cfg, why);
+        return -2;
+    }
+    if (p.harq_on && !soft_pool) {
+        fprintf(stderr, "libnrldpc_mi355: HARQ_on needs soft-buffer slots (soft_pool is NULL)\n");
+        return -4;
+    }
+    const size_t per_cb = (size_t)p.edges * (size_t)p.Zc;
+    if ((size_t)n * per_cb > kScratchFloats) {
+        fprintf(stderr,
+                "libnrldpc_mi355: %d codeblocks of BG%d Zc=%d need %zu message floats; the "
+                "allocation-free decode holds %zu (at most %zu codeblocks of this code per call)\n",
+                n, p.bgn, p.Zc, (size_t)n * per_cb, kScratchFloats, kScratchFloats / per_cb);
+        return -4;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    const NoAllocDevice *nd = noalloc_device(dev);
+    if (!nd) return -1;
+    const NoAllocTables &t = noalloc_host_tables();
+
+    DecOut o;
+    o.desc = nullptr;
+    o.llr = llr;
+    o.soft = soft_pool;
+    o.hard = hard;
+    o.status = status;
+    o.iters = iterations;
+    o.lq = lq;
+    o.work = nd->scratch;
+    o.etab = nd->tables;
+    o.crc = nd->tables + t.crc_off;
+    DecParams dp = decoder_params(*decoder_config);
+    CbDesc base = codeblock_desc(p, t.etab_off[p.bgn - 1][p.Zc]);
+    base.llr_off = 0;
+    base.hard_off = 0;
+    base.work_off = 0;
+    base.soft_off = soft_pool ? 0u : kNone;
+    base.lq_off = lq ? 0u : kNone;
+    uint32_t edges = (uint32_t)p.edges;
+    const size_t lds = ((size_t)(p.N + 2 * p.Zc) * sizeof(float) + 15) & ~(size_t)15;
+    void *args[] = {&o, &dp, &base, &edges};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(&ldpc_bp_uniform_kernel),
+                                   dim3(n), dim3(workgroup_threads(p.Zc)), args, lds,
+                                   reinterpret_cast<hipStream_t>(stream));
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "libnrldpc_mi355: allocation-free BP launch failed: %s\n",
+                hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
 }
 #endif  // NRLDPC_CPU_EMU

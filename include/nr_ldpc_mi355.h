@@ -305,6 +305,32 @@ int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src, const ui
 /* 1 when the library runs decoder algo `algo` (0-4) on the GPU, else 0. */
 int32_t ldpc_mi355_algo_available(int32_t algo);
 
+/* Allocation-free FP32 BP decode.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Decodes n
+ * codeblocks of one config *cfg with decoder row *decoder_config (algo 0
+ * only) like ldpc_mi355_plan_create + ldpc_mi355_plan_decode on a plan whose
+ * codeblock i has LLR offset i*E and soft slot i, and gives the same
+ * results, but calls no allocator: the check-to-variable messages live in a
+ * 32 MiB device array of the library (one per GPU, held until the process
+ * ends), so n * nnz(BG) * Zc * 4 bytes must fit in it (nnz 316 for BG1, 197
+ * for BG2: n <= 69 at BG1 Zc=384, n <= 409 at BG2 Zc=104); larger n returns
+ * -4 with a message.  Its first call on a GPU sets up that GPU's tables
+ * (one synchronous copy), later calls only launch.  Device pointers: llr
+ * (n*E int16), soft_pool (slots of 68*384 int16, slot i for codeblock i; may
+ * be NULL unless HARQ_on), hard (n*B/8 bytes), status and iterations (n
+ * int32 each), lq (n*(N+2Zc) float posteriors, or NULL).  Asynchronous on
+ * `stream` (a hipStream_t; NULL = HIP's null stream).  Every call on one GPU
+ * shares its message array: calls that overlap on a GPU (other streams,
+ * other host threads) race on it and must be ordered by the caller.
+ * Returns 0, -2 invalid config, -3 algo other than 0, -4 invalid argument or
+ * n too large, -1 GPU error. */
+int32_t ldpc_mi355_bp_decode_noalloc(const ldpc_in_config_t *cfg,
+                                     const ldpc_decoder_cfg_t *decoder_config, int32_t n,
+                                     const int16_t *llr, int16_t *soft_pool, uint8_t *hard,
+                                     int32_t *status, int32_t *iterations, float *lq,
+                                     void *stream);
+
 /* Blocks until the library stream is idle.  0 or -1. */
 int32_t ldpc_mi355_sync(void);
 
