@@ -42,8 +42,12 @@ BATCH = 256
 NOISE_DB = -1.0        # Es/N0 = 1 dB with unit-energy symbols
 EDGES_BG1 = 316
 HBM_PEAK = 8.0e12
+CLOCK_HZ = 2.4e9       # MI355X peak engine clock
+SIMDS = 256 * 4        # 256 CUs x 4 SIMDs
+# VALU issue: a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
+VALU_PEAK = SIMDS * CLOCK_HZ / 4    # wave-instructions per second
 # LDS: 128 B per clock per CU (CDNA3 figure, taken for gfx950) x 256 CUs x 2.4 GHz
-LDS_PEAK = 128 * 256 * 2.4e9
+LDS_PEAK = 128 * 256 * CLOCK_HZ
 
 
 def algorithmic_bytes(iters, E=25344, B=8448, zc=384, w=8):
@@ -60,12 +64,58 @@ def lds_bytes(iters, zc=384):
     return int(np.sum(iters * 4 * EDGES_BG1 * zc))
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_minsum_i16_bg1_zc384_b256.json")
+def load_profile(name):
+    path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
-    return None
+            return json.load(f)
+    return {}
+
+
+def load_traffic():
+    return load_profile("hbm_traffic_minsum_i16_bg1_zc384_b256.json").get(
+        "traffic_bytes_per_launch")
+
+
+def valu_instructions(iters):
+    """Wave-VALU instructions of one launch of the specialized BG1 Zc=384
+    kernel: per codeblock (its 6 waves) a fixed part (rate recovery, CRC
+    setup, outputs) plus a per-iteration part, each a deterministic
+    instruction stream
+    (counted with SQ_INSTS_VALU at two iteration limits,
+    profiles/valu_counts_minsum_i16_bg1_zc384_b256.json), times the
+    iterations this launch executed."""
+    prof = load_profile("valu_counts_minsum_i16_bg1_zc384_b256.json")
+    if not prof:
+        return None
+    iters = np.asarray(iters, np.int64)
+    return (len(iters) * prof["per_codeblock_wave_instr"] +
+            int(iters.sum()) * prof["per_iteration_wave_instr"])
+
+
+def host_cpu():
+    """CPU model, logical CPUs visible to this process and physical cores."""
+    model, phys = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid, core = None, None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    core = v
+                elif not k and pid is not None:
+                    phys.add((pid, core))
+                    pid = core = None
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "logical_cpus_visible": visible,
+            "logical_cpus_host": os.cpu_count(), "physical_cores_host": len(phys) or None}
 
 
 def cpu_baseline(llr, threads):
@@ -428,11 +478,14 @@ def bench_bler(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--latency-reps", type=int, default=200)
+    ap.add_argument("--latency-reps", type=int, default=1000)
+    ap.add_argument("--max-iterations", type=int, default=None,
+                    help="with --profile only: decoder row iteration limit (to count the "
+                         "per-iteration and fixed VALU instructions of the kernel)")
     ap.add_argument("--inflight", type=int, default=2,
                     help="headline workload: launches in flight in the pipelined timed "
                          "region (one stream and output set each; 1 = single stream)")
@@ -455,6 +508,8 @@ def main():
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
+    if args.max_iterations is not None and not args.profile:
+        ap.error("--max-iterations changes the workload: only with --profile")
     if args.gpus != 1:
         print(f"bench.py: the decoder runs on one GPU; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would shard codeblocks by index, with no exchange)",
@@ -483,7 +538,10 @@ def main():
     torch.cuda.set_device(dev)
     llr, payload = synth_batch(CFG, n, NOISE_DB, seed=1, threads=os.cpu_count() or 8)
     d_llr = torch.from_numpy(llr.reshape(-1)).to(dev)
-    plan = BatchPlan([CFG] * n, ROW)
+    row = list(ROW)
+    if args.max_iterations is not None:
+        row[1] = args.max_iterations
+    plan = BatchPlan([CFG] * n, row)
     out = plan.alloc_outputs(dev)
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
@@ -562,16 +620,31 @@ def main():
         per_cb_gbps = n_pc * CFG[1] / (time.perf_counter() - t) / 1e9
 
     avg_kernel_s = float(kernel_ms.mean()) / 1e3
+    # the binding resource is VALU issue (decoder state stays in LDS and
+    # VGPRs; HBM carries little more than the compulsory I/O)
+    vi = valu_instructions(iters)
+    traffic = load_traffic()
     alg = algorithmic_bytes(iters)
-    achieved = alg / avg_kernel_s
-    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": load_traffic(),
-            "algorithmic_bytes_per_launch": alg,
-            "compulsory_bytes_per_launch": int(n * (2 * 25344 + 1056)),
+    roof = {"bound": "valu",
+            "achieved": vi / avg_kernel_s / 1e9 if vi else None,
+            "peak": VALU_PEAK / 1e9, "unit": "G wave-instr/s",
+            "frac": vi / avg_kernel_s / VALU_PEAK if vi else None,
+            "traffic": traffic,
+            "valu_wave_instr_per_launch": vi,
             "kernel_ms_avg": float(kernel_ms.mean()),
-            "note": "decoder state stays in LDS and VGPRs, so achieved is the effective rate "
-                    "against the reference layout's per-edge traffic and may exceed the HBM peak; "
-                    "measured HBM traffic is `traffic`, LDS belief traffic is `lds`",
+            "note": "achieved = counted wave-VALU instructions of this launch (fixed part per "
+                    "codeblock + per-iteration part x executed iterations) / kernel time; peak = "
+                    "1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction. One codeblock's 6 "
+                    "waves sit 2-2-1-1 on a CU's 4 SIMDs, so full issue on the busiest SIMDs is "
+                    "frac 0.75",
+            "hbm": {"achieved": traffic / avg_kernel_s / 1e9 if traffic else None,
+                    "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": traffic / avg_kernel_s / HBM_PEAK if traffic else None,
+                    "compulsory_bytes_per_launch": int(n * (2 * 25344 + 1056))},
+            "reference_layout_effective": {
+                "bytes_per_launch": alg, "GBps": alg / avg_kernel_s / 1e9,
+                "note": "the reference's per-edge LQ/Lr traffic (8 B per edge per iteration) "
+                        "over kernel time; those bytes never reach HBM here"},
             "lds": {"achieved": lds_bytes(iters) / avg_kernel_s / 1e9,
                     "peak": LDS_PEAK / 1e9, "unit": "GB/s",
                     "frac": lds_bytes(iters) / avg_kernel_s / LDS_PEAK}}
@@ -579,20 +652,33 @@ def main():
     log("per-codeblock path done")
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline")
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8
-        threads = max(1, min(threads, 16))
+        info = host_cpu()
+        # one GPU's share of the host: at most 16 worker threads on this box
+        threads = max(1, min(info["logical_cpus_visible"] or 1, 16))
+        log(f"cpu baseline: {threads} threads ({info})")
         reps = 3
         secs = 0.0
         for _ in range(reps):
-            s, cst, cits, chard = cpu_baseline(llr, threads)
-            secs += s
+            s_, cst, cits, chard = cpu_baseline(llr, threads)
+            secs += s_
         same = bool(np.array_equal(cst, status) and np.array_equal(cits, iters) and
                     np.array_equal(chard, hard))
+        n1 = 16
+        s1, _, _, _ = cpu_baseline(llr[:n1], 1)
+        single_mbps = n1 * CFG[1] / s1 / 1e6
         cpu = {"value": reps * info_bits / secs / 1e9, "unit": "Gbps", "cores": threads,
                "kind": "port",
-               "sample": f"{reps} passes over the same {n}-codeblock batch, one codeblock per "
-                         f"thread task; bit-exact with the GPU result: {same}"}
+               "sample": f"{reps} passes over the same {n}-codeblock batch on {threads} threads "
+                         f"(one codeblock per task, {reps * info_bits / secs / 1e6:.2f} Mbit/s; "
+                         f"bit-exact with the GPU result: {same}); single core: {n1} codeblocks "
+                         f"on 1 thread, {single_mbps:.3f} Mbit/s",
+               "single_core_mbps": single_mbps,
+               "all_threads_mbps": reps * info_bits / secs / 1e6,
+               "mean_iterations": float(cits.mean()),
+               "cpu_model": info["model"], "logical_cpus_visible": info["logical_cpus_visible"],
+               "logical_cpus_host": info["logical_cpus_host"],
+               "physical_cores_host": info["physical_cores_host"],
+               "threads_note": "one GPU's share of the host cores is 16 threads on this box"}
 
     line = {
         "metric": "decoded info Gbps (BG1 Zc=384, 8 it) + p99 decode latency, BLER-matched",
