@@ -15,6 +15,7 @@ CSRC_DIR = os.path.join(PKG_DIR, "csrc")
 EXPORTED_FUNCTIONS = (
     "calculate_crc", "gen_ldpc_parameters", "dequantize_int16_to_llr",
     "c_ldpc_5g_decode", "c_ldpc_5g_encode", "tx_rx_process",
+    "cuda_ldpc_5g_decode", "cuda_ldpc_5g_encode",
     "ldpc_mi355_decode_ex", "ldpc_mi355_channel_seed", "ldpc_mi355_plan_create",
     "ldpc_mi355_plan_layout", "ldpc_mi355_plan_decode", "ldpc_mi355_plan_destroy",
     "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
@@ -112,6 +113,11 @@ def load():
     L.ldpc_mi355_decode_ex.restype = ctypes.c_int32
     L.c_ldpc_5g_encode.argtypes = [P, P, ctypes.POINTER(LdpcInConfig)]
     L.c_ldpc_5g_encode.restype = ctypes.c_int32
+    L.cuda_ldpc_5g_decode.argtypes = [P, P, ctypes.POINTER(LdpcInConfig),
+                                      ctypes.POINTER(DecoderConfig)]
+    L.cuda_ldpc_5g_decode.restype = ctypes.c_int32
+    L.cuda_ldpc_5g_encode.argtypes = [P, P, ctypes.POINTER(LdpcInConfig)]
+    L.cuda_ldpc_5g_encode.restype = ctypes.c_int32
     L.tx_rx_process.argtypes = [P, ctypes.c_size_t, ctypes.c_uint16, ctypes.c_float, P]
     L.tx_rx_process.restype = ctypes.c_int
     L.ldpc_mi355_channel_seed.argtypes = [ctypes.c_uint64]

@@ -540,6 +540,14 @@ extern "C" int32_t c_ldpc_5g_encode(uint8_t *infobytes, uint8_t *rm_bytes,
     return 0;
 }
 
+// CUDA/cuda_ldpc_5g_encoder.cu:47-84: the CUDA library's top-level encoder
+// produces the same bytes as c_ldpc_5g_encode (CRC attach, encode, rate
+// match); bound by scripts/sim_cuda_ldpc_encoder_decoder.py:65.
+extern "C" int32_t cuda_ldpc_5g_encode(uint8_t *infobytes, uint8_t *rm_bytes,
+                                       ldpc_in_config_t *inconfig) {
+    return c_ldpc_5g_encode(infobytes, rm_bytes, inconfig);
+}
+
 // ---------------------------------------------------------------------------
 // Test channel (c_lib/tx_rx_process.c:286-324).  Gray QAM of TS 38.211
 // 5.1.2-5.1.6: bits 0,2,4,.. drive the I axis and 1,3,5,.. the Q axis, each

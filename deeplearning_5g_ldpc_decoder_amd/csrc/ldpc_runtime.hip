@@ -154,10 +154,42 @@ uint32_t uniform_pair(int bgn, int zc) { return ((uint32_t)bgn << 16) | (uint32_
 // ---------------------------------------------------------------------------
 // Per-codeblock path (the reference ABI): one H2D, one launch, one D2H.
 // ---------------------------------------------------------------------------
+namespace {
+// The reference's c_lib and CUDA decoder wrappers differ only in which algo
+// IDs they accept and what they run for them; everything else is this path.
+enum class AlgoMap { kCLib, kCuda };
+
+int32_t decode_one(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                   ldpc_decoder_cfg_t *decoder_config, int32_t *iterations, void *posteriors,
+                   AlgoMap map);
+}  // namespace
+
 extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
                                         ldpc_in_config_t *inconfig,
                                         ldpc_decoder_cfg_t *decoder_config, int32_t *iterations,
                                         void *posteriors) {
+    return decode_one(LLr, hardbits, inconfig, decoder_config, iterations, posteriors,
+                      AlgoMap::kCLib);
+}
+
+extern "C" int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                                    ldpc_decoder_cfg_t *decoder_config) {
+    return decode_one(LLr, hardbits, inconfig, decoder_config, nullptr, nullptr, AlgoMap::kCLib);
+}
+
+// CUDA/cuda_ldpc_5g_decoder.cu:31-58: algo 1 is the int16 min-sum, algo 4 the
+// FP32 *BP* decoder (CUDA/floatpoint_ldpc_decoder.cu), anything else -3 after
+// rate recovery.
+extern "C" int32_t cuda_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits,
+                                       ldpc_in_config_t *inconfig,
+                                       ldpc_decoder_cfg_t *decoder_config) {
+    return decode_one(LLr, hardbits, inconfig, decoder_config, nullptr, nullptr, AlgoMap::kCuda);
+}
+
+namespace {
+int32_t decode_one(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                   ldpc_decoder_cfg_t *decoder_config, int32_t *iterations, void *posteriors,
+                   AlgoMap map) {
     if (iterations) *iterations = 0;
     Params p;
     if (const char *why = derive(*inconfig, p)) {
@@ -168,8 +200,10 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     if (r.ensure() != 0) return -1;
     std::lock_guard<std::mutex> guard(r.mu);
 
-    const bool valid_algo = algo_supported(decoder_config->algo);
-    const int algo = valid_algo ? decoder_config->algo : 1;
+    const int asked = decoder_config->algo;
+    const bool valid_algo =
+        map == AlgoMap::kCLib ? algo_supported(asked) : (asked == 1 || asked == 4);
+    const int algo = !valid_algo ? 1 : (map == AlgoMap::kCuda && asked == 4) ? 0 : asked;
     ScaleSet scale;
     if (table_algo(algo)) {
         make_scale_set(*decoder_config, scale);
@@ -231,7 +265,11 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     memcpy(&its, r.h_out + OutLayout::status + 4, 4);
     memcpy(HARQ_memory, r.h_out + OutLayout::soft, (size_t)n * 2);
     if (!valid_algo) {
-        fprintf(stderr, "Invalid decoding algorithm selection: %u\n", decoder_config->algo);
+        if (map == AlgoMap::kCLib)
+            fprintf(stderr, "Invalid decoding algorithm selection: %u\n", decoder_config->algo);
+        else
+            fprintf(stderr, "Unsupported CUDA LDPC decoding algorithm selection: %u\n",
+                    decoder_config->algo);
         return -3;
     }
     if (its > 0) memcpy(hardbits, r.h_out + OutLayout::hard, (size_t)p.B / 8);
@@ -240,10 +278,7 @@ extern "C" int32_t ldpc_mi355_decode_ex(int16_t *LLr, uint8_t *hardbits,
     return st;
 }
 
-extern "C" int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
-                                    ldpc_decoder_cfg_t *decoder_config) {
-    return ldpc_mi355_decode_ex(LLr, hardbits, inconfig, decoder_config, nullptr, nullptr);
-}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Batched plans

@@ -119,6 +119,20 @@ int32_t c_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inco
  * (Host code; the encoder buffer is sized 68*Zc, so Zc >= 373 is safe.) */
 int32_t c_ldpc_5g_encode(uint8_t *infobytes, uint8_t *rm_bytes, ldpc_in_config_t *inconfig);
 
+/* CUDA/cuda_ldpc_5g_decoder.cu:31 — the CUDA library's twin of
+ * c_ldpc_5g_decode, bound by scripts/sim_cuda_ldpc_encoder_decoder.py:66
+ * (decode_symbol="cuda_ldpc_5g_decode").  Same arguments, HARQ semantics and
+ * return codes, with the CUDA library's algo IDs: 1 is the int16 min-sum and
+ * 4 is FP32 layered *BP* (CUDA/cuda_ldpc_5g_decoder.cu:49-51 routes 4 to
+ * its float BP kernel, where c_lib's 4 is FP32 min-sum); any other algo,
+ * 0 included, returns -3 after rate recovery. */
+int32_t cuda_ldpc_5g_decode(int16_t *LLr, uint8_t *hardbits, ldpc_in_config_t *inconfig,
+                            ldpc_decoder_cfg_t *decoder_config);
+
+/* CUDA/cuda_ldpc_5g_encoder.cu:47 — the CUDA library's encoder symbol
+ * (encode_symbol="cuda_ldpc_5g_encode"); the same bytes as c_ldpc_5g_encode. */
+int32_t cuda_ldpc_5g_encode(uint8_t *infobytes, uint8_t *rm_bytes, ldpc_in_config_t *inconfig);
+
 /* c_lib/nr_ldpc.h:123, c_lib/tx_rx_process.c:286 — test channel: Gray QAM,
  * complex AWGN of power 10^(noise_power_db/10), max-log LLRs x128 rounded
  * and clipped to +-10000.  The noise comes from this library's own

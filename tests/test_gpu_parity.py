@@ -348,3 +348,60 @@ def test_missing_soft_pool_refused_before_launch(L):
                                    ctypes.byref(bad))
     assert rc == -4
     L.ldpc_mi355_batch_destroy(b)
+
+
+def _cuda_abi(L, llr, cfg, row, harq_in=None):
+    bgn, A, crc, E, rv, Qm, h = cfg
+    B = A + (16 if crc == 0 else 24)
+    hv = harq_view(L)
+    if harq_in is not None:
+        hv[:] = 0
+        hv[:len(harq_in)] = harq_in
+    hard = np.full(B // 8, 0xA5, np.uint8)
+    llr = np.ascontiguousarray(llr, np.int16)
+    st = L.cuda_ldpc_5g_decode(ptr(llr), ptr(hard), ctypes.byref(LdpcInConfig(*cfg)),
+                               ctypes.byref(DecoderConfig(*row)))
+    return dict(status=st, hard=hard, soft=hv.copy())
+
+
+@pytest.mark.parametrize("case", ALGO1[:12], ids=case_id)
+def test_cuda_twin_algo1_matches_reference(L, case):
+    """cuda_ldpc_5g_decode (the symbol scripts/sim_cuda_ldpc_encoder_decoder.py
+    binds) with algo 1: the reference fixtures, like c_ldpc_5g_decode."""
+    out = _cuda_abi(L, case["llr"], case["cfg"], case["row"], case["harq_in"])
+    n = len(case["soft"])
+    assert out["status"] == case["status"]
+    assert np.array_equal(out["soft"][:n], case["soft"])
+    if case["row"][1]:
+        assert np.array_equal(out["hard"], case["hard"])
+
+
+@pytest.mark.parametrize("case", ALGO0, ids=case_id)
+def test_cuda_twin_algo4_is_bp(L, case):
+    """cuda_ldpc_5g_decode with algo 4 runs FP32 BP
+    (CUDA/cuda_ldpc_5g_decoder.cu:49-51): the same status, hard bits and HARQ
+    buffer as algo 0 through c_ldpc_5g_decode, on the BP fixtures."""
+    row = list(case["row"])
+    want = abi_decode(L, case["llr"], case["cfg"], row, case["harq_in"])
+    row4 = [4] + row[1:]
+    got = _cuda_abi(L, case["llr"], case["cfg"], row4, case["harq_in"])
+    n = len(case["soft"])
+    assert got["status"] == want["status"] == case["status"]
+    assert np.array_equal(got["soft"][:n], want["soft"][:n])
+    if row[1]:
+        assert np.array_equal(got["hard"], want["hard"])
+
+
+@pytest.mark.parametrize("algo", [0, 2, 3, 5])
+def test_cuda_twin_refuses_other_algos(L, algo):
+    """Only algos 1 and 4 exist under the CUDA symbol: others return -3 after
+    rate recovery, hard bits untouched."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(algo)
+    cfg = (2, 1016, 2, 3120, 0, 2, 0)
+    llr = rng.integers(-500, 500, 3120).astype(np.int16)
+    out = _cuda_abi(L, llr, cfg, [algo, 8, 22936, 0, 1280, 32767, -32767])
+    assert out["status"] == -3
+    assert np.all(out["hard"] == 0xA5)
+    want = O.rate_recover(O.code(*cfg), 1280, llr)
+    assert np.array_equal(out["soft"][:len(want)], want)

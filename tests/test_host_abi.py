@@ -122,7 +122,12 @@ def test_dequantize(L):
         assert L.dequantize_int16_to_llr(v) == np.float32(v) / np.float32(128)
 
 
-def test_encoder_matches_reference(L):
+@pytest.mark.parametrize("symbol", ["c_ldpc_5g_encode", "cuda_ldpc_5g_encode"])
+def test_encoder_matches_reference(L, symbol):
+    """Both encoder symbols the reference's simulation drivers bind
+    (scripts/sim_c_ldpc_encoder_decoder.py, sim_cuda_ldpc_encoder_decoder.py:65)
+    against the reference encoder's outputs."""
+    encode = getattr(L, symbol)
     z = np.load(os.path.join(GOLDEN, "encoder_vectors.npz"))
     po = ro = 0
     for m in z["meta"]:
@@ -134,8 +139,7 @@ def test_encoder_matches_reference(L):
         ro += nb
         out = np.zeros(nb, np.uint8)
         info = np.packbits(payload)
-        st = L.c_ldpc_5g_encode(ptr(info), ptr(out), ctypes.byref(
-            LdpcInConfig(bgn, A, crctype, E, rv, Qm, 0)))
+        st = encode(ptr(info), ptr(out), ctypes.byref(LdpcInConfig(bgn, A, crctype, E, rv, Qm, 0)))
         assert st == 0
         # compare the E valid bits (the last byte's padding is unspecified)
         assert np.array_equal(np.unpackbits(out)[:E], np.unpackbits(want)[:E]), m
