@@ -217,26 +217,16 @@ def bench_train(args):
     in [0, 2] dB per codeblock, BCE on -LLR of the B bits, float64 grad-norm
     clip, Adam.  A step = forward + loss + backward + optimizer step over one
     fresh batch synthesized on the device inside the step (encode, rate
-    match, channel, float rate recovery).  Beside it:
-    the same step through torch autograd of a torch restatement of the
-    reference model (tests/unrolled_torch_ref.py) on the same GPU."""
+    match, channel, float rate recovery)."""
     import torch
     from deeplearning_5g_ldpc_decoder_amd.trainer import BG1_ZC96_WORKLOAD, _clip_grad_norm_float64
     from deeplearning_5g_ldpc_decoder_amd.unrolled import (CTableNeuralLayeredMinSumDecoder,
-                                                           load_ldpc_table_config,
-                                                           synth_float_batch)
+                                                           load_ldpc_table_config)
     w = BG1_ZC96_WORKLOAD
     n = args.batch if args.batch != BATCH else w["batch_size"]
     tc = load_ldpc_table_config(w["bgn"], w["A"], w["crctype"], w["E"], w["rv"], w["Qm"])
     clip = 32767.0 / 128.0
     T = w["num_iterations"]
-    log(f"synthesizing 4 x {n} training codeblocks")
-    rng = np.random.default_rng(0)
-    pool = []
-    for k in range(4):
-        llr, bits = synth_float_batch(tc, rng.uniform(-2.0, 0.0, n), seed=k,
-                                      threads=min(16, os.cpu_count() or 8))
-        pool.append((torch.from_numpy(llr).cuda(), torch.from_numpy(bits.astype(np.float32)).cuda()))
     model = CTableNeuralLayeredMinSumDecoder(
         tc, num_iterations=T, alpha_mode=w["alpha_mode"], beta_mode=w["beta_mode"],
         init_alpha=w["init_alpha"], init_beta=w["init_beta"], learnable_beta=True,
@@ -287,39 +277,6 @@ def bench_train(args):
     log(f"kernels: {ms:.3f} ms/step (synthesis {synth_ms:.3f} ms, forward {fwd_ms:.3f} ms, "
         f"backward {bwd_ms:.3f} ms)")
 
-    ref = None
-    if not args.no_reference:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from unrolled_torch_ref import unrolled_forward
-        rmodel = CTableNeuralLayeredMinSumDecoder(
-            tc, num_iterations=T, alpha_mode=w["alpha_mode"], beta_mode=w["beta_mode"],
-            init_alpha=w["init_alpha"], init_beta=w["init_beta"], learnable_beta=True,
-            clip_llr=clip).cuda()
-        ropt = torch.optim.Adam(rmodel.parameters(), lr=w["lr"])
-
-        def rstep(k):
-            x, tgt = pool[k % len(pool)]
-            ropt.zero_grad(set_to_none=True)
-            q, _ = unrolled_forward(tc, x, rmodel.alpha_table(), rmodel.beta_table(), clip=clip)
-            loss = crit(-q[:, :B], tgt)
-            loss.backward()
-            _clip_grad_norm_float64(rmodel.parameters(), 1.0)
-            ropt.step()
-            return loss
-
-        log("reference-style torch autograd step (warmup 1, timed 2)")
-        rstep(0)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for k in range(2):
-            rstep(k + 1)
-        torch.cuda.synchronize()
-        rms = (time.perf_counter() - t) * 1e3 / 2
-        ref = {"ms_per_step": rms, "codeblocks_per_s": n / rms * 1e3,
-               "peak_memory_GB": torch.cuda.max_memory_allocated() / 2 ** 30,
-               "what": "torch autograd through a torch restatement of "
-                       "deep_learning/deeplearning_ldpc_model.py:315-410 on the same GPU"}
-        log(f"reference-style: {rms:.1f} ms/step")
     k = model.kernel()
     line = {"metric": "training codeblocks/s (BG1 Zc=96, 10 unrolled iterations, batch 2048)",
             "value": n / ms * 1e3, "unit": "codeblocks/s", "n_gpus": 1, "steps": args.steps,
@@ -331,9 +288,7 @@ def bench_train(args):
                        "parallelism": "single-gpu"},
             "input_synthesis_ms": synth_ms, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
             "record_MB": k.record_bytes(n) / 2 ** 20, "scratch_MB": k.scratch_bytes(n) / 2 ** 20,
-            "loss_first": losses[0], "loss_last": losses[-1],
-            "reference_autograd": ref,
-            "speedup_vs_reference_autograd": (ref["ms_per_step"] / ms) if ref else None}
+            "loss_first": losses[0], "loss_last": losses[-1]}
     print(json.dumps(line))
 
 
@@ -503,8 +458,6 @@ def main():
                     help="serve workload: codeblocks per batch at most")
     ap.add_argument("--serve-contexts", type=int, default=2,
                     help="serve workload: launch contexts (one stream each) in flight at most")
-    ap.add_argument("--no-reference", action="store_true",
-                    help="train workload: skip the torch-autograd reference-style step")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True

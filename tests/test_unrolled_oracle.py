@@ -7,27 +7,15 @@
   clip = max_llr/128;
 * gradient: FP64 central finite differences of the FP64 twin (every alpha
   and beta entry and a sample of input beliefs);
-* conventions: torch autograd through a torch restatement of the reference
-  forward (tests/unrolled_torch_ref.py), including detach, clipping and
-  per-iteration outputs, which finite differences cannot see.
+* conventions (detach, clipping, punctured zeros and ties, per-iteration
+  outputs), which finite differences cannot see: against the reference's
+  own module in tests/test_unrolled_reference.py.
 """
-import ctypes
-from types import SimpleNamespace
-
 import numpy as np
 import pytest
 
 from conftest import load_decode_cases
 
-torch = pytest.importorskip("torch")
-
-
-def table_of(c):
-    """Row offsets / columns / shifts of an oracle code, as a table config."""
-    def arr(p, n):
-        return [int(v) for v in ctypes.cast(p, ctypes.POINTER(ctypes.c_uint16))[:n]]
-    return SimpleNamespace(Zc=c.Zc, num_rows=c.rows, row_offsets=arr(c.row_off, c.rows + 1),
-                           hbg_columns=arr(c.col, c.nnz), vij=arr(c.vij, c.nnz))
 
 
 def test_forward_matches_reference_fp32_fixtures(oracle_mod):
@@ -104,48 +92,6 @@ def test_gradient_matches_fd64(oracle_mod, cfg, T, clip):
         rel = np.linalg.norm(an - fd) / max(np.linalg.norm(fd), 1e-12)
         assert rel < 1e-6, rel
     assert np.abs(ga).max() > 0 and np.abs(gb).max() > 0
-
-
-@pytest.mark.parametrize("cfg,T", CASES)
-@pytest.mark.parametrize("clip", [None, 4.0])
-@pytest.mark.parametrize("detach", [False, True])
-def test_gradient_matches_torch_autograd(oracle_mod, cfg, T, clip, detach):
-    from unrolled_torch_ref import unrolled_forward
-    O = oracle_mod
-    c = O.code(*cfg)
-    tab = table_of(c)
-    batch = 3
-    rng = np.random.default_rng(11)
-    n = c.N + 2 * c.Zc
-    llr = rng.normal(0.0, 3.0, (batch, n)).astype(np.float32)
-    llr[:, :2 * c.Zc] = 0.0                 # punctured columns, as in real inputs
-    alpha = rng.uniform(0.55, 0.95, (T, c.rows)).astype(np.float32)
-    beta = rng.uniform(0.0, 0.3, (T, c.rows)).astype(np.float32)
-    w = rng.normal(size=(batch, n)).astype(np.float32)
-    wi = (rng.normal(size=(T, batch, n)) * 0.3).astype(np.float32)
-
-    x = torch.tensor(llr, requires_grad=True)
-    a = torch.tensor(alpha, requires_grad=True)
-    b = torch.tensor(beta, requires_grad=True)
-    lq, its = unrolled_forward(tab, x, a, b, clip=clip, detach=detach)
-    L = (lq * torch.tensor(w)).sum() + sum((its[t] * torch.tensor(wi[t])).sum()
-                                          for t in range(T))
-    L.backward()
-    ga = np.zeros((T, c.rows))
-    gb = np.zeros((T, c.rows))
-    for k in range(batch):
-        q, it = O.unrolled_forward(c, T, llr[k], alpha, beta, clip=clip, precision="f32",
-                                   want_iters=True)
-        assert np.array_equal(q, lq[k].detach().numpy())
-        assert np.array_equal(it, np.stack([its[t][k].detach().numpy() for t in range(T)]))
-        gl, a_k, b_k = O.unrolled_backward(c, T, llr[k], alpha, beta, w[k], wi[:, k], clip=clip,
-                                           detach=detach, precision="f32")
-        ref = x.grad[k].numpy()
-        assert np.allclose(gl, ref, rtol=1e-5, atol=1e-5 * max(np.abs(ref).max(), 1.0))
-        ga += a_k
-        gb += b_k
-    for mine, ref in ((ga, a.grad.numpy()), (gb, b.grad.numpy())):
-        assert np.abs(mine - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-3)
 
 
 @pytest.mark.parametrize("cfg,T", CASES)
