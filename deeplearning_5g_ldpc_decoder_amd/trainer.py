@@ -209,25 +209,94 @@ BG1_ZC96_WORKLOAD = dict(bgn=1, A=2088, crctype=2, E=6336, rv=0, Qm=2, num_itera
                          training_noise_power_db_range=(-2.0, 0.0), per_codeblock_noise=True)
 
 
+def evaluate_algo2_vs_algo1(model, noise_power_dbs, n=4096, seed=0, alpha_q15=22936):
+    """Block error rates of the model's exported algo-2 tables, installed in
+    the library (load_algo2_tables), against the int16 min-sum with one
+    alpha (algo 1), both for the model's number of iterations with CRC early
+    stop, on the same synthesized codeblocks (n per noise power).  The
+    library's default algo-2 tables are restored afterwards."""
+    import torch
+    from .batch import BatchPlan, synth_batch
+    from .tables import set_trained_tables
+    from .unrolled import load_algo2_tables
+    c = model.config
+    cfg = (c.bgn, c.A, c.crctype, c.E, c.rv, c.Qm, 0)
+    T = model.num_iterations
+    rows = {"algo2_trained": [2, T, 0, 0, 1280, 32767, -32767],
+            "algo1": [1, T, alpha_q15, 0, 1280, 32767, -32767]}
+    load_algo2_tables(model)
+    out = []
+    try:
+        for k, nz in enumerate(noise_power_dbs):
+            llr, _ = synth_batch(cfg, n, float(nz), seed=seed * 1000 + k, want_payload=False)
+            d = torch.from_numpy(llr.reshape(-1)).cuda()
+            point = {"noise_power_db": float(nz), "EsN0_dB": -float(nz), "codeblocks": n}
+            for name, row in rows.items():
+                plan = BatchPlan([cfg] * n, row)
+                res = plan.decode(d)
+                st = res["status"].cpu().numpy()
+                point[name] = {"bler": float(np.mean(st != 0)),
+                               "mean_iterations": float(res["iters"].float().mean())}
+                plan.close()
+            out.append(point)
+    finally:
+        set_trained_tables(c.bgn)
+    return {"decoder_rows": rows, "points": out}
+
+
 def main(argv=None):
-    ap = argparse.ArgumentParser()
+    ap = argparse.ArgumentParser(
+        description="Train alpha/beta of the BG1 Zc=96 workload (rate 1/3 QPSK, 10 unrolled "
+                    "iterations, per-(iteration, degree) parameters, batch 2048, Adam 1e-3 with "
+                    "cosine annealing) and export the algo-2 tables.")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--noise-range", type=float, nargs=2, default=None,
+                    metavar=("LOW", "HIGH"),
+                    help="noise power range in dB, one draw per codeblock (default -2 0: "
+                         "Es/N0 uniform in [0, 2] dB)")
     ap.add_argument("--out", default=None, help="directory for the algo-2 table export")
     ap.add_argument("--save", default=None)
     ap.add_argument("--history", default=None)
+    ap.add_argument("--evaluate", type=float, nargs="*", default=None, metavar="NOISE_DB",
+                    help="after training: BLER of the exported algo-2 tables against algo 1 "
+                         "at these noise powers (dB)")
+    ap.add_argument("--summary", default=None, help="JSON summary of the run")
     args = ap.parse_args(argv)
-    from .unrolled import export_algo2_tables
+    from .unrolled import algo2_tables, export_algo2_tables
     cfg = dict(BG1_ZC96_WORKLOAD)
-    cfg.update(epochs=args.steps, batch_size=args.batch, save_path=args.save, noise_power_db=-1.0)
+    if args.noise_range is not None:
+        cfg["training_noise_power_db_range"] = tuple(args.noise_range)
+    cfg.update(epochs=args.steps, batch_size=args.batch, save_path=args.save, noise_power_db=-1.0,
+               verbose=False)
     hist = []
+    t0 = time.time()
     model, _, best = train_c_table_ldpc_model(history=hist, **cfg)
-    print(f"best loss {best:.6f}", flush=True)
+    wall = time.time() - t0
+    print(f"{args.steps} steps in {wall:.1f} s, best loss {best:.6g}", flush=True)
+    summary = {"workload": {k: (list(v) if isinstance(v, tuple) else v)
+                            for k, v in cfg.items() if k != "save_path"},
+               "steps": args.steps, "wall_s": wall,
+               "loss": {"first": hist[0]["loss"], "last": hist[-1]["loss"], "best": best,
+                        "every_100": [h["loss"] for h in hist[::100]]}}
     if args.out:
-        print("exported", export_algo2_tables(model, args.out), flush=True)
+        summary["exported"] = [str(p) for p in export_algo2_tables(model, args.out)]
+        print("exported", summary["exported"], flush=True)
+    fa, fb = algo2_tables(model, num_iterations=model.num_iterations)
+    summary["algo2_alpha_q15"] = fa.tolist()
+    summary["algo2_beta_llr128"] = fb.tolist()
+    if args.evaluate:
+        summary["evaluation"] = evaluate_algo2_vs_algo1(model, args.evaluate)
+        for p in summary["evaluation"]["points"]:
+            print(f"noise {p['noise_power_db']:+.2f} dB: algo 2 (trained) BLER "
+                  f"{p['algo2_trained']['bler']:.4f}, algo 1 BLER {p['algo1']['bler']:.4f}",
+                  flush=True)
     if args.history:
         with open(args.history, "w") as f:
             json.dump(hist, f)
+    if args.summary:
+        with open(args.summary, "w") as f:
+            json.dump(summary, f, indent=1)
     return 0
 
 
