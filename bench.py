@@ -298,43 +298,42 @@ def bench_serve(args):
     waterfall) arrive as a Poisson process from 16 UEs x 4 HARQ processes,
     each with a 0.5 ms deadline; failed transmissions come back after 1 ms with
     the next redundancy version (0, 2, 3, 1) and soft-combine into device
-    HARQ buffers.  Continuous batching over --serve-contexts launch contexts
-    (default 2), admission by a
-    256 MB device LLR pool.  First the capacity (requests/s with every request
-    queued at once), then offered loads of 0.2-1.0 x capacity."""
+    HARQ buffers.  Continuous batching over --serve-contexts launch contexts,
+    admission by a 256 MB device LLR pool, the loop in C++
+    (ldpc_mi355_serve_run; --serve-loop python runs the Python one).  First the
+    unloaded latency (each template alone), the capacity (requests/s with
+    every request queued at once), then offered loads of 0.2-1.0 x capacity."""
     from deeplearning_5g_ldpc_decoder_amd.serving import (DevicePool, GpuExecutor, HarqSlots,
-                                                          RequestPool, Server, poisson_arrivals)
+                                                          NativeServer, RequestPool, Server,
+                                                          poisson_arrivals)
     log("synthesizing 128 request templates (4 redundancy versions each)")
     pool = RequestPool(128, seed=11, max_cbs=64, threads=min(16, os.cpu_count() or 8))
     harq = HarqSlots(16, 4, 64)
     llr_pool = DevicePool(256 << 20)
-    ex = GpuExecutor(pool, harq, llr_pool, n_contexts=args.serve_contexts,
-                     max_batch_codeblocks=args.serve_batch)
-    srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=1e-3)
+    native = args.serve_loop == "native"
+    if args.serve_contexts is None:
+        args.serve_contexts = 2
     rng = np.random.default_rng(5)
+    if native:
+        srv = NativeServer(pool, harq, llr_pool, n_contexts=args.serve_contexts,
+                           max_batch_codeblocks=args.serve_batch, deadline_s=0.5e-3,
+                           harq_rtt_s=1e-3, llr_on_device=args.serve_llr == "device")
+    else:
+        ex = GpuExecutor(pool, harq, llr_pool, n_contexts=args.serve_contexts,
+                         max_batch_codeblocks=args.serve_batch)
+        srv = Server(ex, deadline_s=0.5e-3, harq_rtt_s=1e-3)
 
-    def run(arr):
+    def run(arr, tpl=None, ues=None):
         n = len(arr)
-        return srv.run(arr, rng.integers(0, len(pool), n), ues=rng.integers(0, 16, n))
+        tpl = rng.integers(0, len(pool), n) if tpl is None else tpl
+        ues = rng.integers(0, 16, n) if ues is None else ues
+        return srv.run(arr, tpl, ues=ues)
 
     run(np.zeros(64))          # warmup
-    # unloaded latency: each request alone (copy in, decode, copy out, detect)
-    import torch
-    from deeplearning_5g_ldpc_decoder_amd.serving import Request
-    alone = []
-    for t in range(len(pool)):
-        r = Request(rid=t, template=t, ue=0, proc=0, rv_index=0, arrival=0.0, deadline=0.0,
-                    n_cbs=len(pool.cfgs[t]))
-        region = llr_pool.alloc(pool.elems[t])
-        ctx = ex.free_context()
-        t0 = time.perf_counter()
-        ex.launch(ctx, [r], region)
-        while not ex.poll():
-            pass
-        ex.collect(ctx)
-        alone.append(time.perf_counter() - t0)
-        llr_pool.release(*region)
-    alone = np.array(alone) * 1e3
+    # unloaded latency: every template alone in the system (arrivals 3 ms
+    # apart, longer than a request and its retransmission take)
+    st = run(np.arange(len(pool)) * 3e-3, np.arange(len(pool)), np.zeros(len(pool), np.int64))
+    alone = np.array([r.latency for r in st.requests if r.rv_index == 0]) * 1e3
     unloaded = {"p50_ms": float(np.percentile(alone, 50)),
                 "p99_ms": float(np.percentile(alone, 99)), "requests": len(alone)}
     log(f"unloaded latency p50 {unloaded['p50_ms']:.3f} ms p99 {unloaded['p99_ms']:.3f} ms")
@@ -352,7 +351,8 @@ def bench_serve(args):
         log(f"load {f:.1f}: p50 {s['latency_ms']['p50']:.3f} ms p99 {s['latency_ms']['p99']:.3f} "
             f"ms, miss {s['deadline_miss_rate']:.3f}, {s['info_gbps']:.2f} Gbps")
         sweep.append(s)
-    ex.close()
+    if not native:
+        ex.close()
     mid = sweep[2]
     line = {"metric": "served decode p99 latency at 0.6x capacity (mixed sizes, HARQ, 0.5 ms "
                       "deadline)",
@@ -362,6 +362,8 @@ def bench_serve(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
             "config": {"workload": "serve_mixed_1to64cb_harq16x4_poisson",
                        "decoder_row": ROW, "contexts": args.serve_contexts,
+                       "loop": args.serve_loop,
+                       "llr_source": args.serve_llr if native else "host",
                        "max_batch_codeblocks": args.serve_batch,
                        "llr_pool_MB": 256, "harq_slots": 16 * 4 * 64,
                        "parallelism": "single-gpu"},
@@ -456,8 +458,15 @@ def main():
                          "FP32 BG2 BLER sweep")
     ap.add_argument("--serve-batch", type=int, default=256,
                     help="serve workload: codeblocks per batch at most")
-    ap.add_argument("--serve-contexts", type=int, default=2,
-                    help="serve workload: launch contexts (one stream each) in flight at most")
+    ap.add_argument("--serve-contexts", type=int, default=None,
+                    help="serve workload: launch contexts (one stream each) in flight at most "
+                         "(default 2)")
+    ap.add_argument("--serve-llr", default="device", choices=["device", "host"],
+                    help="serve workload, native loop: request LLRs resident in HBM (as a GPU "
+                         "demapper leaves them; copied device to device into the batch's pool "
+                         "region) or in pinned host memory (copied over PCIe)")
+    ap.add_argument("--serve-loop", default="native", choices=["native", "python"],
+                    help="serve workload: the C++ loop (ldpc_mi355_serve_run) or the Python one")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True

@@ -7,6 +7,8 @@ there is no CPU fallback for the decode path.
 import ctypes
 import os
 
+import numpy as np
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libnrldpc_mi355.so")
 CSRC_DIR = os.path.join(PKG_DIR, "csrc")
@@ -30,7 +32,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_copy_spans",
     "ldpc_mi355_set_trained_tables", "ldpc_mi355_get_trained_tables",
     "ldpc_mi355_set_degree_alpha", "ldpc_mi355_get_degree_alpha",
-    "ldpc_mi355_bp_decode_noalloc",
+    "ldpc_mi355_bp_decode_noalloc", "ldpc_mi355_serve_run",
 )
 EXPORTED_DATA = (
     "BG1_HBG_offset_array", "BG1_HBG_array", "BG1_HBG_Vij_array",
@@ -66,6 +68,40 @@ class CbDescriptor(ctypes.Structure):
     """ldpc_mi355_cb_t: one codeblock of a batch."""
     _fields_ = [("cfg", LdpcInConfig), ("reserved", ctypes.c_int16),
                 ("llr_offset", ctypes.c_uint32), ("soft_slot", ctypes.c_int32)]
+
+
+class ServeCfg(ctypes.Structure):
+    """ldpc_mi355_serve_cfg_t."""
+    _fields_ = [("n_contexts", ctypes.c_int32), ("max_batch_codeblocks", ctypes.c_int32),
+                ("n_ues", ctypes.c_int32), ("n_procs", ctypes.c_int32),
+                ("max_cbs", ctypes.c_int32), ("llr_on_device", ctypes.c_int32),
+                ("deadline_s", ctypes.c_double), ("harq_rtt_s", ctypes.c_double),
+                ("max_wall_s", ctypes.c_double), ("llr_pool_elems", ctypes.c_uint64)]
+
+
+class ServeTemplate(ctypes.Structure):
+    """ldpc_mi355_serve_template_t."""
+    _fields_ = [("n_cbs", ctypes.c_int32), ("cb_first", ctypes.c_int32),
+                ("llr_elems", ctypes.c_uint64), ("llr_host", ctypes.c_uint64 * 4)]
+
+
+class ServeStatsC(ctypes.Structure):
+    """ldpc_mi355_serve_stats_t."""
+    _fields_ = [("n_records", ctypes.c_int32), ("batches", ctypes.c_int32),
+                ("admission_waits", ctypes.c_int32), ("max_blocked", ctypes.c_int32),
+                ("max_inflight", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n_cb_results", ctypes.c_int64), ("hard_bytes", ctypes.c_int64),
+                ("peak_pool_elems", ctypes.c_uint64), ("wall_s", ctypes.c_double),
+                ("host_launch_s", ctypes.c_double), ("host_collect_s", ctypes.c_double)]
+
+
+# ldpc_mi355_serve_record_t as a numpy record
+SERVE_RECORD_DTYPE = np.dtype([
+    ("template_id", "<i4"), ("ue", "<i4"), ("proc", "<i4"), ("rv_index", "<i4"),
+    ("n_cbs", "<i4"), ("n_pass", "<i4"), ("cb_first", "<i8"), ("hard_first", "<i8"),
+    ("batch", "<i4"), ("final_tx", "<i4"), ("arrival_s", "<f8"), ("admitted_s", "<f8"),
+    ("done_s", "<f8")])
+assert SERVE_RECORD_DTYPE.itemsize == 72
 
 
 _lib = None
@@ -146,6 +182,12 @@ def load():
                                                ctypes.POINTER(DecoderConfig), ctypes.c_int32,
                                                P, P, P, P, P, P, P]
     L.ldpc_mi355_bp_decode_noalloc.restype = ctypes.c_int32
+    L.ldpc_mi355_serve_run.argtypes = [ctypes.POINTER(ServeCfg), ctypes.POINTER(DecoderConfig),
+                                       ctypes.POINTER(ServeTemplate), ctypes.c_int32, P, P, P,
+                                       P, P, P, P, ctypes.c_int32, P, ctypes.c_int32, P, P,
+                                       ctypes.c_int64, P, ctypes.c_int64,
+                                       ctypes.POINTER(ServeStatsC)]
+    L.ldpc_mi355_serve_run.restype = ctypes.c_int32
     L.ldpc_mi355_version.argtypes = []
     L.ldpc_mi355_version.restype = ctypes.c_char_p
     L.ldpc_mi355_synth_batch.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32,

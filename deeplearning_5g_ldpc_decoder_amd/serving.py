@@ -574,6 +574,103 @@ class Server:
         return stats
 
 
+class NativeServer:
+    """The same served loop run natively (ldpc_mi355_serve_run): request
+    templates, pinned LLRs, the device LLR pool and HARQ slots as for
+    GpuExecutor/Server, host work per batch in C++.  run() returns ServeStats
+    with one Request per transmission, like Server.run."""
+
+    def __init__(self, pool: RequestPool, harq: HarqSlots, llr_pool: DevicePool,
+                 row=DEFAULT_ROW, n_contexts=4, max_batch_codeblocks=256, deadline_s=0.5e-3,
+                 harq_rtt_s=1.0e-3, record_hard=False, llr_on_device=False):
+        if pool.pinned is None:
+            raise ValueError("the served loop needs the request LLRs in pinned memory")
+        # llr_on_device: the request LLRs sit in HBM, as a demapper on the GPU
+        # would leave them, and batches copy them device to device
+        self.llr_src = pool.pinned.cuda() if llr_on_device else pool.pinned
+        self.llr_on_device = bool(llr_on_device)
+        self.pool, self.harq, self.llr_pool = pool, harq, llr_pool
+        self.row = tuple(int(v) for v in row)
+        self.n_contexts = int(n_contexts)
+        self.max_cbs = int(max_batch_codeblocks)
+        self.deadline_s, self.harq_rtt_s = deadline_s, harq_rtt_s
+        self.record_hard = record_hard
+        n = len(pool)
+        self.templates = (_lib.ServeTemplate * n)()
+        cfgs = []
+        for t in range(n):
+            tp = self.templates[t]
+            tp.n_cbs = pool.sizes[t]
+            tp.cb_first = len(cfgs)
+            a0, b0 = pool.span(t, 0)
+            tp.llr_elems = b0 - a0
+            for rvi in range(len(RV_SEQUENCE)):
+                a, b = pool.span(t, rvi)
+                assert b - a == b0 - a0
+                tp.llr_host[rvi] = a
+            cfgs.extend(pool.cfgs[t])
+        self.cb_cfg = (LdpcInConfig * len(cfgs))(*[LdpcInConfig(*[int(v) for v in c])
+                                                     for c in cfgs])
+        self.batches = 0
+
+    def run(self, arrivals, templates, ues=None, max_wall_s=60.0):
+        L = _lib.load()
+        pool = self.pool
+        n_arr = len(arrivals)
+        t_arr = np.ascontiguousarray(arrivals, np.float64)
+        tpl = np.ascontiguousarray(templates, np.int32)
+        ue = (np.arange(n_arr, dtype=np.int32) % self.harq.n_ues if ues is None
+              else np.ascontiguousarray(ues, np.int32))
+        max_rec = 4 * n_arr + 1
+        recs = np.zeros(max_rec, _lib.SERVE_RECORD_DTYPE)
+        max_cb = int(sum(pool.sizes[int(k)] for k in tpl)) * 4 + 1
+        st = np.zeros(max_cb, np.int32)
+        its = np.zeros(max_cb, np.int32)
+        hard_cap = int(sum(pool.cb_hard[int(k)].sum() for k in tpl)) * 4 if self.record_hard else 0
+        hard = np.zeros(max(hard_cap, 1), np.uint8)
+        cfg = _lib.ServeCfg(n_contexts=self.n_contexts, max_batch_codeblocks=self.max_cbs,
+                            n_ues=self.harq.n_ues, n_procs=self.harq.n_procs,
+                            max_cbs=self.harq.max_cbs, llr_on_device=int(self.llr_on_device),
+                            deadline_s=self.deadline_s,
+                            harq_rtt_s=self.harq_rtt_s, max_wall_s=max_wall_s,
+                            llr_pool_elems=self.llr_pool.elems)
+        out = _lib.ServeStatsC()
+        rc = L.ldpc_mi355_serve_run(
+            ctypes.byref(cfg), ctypes.byref(DecoderConfig(*self.row)), self.templates, len(pool),
+            self.cb_cfg, ptr(self.llr_src), ptr(self.llr_pool.tensor), ptr(self.harq.tensor),
+            ptr(t_arr), ptr(tpl), ptr(ue), n_arr, ptr(recs), max_rec, ptr(st), ptr(its), max_cb,
+            ptr(hard) if self.record_hard else None, hard_cap, ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"ldpc_mi355_serve_run failed ({rc})")
+        stats = ServeStats(cb_info=pool.cb_info)
+        for i, r in enumerate(recs[:out.n_records]):
+            t = int(r["template_id"])
+            a, m = int(r["cb_first"]), int(r["n_cbs"])
+            q = Request(rid=i, template=t, ue=int(r["ue"]), proc=int(r["proc"]),
+                        rv_index=int(r["rv_index"]), arrival=float(r["arrival_s"]),
+                        deadline=float(r["arrival_s"]) + self.deadline_s, n_cbs=m,
+                        info_bits=pool.info_bits[t], admitted=float(r["admitted_s"]),
+                        done=float(r["done_s"]), batch_id=int(r["batch"]))
+            q.status = st[a:a + m].copy()
+            q.iters = its[a:a + m].copy()
+            q.final = bool(r["final_tx"])
+            if self.record_hard and r["hard_first"] >= 0:
+                h0 = int(r["hard_first"])
+                q.hard_len = pool.cb_hard[t]
+                q.hard_bytes = hard[h0:h0 + int(q.hard_len.sum())].copy()
+            stats.requests.append(q)
+        stats.wall_s = out.wall_s
+        stats.batches = out.batches
+        stats.admission_waits = out.admission_waits
+        stats.max_blocked_arrivals = out.max_blocked
+        stats.max_inflight = out.max_inflight
+        stats.host_launch_s = out.host_launch_s
+        stats.host_collect_s = out.host_collect_s
+        stats.peak_pool_bytes = 2 * int(out.peak_pool_elems)
+        self.batches += out.batches
+        return stats
+
+
 def poisson_arrivals(rate_per_s, duration_s, seed=0):
     rng = np.random.default_rng(seed)
     t, out = 0.0, []

@@ -316,6 +316,81 @@ void ldpc_mi355_batch_destroy(ldpc_mi355_batch_t *batch);
 int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *src, const uint64_t *bytes,
                               int32_t n, int32_t kind, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Native served loop.  No counterpart in the reference: the serving   */
+/* pattern (continuous batching, admission by free device memory,      */
+/* device-resident HARQ soft buffers) run in C++ on the calling thread.*/
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t n_contexts;            /* batches in flight at most (one HIP stream each) */
+    int32_t max_batch_codeblocks;  /* codeblocks per batch at most */
+    int32_t n_ues, n_procs;        /* HARQ processes per UE */
+    int32_t max_cbs;               /* codeblocks per request at most: soft slot (ue, proc, cb)
+                                    * is ((ue * n_procs) + proc) * max_cbs + cb */
+    int32_t llr_on_device;         /* 0: llr_host is pinned host memory; 1: a device array */
+    double deadline_s;             /* recorded with each request (not enforced) */
+    double harq_rtt_s;             /* a failed transmission's next rv arrives this much later */
+    double max_wall_s;             /* the loop stops after this long */
+    uint64_t llr_pool_elems;       /* admission budget: int16 elements of llr_pool */
+} ldpc_mi355_serve_cfg_t;
+
+/* A request template: n_cbs codeblocks with configs cb_cfg[cb_first ..]
+ * (their rv and HARQ_on are set per transmission); the LLRs of redundancy
+ * version index r (rv 0, 2, 3, 1) are llr_elems int16 values at element
+ * offset llr_host[r] of the host LLR array, codeblock after codeblock. */
+typedef struct {
+    int32_t n_cbs, cb_first;
+    uint64_t llr_elems;
+    uint64_t llr_host[4];
+} ldpc_mi355_serve_template_t;
+
+/* One transmission, as served.  Codeblock results: status and iterations at
+ * [cb_first, cb_first + n_cbs) of the result arrays; hard bytes at byte
+ * hard_first of hard_out (-1 when not recorded). */
+typedef struct {
+    int32_t template_id, ue, proc, rv_index;
+    int32_t n_cbs, n_pass;          /* codeblocks, codeblocks that passed CRC */
+    int64_t cb_first, hard_first;
+    int32_t batch, final_tx;        /* batch index; 1 = last transmission of its chain */
+    double arrival_s, admitted_s, done_s;   /* seconds from the start of the run */
+} ldpc_mi355_serve_record_t;
+
+typedef struct {
+    int32_t n_records, batches, admission_waits, max_blocked, max_inflight, reserved;
+    int64_t n_cb_results, hard_bytes;
+    uint64_t peak_pool_elems;
+    double wall_s, host_launch_s, host_collect_s;
+} ldpc_mi355_serve_stats_t;
+
+/* Runs the served loop until every arrival's HARQ chain is done (or
+ * max_wall_s): new-data arrivals (arrival_s relative to the start, template,
+ * UE) wait for a free HARQ process of their UE, then queue; whenever one of
+ * n_contexts batch contexts is free, the queue head is admitted as one batch
+ * of whole requests within max_batch_codeblocks and a first-fit region of
+ * llr_pool (device, llr_pool_elems int16); each batch is copied from the
+ * request LLRs (llr_host: pinned host memory, or device memory with
+ * llr_on_device) into its region, decoded with decoder_config
+ * and its results copied back asynchronously on its context's stream; a
+ * transmission with a failed codeblock is retransmitted with the next rv of
+ * 0, 2, 3, 1 after harq_rtt_s and combined into the same soft slots of
+ * soft_pool (device, 68*384 int16 per slot).  Fills records (one per
+ * transmission, max_records), per-codeblock cb_status / cb_iterations
+ * (max_cb_results) and, when hard_out is not NULL, the hard bytes
+ * (hard_capacity bytes; later ones are not recorded), and stats.
+ * Returns 0; -4 invalid argument or a result array too small; -5 a request
+ * larger than the LLR pool; -1 GPU error.  Allocates its contexts, streams
+ * and result buffers once at the start, nothing inside the loop. */
+int32_t ldpc_mi355_serve_run(const ldpc_mi355_serve_cfg_t *cfg,
+                             const ldpc_decoder_cfg_t *decoder_config,
+                             const ldpc_mi355_serve_template_t *templates, int32_t n_templates,
+                             const ldpc_in_config_t *cb_cfg, const int16_t *llr_host,
+                             int16_t *llr_pool, int16_t *soft_pool, const double *arrival_s,
+                             const int32_t *arrival_template, const int32_t *arrival_ue,
+                             int32_t n_arrivals, ldpc_mi355_serve_record_t *records,
+                             int32_t max_records, int32_t *cb_status, int32_t *cb_iterations,
+                             int64_t max_cb_results, uint8_t *hard_out, int64_t hard_capacity,
+                             ldpc_mi355_serve_stats_t *stats);
+
 /* 1 when the library runs decoder algo `algo` (0-4) on the GPU, else 0. */
 int32_t ldpc_mi355_algo_available(int32_t algo);
 
