@@ -298,11 +298,19 @@ def bench_serve(args):
     waterfall) arrive as a Poisson process from 16 UEs x 4 HARQ processes,
     each with a 0.5 ms deadline; failed transmissions come back after 1 ms with
     the next redundancy version (0, 2, 3, 1) and soft-combine into device
-    HARQ buffers.  Continuous batching over --serve-contexts launch contexts,
-    admission by a 256 MB device LLR pool, the loop in C++
-    (ldpc_mi355_serve_run; --serve-loop python runs the Python one).  First the
-    unloaded latency (each template alone), the capacity (requests/s with
-    every request queued at once), then offered loads of 0.2-1.0 x capacity."""
+    HARQ buffers.  Continuous batching over --serve-contexts launch contexts
+    (one HIP stream each), the loop in C++ (ldpc_mi355_serve_run; --serve-loop
+    python runs the Python one).  Request LLRs are resident in HBM and decoded
+    in place (--serve-llr host_mapped: in pinned host memory, read in place
+    over PCIe; host_copy: copied from there into a 256 MB device pool that
+    admission budgets).  First the unloaded latency (each template
+    alone), the capacity (requests/s with every request queued at once), then
+    offered loads of 0.2-1.0 x capacity."""
+    # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process;
+    # streams sharing a queue run their kernels one after another, so with
+    # fewer queues than contexts concurrent batches serialize.  Set before
+    # the first HIP call of the process.
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.serve_hw_queues)
     from deeplearning_5g_ldpc_decoder_amd.serving import (DevicePool, GpuExecutor, HarqSlots,
                                                           NativeServer, RequestPool, Server,
                                                           poisson_arrivals)
@@ -312,12 +320,13 @@ def bench_serve(args):
     llr_pool = DevicePool(256 << 20)
     native = args.serve_loop == "native"
     if args.serve_contexts is None:
-        args.serve_contexts = 2
+        args.serve_contexts = 4 if native else 2
     rng = np.random.default_rng(5)
     if native:
         srv = NativeServer(pool, harq, llr_pool, n_contexts=args.serve_contexts,
                            max_batch_codeblocks=args.serve_batch, deadline_s=0.5e-3,
-                           harq_rtt_s=1e-3, llr_on_device=args.serve_llr == "device")
+                           harq_rtt_s=1e-3,
+                           llr_source=args.serve_llr)
     else:
         ex = GpuExecutor(pool, harq, llr_pool, n_contexts=args.serve_contexts,
                          max_batch_codeblocks=args.serve_batch)
@@ -353,6 +362,11 @@ def bench_serve(args):
         sweep.append(s)
     if not native:
         ex.close()
+    batch_path = serve_mix_batched(pool, args)
+    served_cbs = sat["codeblocks"] / st.wall_s
+    log(f"batch path, the same mix: {batch_path['codeblocks_per_s']:.0f} codeblocks/s, "
+        f"{batch_path['info_gbps']:.2f} Gbps decoded; served at saturation {served_cbs:.0f} "
+        "codeblocks/s")
     mid = sweep[2]
     line = {"metric": "served decode p99 latency at 0.6x capacity (mixed sizes, HARQ, 0.5 ms "
                       "deadline)",
@@ -363,13 +377,53 @@ def bench_serve(args):
             "config": {"workload": "serve_mixed_1to64cb_harq16x4_poisson",
                        "decoder_row": ROW, "contexts": args.serve_contexts,
                        "loop": args.serve_loop,
-                       "llr_source": args.serve_llr if native else "host",
+                       "llr_source": args.serve_llr if native else "host_copy",
+                       "hw_queues": args.serve_hw_queues,
                        "max_batch_codeblocks": args.serve_batch,
                        "llr_pool_MB": 256, "harq_slots": 16 * 4 * 64,
                        "parallelism": "single-gpu"},
             "capacity_requests_per_s": cap, "unloaded": unloaded, "saturated": sat,
+            "saturated_codeblocks_per_s": served_cbs, "batch_path": batch_path,
+            "served_vs_batch_codeblock_rate": served_cbs / batch_path["codeblocks_per_s"],
             "sweep": sweep}
     print(json.dumps(line))
+
+
+def serve_mix_batched(pool, args):
+    """The served request mix through the batch path: the first transmission
+    (rv 0) of every request template, all codeblocks in one launch of the
+    same decoder row, timed over --steps launches with HIP events.  Gives the
+    rate the GPU decodes this mix at when nothing limits concurrency (no
+    deadlines, HARQ processes or per-batch round trips)."""
+    import torch
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    cfgs, parts, info = [], [], []
+    for t in range(len(pool)):
+        parts.append(np.asarray(pool.llr(t, 0)[0]))
+        cfgs += [tuple(c[:4]) + (0, c[5], 0) for c in pool.cfgs[t]]
+        info.append(pool.cb_info[t])
+    info = np.concatenate(info)
+    plan = BatchPlan(cfgs, ROW)
+    d_llr = torch.from_numpy(np.concatenate(parts)).cuda()
+    out = plan.alloc_outputs("cuda")
+    s = torch.cuda.current_stream()
+    for _ in range(3):
+        plan.launch(d_llr, out, stream=s.cuda_stream)
+    steps = max(10, min(args.steps, 200))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        plan.launch(d_llr, out, stream=s.cuda_stream)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    ok = out["status"].cpu().numpy()[:len(cfgs)] == 0
+    plan.close()
+    return {"codeblocks": len(cfgs), "launch_ms": ms, "steps": steps,
+            "codeblocks_per_s": len(cfgs) / (ms * 1e-3),
+            "info_gbps": float(info.sum()) / (ms * 1e-3) / 1e9,
+            "passed_info_gbps": float(info[ok].sum()) / (ms * 1e-3) / 1e9,
+            "codeblock_bler": float(1.0 - ok.mean())}
 
 
 def bench_bler(args):
@@ -460,11 +514,16 @@ def main():
                     help="serve workload: codeblocks per batch at most")
     ap.add_argument("--serve-contexts", type=int, default=None,
                     help="serve workload: launch contexts (one stream each) in flight at most "
-                         "(default 2)")
-    ap.add_argument("--serve-llr", default="device", choices=["device", "host"],
+                         "(default 4 with the native loop, 2 with the Python loop)")
+    ap.add_argument("--serve-hw-queues", type=int, default=8,
+                    help="serve workload: GPU_MAX_HW_QUEUES for the process (hardware queues "
+                         "the HIP streams are spread over; at most 32)")
+    ap.add_argument("--serve-llr", default="device",
+                    choices=["device", "host_mapped", "host_copy"],
                     help="serve workload, native loop: request LLRs resident in HBM (as a GPU "
-                         "demapper leaves them; copied device to device into the batch's pool "
-                         "region) or in pinned host memory (copied over PCIe)")
+                         "demapper leaves them) and decoded in place; in pinned host memory, "
+                         "decoded in place over PCIe; or in pinned host memory and copied into "
+                         "the batch's region of the device pool")
     ap.add_argument("--serve-loop", default="native", choices=["native", "python"],
                     help="serve workload: the C++ loop (ldpc_mi355_serve_run) or the Python one")
     args = ap.parse_args()
@@ -472,6 +531,8 @@ def main():
         args.no_cpu = True
     if args.max_iterations is not None and not args.profile:
         ap.error("--max-iterations changes the workload: only with --profile")
+    if not 1 <= args.serve_hw_queues <= 32:
+        ap.error("--serve-hw-queues: 1 to 32")
     if args.gpus != 1:
         print(f"bench.py: the decoder runs on one GPU; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would shard codeblocks by index, with no exchange)",

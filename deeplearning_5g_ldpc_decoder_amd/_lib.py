@@ -70,11 +70,15 @@ class CbDescriptor(ctypes.Structure):
                 ("llr_offset", ctypes.c_uint32), ("soft_slot", ctypes.c_int32)]
 
 
+# ldpc_mi355_serve_cfg_t.llr_source
+LLR_SOURCES = {"host_copy": 0, "device": 1, "host_mapped": 2}
+
+
 class ServeCfg(ctypes.Structure):
     """ldpc_mi355_serve_cfg_t."""
     _fields_ = [("n_contexts", ctypes.c_int32), ("max_batch_codeblocks", ctypes.c_int32),
                 ("n_ues", ctypes.c_int32), ("n_procs", ctypes.c_int32),
-                ("max_cbs", ctypes.c_int32), ("llr_on_device", ctypes.c_int32),
+                ("max_cbs", ctypes.c_int32), ("llr_source", ctypes.c_int32),
                 ("deadline_s", ctypes.c_double), ("harq_rtt_s", ctypes.c_double),
                 ("max_wall_s", ctypes.c_double), ("llr_pool_elems", ctypes.c_uint64)]
 

@@ -3,9 +3,9 @@
 // device memory, HARQ soft buffers resident on the device.
 //
 // Requests are transmissions of one HARQ process of one UE: 1..max_cbs
-// codeblocks whose channel LLRs sit in pinned host memory, one contiguous
-// span per redundancy version.  The loop runs in real time on the calling
-// thread:
+// codeblocks whose channel LLRs sit in pinned host memory or in device
+// memory, one contiguous span per redundancy version.  The loop runs in real
+// time on the calling thread:
 //   * new-data arrivals wait for a free HARQ process of their UE, then join
 //     a FIFO queue; retransmissions join it harq_rtt_s after the failed
 //     transmission completed;
@@ -14,17 +14,20 @@
 //     as fit max_batch_codeblocks and a first-fit region of the device LLR
 //     pool: requests that do not fit wait until a running batch frees its
 //     region;
-//   * a batch's LLR spans are copied into its region, decoded (the decoder
-//     row of the contexts) and its status, iterations and hard bytes copied
-//     back, all asynchronously on the context's stream; an event per batch is
-//     polled for completion;
+//   * a batch's LLR spans are copied into its region (host LLRs) or read in
+//     place (device LLRs, as a demapper on the GPU leaves them: no region,
+//     no copy), decoded (the decoder row of the contexts) and its status,
+//     iterations and hard bytes copied back, all asynchronously on the
+//     context's stream; an event per batch is polled for completion;
 //   * a transmission whose codeblocks do not all pass CRC is retransmitted
 //     with the next redundancy version of 0, 2, 3, 1 and soft-combined on
 //     the device into the same slots (HARQ_on = 1); slot (ue, proc, cb) is
 //     ((ue * n_procs) + proc) * max_cbs + cb of the soft-buffer array.
-// Host work per batch is descriptor building and three calls into the HIP
-// runtime; nothing is allocated inside the loop.
+// Host work per batch is descriptor building, one copy per request (host
+// LLRs only) and three calls into the HIP runtime; nothing is allocated
+// inside the loop.
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -125,16 +128,25 @@ extern "C" int32_t ldpc_mi355_serve_run(
     const int32_t *arrival_ue, int32_t n_arrivals, ldpc_mi355_serve_record_t *records,
     int32_t max_records, int32_t *cb_status, int32_t *cb_iterations, int64_t max_cb_results,
     uint8_t *hard_out, int64_t hard_capacity, ldpc_mi355_serve_stats_t *stats) {
+    const bool in_place = cfg && cfg->llr_source != LDPC_MI355_LLR_HOST_COPY;
     if (!cfg || !decoder_config || !templates || n_templates <= 0 || !cb_cfg || !llr_host ||
-        !llr_pool || !soft_pool || n_arrivals < 0 || (n_arrivals && (!arrival_s ||
-        !arrival_template || !arrival_ue)) || !records || !cb_status || !cb_iterations ||
-        !stats || cfg->n_contexts <= 0 || cfg->max_batch_codeblocks <= 0 || cfg->n_ues <= 0 ||
-        cfg->n_procs <= 0 || cfg->max_cbs <= 0 || cfg->llr_pool_elems == 0)
+        (!in_place && (!llr_pool || cfg->llr_pool_elems == 0)) || !soft_pool ||
+        n_arrivals < 0 || (n_arrivals && (!arrival_s || !arrival_template || !arrival_ue)) ||
+        !records || !cb_status || !cb_iterations || !stats || cfg->n_contexts <= 0 ||
+        cfg->max_batch_codeblocks <= 0 || cfg->n_ues <= 0 || cfg->n_procs <= 0 ||
+        cfg->max_cbs <= 0 || cfg->llr_source < LDPC_MI355_LLR_HOST_COPY ||
+        cfg->llr_source > LDPC_MI355_LLR_HOST_MAPPED)
         return -4;
-    for (int32_t i = 0; i < n_templates; ++i)
+    for (int32_t i = 0; i < n_templates; ++i) {
         if (templates[i].n_cbs <= 0 || templates[i].n_cbs > cfg->max_cbs ||
             templates[i].n_cbs > cfg->max_batch_codeblocks)
             return -4;
+        // read in place, a codeblock's LLRs are addressed by a 32-bit element
+        // offset from llr_host
+        if (in_place)
+            for (int r = 0; r < 4; ++r)
+                if (templates[i].llr_host[r] + templates[i].llr_elems > UINT32_MAX) return -4;
+    }
     for (int32_t i = 0; i < n_arrivals; ++i)
         if (arrival_template[i] < 0 || arrival_template[i] >= n_templates || arrival_ue[i] < 0 ||
             arrival_ue[i] >= cfg->n_ues)
@@ -171,11 +183,21 @@ extern "C" int32_t ldpc_mi355_serve_run(
         if (pos != templates[t].llr_elems) rc = -4;
     }
 
-    // LLRs from pinned host memory (over PCIe), or from a device array (as a
-    // demapper on the GPU would leave them)
-    const hipMemcpyKind copy_kind =
-        cfg->llr_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    FirstFit pool(cfg->llr_pool_elems);
+    // LLRs from pinned host memory, copied over PCIe into a region of
+    // llr_pool, or read where they are: in device memory, or in pinned host
+    // memory through its device address
+    const int16_t *llr_base = in_place ? llr_host : llr_pool;
+    if (rc == 0 && cfg->llr_source == LDPC_MI355_LLR_HOST_MAPPED) {
+        void *mapped = nullptr;
+        if (hipHostGetDevicePointer(&mapped, const_cast<int16_t *>(llr_host), 0) != hipSuccess ||
+            !mapped) {
+            fprintf(stderr, "libnrldpc_mi355: llr_host is not pinned host memory mapped for the "
+                            "device\n");
+            rc = -4;
+        }
+        llr_base = static_cast<const int16_t *>(mapped);
+    }
+    FirstFit pool(in_place ? 0 : cfg->llr_pool_elems);
     std::priority_queue<Event, std::vector<Event>, std::greater<Event>> pending;
     uint64_t seq = 0;
     for (int32_t i = 0; i < n_arrivals; ++i)
@@ -249,7 +271,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 k += r.n_cbs;
                 h += (size_t)nb;
             }
-            pool.release(c.region_off, c.region_len);
+            if (c.region_len) pool.release(c.region_off, c.region_len);
             c.busy = false;
             c.records.clear();
             --inflight;
@@ -315,7 +337,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 ++take;
             }
             uint64_t got = 0;
-            int64_t off = take ? pool.alloc(elems, &got) : -1;
+            int64_t off = in_place ? 0 : take ? pool.alloc(elems, &got) : -1;
             while (take && off < 0) {   // admission by free device memory
                 --take;
                 elems -= templates[records[queue[take]].template_id].llr_elems;
@@ -336,9 +358,12 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 const ldpc_mi355_serve_template_t &tp = templates[r.template_id];
                 r.admitted_s = now;
                 r.batch = stats->batches;
-                if (hipMemcpyAsync(llr_pool + pos, llr_host + tp.llr_host[r.rv_index],
-                                   tp.llr_elems * sizeof(int16_t), copy_kind,
-                                   c->stream) != hipSuccess) {
+                uint64_t span = pos;
+                if (in_place) {
+                    span = tp.llr_host[r.rv_index];
+                } else if (hipMemcpyAsync(llr_pool + pos, llr_host + tp.llr_host[r.rv_index],
+                                          tp.llr_elems * sizeof(int16_t), hipMemcpyHostToDevice,
+                                          c->stream) != hipSuccess) {
                     rc = -1;
                     break;
                 }
@@ -349,7 +374,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
                     d.cfg.rv = (uint16_t)kRv[r.rv_index];
                     d.cfg.HARQ_on = r.rv_index > 0 ? 1 : 0;
                     d.reserved = 0;
-                    d.llr_offset = (uint32_t)(pos + cb_rel[(size_t)r.template_id][(size_t)j]);
+                    d.llr_offset = (uint32_t)(span + cb_rel[(size_t)r.template_id][(size_t)j]);
                     d.soft_slot = slot0 + j;
                 }
                 pos += tp.llr_elems;
@@ -358,7 +383,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
             if (rc) break;
             int32_t *d_status = reinterpret_cast<int32_t *>(c->d_out);
             int32_t bad = -1;
-            if (ldpc_mi355_batch_decode(c->batch, c->descs.data(), n, llr_pool, soft_pool,
+            if (ldpc_mi355_batch_decode(c->batch, c->descs.data(), n, llr_base, soft_pool,
                                         c->d_out + (size_t)8 * cap, d_status, d_status + cap,
                                         nullptr, c->stream, &bad) != 0) {
                 rc = -1;

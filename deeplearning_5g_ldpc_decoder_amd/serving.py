@@ -582,13 +582,19 @@ class NativeServer:
 
     def __init__(self, pool: RequestPool, harq: HarqSlots, llr_pool: DevicePool,
                  row=DEFAULT_ROW, n_contexts=4, max_batch_codeblocks=256, deadline_s=0.5e-3,
-                 harq_rtt_s=1.0e-3, record_hard=False, llr_on_device=False):
+                 harq_rtt_s=1.0e-3, record_hard=False, llr_source="host_copy"):
         if pool.pinned is None:
             raise ValueError("the served loop needs the request LLRs in pinned memory")
-        # llr_on_device: the request LLRs sit in HBM, as a demapper on the GPU
-        # would leave them, and batches copy them device to device
-        self.llr_src = pool.pinned.cuda() if llr_on_device else pool.pinned
-        self.llr_on_device = bool(llr_on_device)
+        # llr_source "device": the request LLRs sit in HBM, as a demapper on
+        # the GPU would leave them, and batches decode them in place;
+        # "host_mapped": batches decode the pinned host LLRs in place, the
+        # kernel reading them over PCIe; "host_copy": each batch copies them
+        # from pinned host memory into its region of the device LLR pool (the
+        # pool goes unused in place)
+        if llr_source not in _lib.LLR_SOURCES:
+            raise ValueError(f"llr_source: one of {sorted(_lib.LLR_SOURCES)}")
+        self.llr_src = pool.pinned.cuda() if llr_source == "device" else pool.pinned
+        self.llr_source = llr_source
         self.pool, self.harq, self.llr_pool = pool, harq, llr_pool
         self.row = tuple(int(v) for v in row)
         self.n_contexts = int(n_contexts)
@@ -630,7 +636,7 @@ class NativeServer:
         hard = np.zeros(max(hard_cap, 1), np.uint8)
         cfg = _lib.ServeCfg(n_contexts=self.n_contexts, max_batch_codeblocks=self.max_cbs,
                             n_ues=self.harq.n_ues, n_procs=self.harq.n_procs,
-                            max_cbs=self.harq.max_cbs, llr_on_device=int(self.llr_on_device),
+                            max_cbs=self.harq.max_cbs, llr_source=_lib.LLR_SOURCES[self.llr_source],
                             deadline_s=self.deadline_s,
                             harq_rtt_s=self.harq_rtt_s, max_wall_s=max_wall_s,
                             llr_pool_elems=self.llr_pool.elems)

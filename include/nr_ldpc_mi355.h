@@ -327,12 +327,24 @@ typedef struct {
     int32_t n_ues, n_procs;        /* HARQ processes per UE */
     int32_t max_cbs;               /* codeblocks per request at most: soft slot (ue, proc, cb)
                                     * is ((ue * n_procs) + proc) * max_cbs + cb */
-    int32_t llr_on_device;         /* 0: llr_host is pinned host memory; 1: a device array */
+    int32_t llr_source;            /* where llr_host is and how batches read it: one of
+                                    * LDPC_MI355_LLR_* below */
     double deadline_s;             /* recorded with each request (not enforced) */
     double harq_rtt_s;             /* a failed transmission's next rv arrives this much later */
     double max_wall_s;             /* the loop stops after this long */
-    uint64_t llr_pool_elems;       /* admission budget: int16 elements of llr_pool */
+    uint64_t llr_pool_elems;       /* admission budget: int16 elements of llr_pool (host LLRs) */
 } ldpc_mi355_serve_cfg_t;
+
+/* llr_source values.  HOST_COPY: llr_host is pinned host memory; each
+ * batch copies its requests' spans into a region of llr_pool, which
+ * admission budgets.  DEVICE: llr_host is device memory (as a demapper on
+ * the GPU leaves it); batches decode it in place.  HOST_MAPPED: llr_host is
+ * pinned host memory (hipHostMalloc / torch pin_memory); batches decode it
+ * in place, the kernel reading it over PCIe.  In place, llr_pool is unused
+ * and every LLR offset must be below 2^32 elements. */
+#define LDPC_MI355_LLR_HOST_COPY 0
+#define LDPC_MI355_LLR_DEVICE 1
+#define LDPC_MI355_LLR_HOST_MAPPED 2
 
 /* A request template: n_cbs codeblocks with configs cb_cfg[cb_first ..]
  * (their rv and HARQ_on are set per transmission); the LLRs of redundancy
@@ -367,9 +379,10 @@ typedef struct {
  * UE) wait for a free HARQ process of their UE, then queue; whenever one of
  * n_contexts batch contexts is free, the queue head is admitted as one batch
  * of whole requests within max_batch_codeblocks and a first-fit region of
- * llr_pool (device, llr_pool_elems int16); each batch is copied from the
- * request LLRs (llr_host: pinned host memory, or device memory with
- * llr_on_device) into its region, decoded with decoder_config
+ * llr_pool (device, llr_pool_elems int16) into which its request LLRs
+ * (llr_host) are copied -- with llr_source DEVICE or HOST_MAPPED each
+ * batch reads its requests' LLRs where they are, with no region and no
+ * copy -- decoded with decoder_config
  * and its results copied back asynchronously on its context's stream; a
  * transmission with a failed codeblock is retransmitted with the next rv of
  * 0, 2, 3, 1 after harq_rtt_s and combined into the same soft slots of

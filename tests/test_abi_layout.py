@@ -6,9 +6,11 @@ ldpc_decoder_cfg_t, the sizes of the exported base-graph tables and
 HARQ_memory, the header constants, and -- by compiling function-pointer
 assignments with -Werror -- the types of the six functions the reference's
 Python callers bind."""
+import ctypes
 import json
 import os
 import shutil
+import subprocess
 import sys
 
 import pytest
@@ -26,3 +28,59 @@ def test_header_layout_equals_reference():
     want.pop("source")
     got = probe(os.path.join(ROOT, "include"), "nr_ldpc_mi355.h")
     assert got == want
+
+
+def _ctypes_fields(struct):
+    out = []
+    for name, typ in struct._fields_:
+        out.append((name, getattr(struct, name).offset, ctypes.sizeof(typ)))
+    return out
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_library_structs_match_ctypes_mirror(tmp_path):
+    """The structs the library adds to the boundary (batch descriptors and the
+    served loop's config, templates, records and statistics), compiled from
+    include/nr_ldpc_mi355.h by gcc, have the sizes and field offsets of their
+    ctypes / numpy mirrors in the Python package; the llr_source constants
+    equal the package's."""
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    rec = _lib.SERVE_RECORD_DTYPE
+    mirrors = {
+        "ldpc_mi355_cb_t": _ctypes_fields(_lib.CbDescriptor),
+        "ldpc_mi355_serve_cfg_t": _ctypes_fields(_lib.ServeCfg),
+        "ldpc_mi355_serve_template_t": _ctypes_fields(_lib.ServeTemplate),
+        "ldpc_mi355_serve_stats_t": _ctypes_fields(_lib.ServeStatsC),
+        "ldpc_mi355_serve_record_t": [(n, rec.fields[n][1], rec.fields[n][0].itemsize)
+                                      for n in rec.names],
+    }
+    sizes = {"ldpc_mi355_cb_t": ctypes.sizeof(_lib.CbDescriptor),
+             "ldpc_mi355_serve_cfg_t": ctypes.sizeof(_lib.ServeCfg),
+             "ldpc_mi355_serve_template_t": ctypes.sizeof(_lib.ServeTemplate),
+             "ldpc_mi355_serve_stats_t": ctypes.sizeof(_lib.ServeStatsC),
+             "ldpc_mi355_serve_record_t": rec.itemsize}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "nr_ldpc_mi355.h"',
+             "int main(void) {"]
+    for t, fields in mirrors.items():
+        lines.append(f'    printf("{t} %zu\\n", sizeof({t}));')
+        for n, _, _ in fields:
+            lines.append(f'    printf("{t}.{n} %zu %zu\\n", offsetof({t}, {n}), '
+                         f'sizeof((({t} *)0)->{n}));')
+    for name, v in _lib.LLR_SOURCES.items():
+        lines.append(f'    printf("LDPC_MI355_LLR_{name.upper()} %d\\n", '
+                     f'LDPC_MI355_LLR_{name.upper()});')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "ext_probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "ext_probe"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe)], check=True)
+    got = dict(line.split(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    for t, fields in mirrors.items():
+        assert int(got[t]) == sizes[t], t
+        for n, off, size in fields:
+            assert got[f"{t}.{n}"] == f"{off} {size}", (t, n)
+    for name, v in _lib.LLR_SOURCES.items():
+        assert int(got[f"LDPC_MI355_LLR_{name.upper()}"]) == v

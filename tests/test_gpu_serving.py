@@ -76,13 +76,14 @@ def test_serving_results_same_alone_and_in_the_mix(oracle_mod):
             pos += ln
 
 
-@pytest.mark.parametrize("llr_on_device", [False, True])
-def test_native_serving_results_same_alone_and_in_the_mix(oracle_mod, llr_on_device):
+@pytest.mark.parametrize("llr_source", ["host_copy", "device", "host_mapped"])
+def test_native_serving_results_same_alone_and_in_the_mix(oracle_mod, llr_source):
     """The C++ served loop (ldpc_mi355_serve_run) on Poisson arrivals with
-    HARQ retransmissions over 4 contexts, request LLRs in pinned host memory
-    or in HBM: every transmission, replayed alone in the same per-process
-    order with fresh HARQ buffers, gives the same status, iterations and hard
-    bits; first transmissions match the oracle."""
+    HARQ retransmissions over 4 contexts, request LLRs copied from pinned host
+    memory, read in place in HBM or read in place in pinned host memory: every
+    transmission, replayed alone in the same per-process order with fresh HARQ
+    buffers, gives the same status, iterations and hard bits; first
+    transmissions match the oracle."""
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from deeplearning_5g_ldpc_decoder_amd.serving import (RV_SEQUENCE, DevicePool, GpuExecutor,
@@ -93,7 +94,7 @@ def test_native_serving_results_same_alone_and_in_the_mix(oracle_mod, llr_on_dev
     harq = HarqSlots(16, 4, 16)
     llr_pool = DevicePool(64 << 20)
     srv = NativeServer(pool, harq, llr_pool, n_contexts=4, max_batch_codeblocks=96,
-                       harq_rtt_s=0.3e-3, record_hard=True, llr_on_device=llr_on_device)
+                       harq_rtt_s=0.3e-3, record_hard=True, llr_source=llr_source)
     rng = np.random.default_rng(6)
     arr = poisson_arrivals(20000.0, 0.02, seed=6)
     _say(f"native serving: {len(arr)} arrivals")
