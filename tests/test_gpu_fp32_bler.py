@@ -16,8 +16,9 @@ Algo 4:
 
 Algo 0 (device tanhf/atanhf, which differ from glibc's in the last ulp):
 * status, iterations and hard decisions equal to the reference's FP32 BP,
-  posteriors within 1e-3 * max|LLR| of it but for at most 2 beliefs per
-  codeblock, and no farther from FP64 than the reference's own FP32 BP
+  posteriors within 1e-3 * max|LLR| of it but for the beliefs of at most one
+  check per codeblock (same sign, within one saturation jump), and no
+  farther from FP64 than the reference's own FP32 BP
   (test_gpu_parity._bp_matches_reference states why FP32 BP and FP64 part
   once beliefs pass the FP32 tanh saturation).
 
