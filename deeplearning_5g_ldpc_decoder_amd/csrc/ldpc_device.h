@@ -59,17 +59,26 @@ struct DecParams {
     const uint32_t *scale[2];
 };
 
-// A packed tile: `count` codeblocks of one (BG, Zc) share one workgroup of
-// kMaxThreads lanes, codeblock i on lanes [i*Zc, (i+1)*Zc) with its beliefs in
-// its own LDS region.  order[first .. first+count) are their indices in the
-// codeblock descriptor array.
-constexpr int kMaxTileCodeblocks = kMaxThreads / 2;
+// Packed tiles: a workgroup of kTileThreads lanes (kTileWaves waves, two per
+// SIMD) holds groups of codeblocks of one base graph.  A group is `count`
+// codeblocks of one lifting size on whole waves starting at lane lane_base,
+// codeblock i on lanes lane_base + [i*Zc, (i+1)*Zc), with its beliefs in its
+// own LDS region at lds_base + i*lds_stride; order[first .. first+count) are
+// their indices in the codeblock descriptor array, and slot_base + i their
+// CRC slots.  A workgroup's descriptor is kTileWaves TileDesc, one per wave:
+// each wave reads the descriptor of the group it belongs to (waves outside
+// any group: count 0).  All kTileWaves entries carry the workgroup's bgn.
+constexpr int kTileWaves = 8;
+constexpr int kTileThreads = kTileWaves * 64;
+constexpr int kMaxTileCodeblocks = kTileThreads / 2;
 struct TileDesc {
     uint32_t first;
     uint32_t etab_off;
+    uint32_t lds_base;                     // LQ entries before the group's first codeblock
     uint16_t count, bgn, Zc, lds_stride;   // lds_stride: LQ entries per codeblock
+    uint16_t lane_base, slot_base;
 };
-static_assert(sizeof(TileDesc) == 16, "TileDesc layout");
+static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 
 // Per-codeblock result flags (status) follow c_ldpc_5g_decode: 0 CRC pass,
 // 1 not converged.

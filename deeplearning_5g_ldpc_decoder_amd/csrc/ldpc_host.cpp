@@ -320,6 +320,23 @@ CbDesc codeblock_desc(const Params &p, uint32_t etab_off) {
     return d;
 }
 
+// Waves of one group of lifting size zc: the fewest waves (at most 6) whose
+// codeblocks fill at least 7/8 of their lanes, else the best-filled count.
+static int group_waves(int zc) {
+    int best = 0, best_used = 0;
+    for (int w = 1; w <= 6; ++w) {
+        const int k = 64 * w / zc;
+        if (k < 1) continue;
+        const int used = k * zc;
+        if (8 * used >= 7 * 64 * w) return w;
+        if (best == 0 || used * 64 * best > best_used * 64 * w) {
+            best = w;
+            best_used = used;
+        }
+    }
+    return best;
+}
+
 void build_tiles(const std::vector<CbDesc> &descs, const uint32_t (*etab_off)[kMaxZc + 1],
                  int (*k_of)(int bgn, int zc), std::vector<TileDesc> &tiles,
                  std::vector<uint32_t> &order, int &lds_entries) {
@@ -329,32 +346,107 @@ void build_tiles(const std::vector<CbDesc> &descs, const uint32_t (*etab_off)[kM
         if (descs[a].bgn != descs[b].bgn) return descs[a].bgn < descs[b].bgn;
         return descs[a].Zc > descs[b].Zc;
     });
-    tiles.clear();
-    lds_entries = 0;
+    // groups: up to k codeblocks of one (BG, Zc) on whole waves
+    struct Group {
+        uint32_t first;
+        uint16_t count, bgn, Zc, lds_stride, waves;
+    };
+    std::vector<Group> groups;
     size_t i = 0;
     while (i < idx.size()) {
         const CbDesc &d0 = descs[idx[i]];
-        const int k = k_of(d0.bgn, d0.Zc);
+        const int w0 = group_waves(d0.Zc);
+        const int k = std::max(1, std::min(64 * w0 / (int)d0.Zc, k_of(d0.bgn, d0.Zc)));
         size_t j = i;
         while (j < idx.size() && j - i < (size_t)k && descs[idx[j]].bgn == d0.bgn &&
                descs[idx[j]].Zc == d0.Zc)
             ++j;
-        TileDesc t;
-        t.first = (uint32_t)i;
-        t.etab_off = etab_off[d0.bgn - 1][d0.Zc];
-        t.count = (uint16_t)(j - i);
-        t.bgn = d0.bgn;
-        t.Zc = d0.Zc;
-        t.lds_stride = (uint16_t)(((d0.N + 2 * d0.Zc) + 7) & ~7);
-        lds_entries = std::max(lds_entries, (int)t.count * (int)t.lds_stride);
-        tiles.push_back(t);
+        Group g;
+        g.first = (uint32_t)i;
+        g.count = (uint16_t)(j - i);
+        g.bgn = d0.bgn;
+        g.Zc = d0.Zc;
+        g.lds_stride = (uint16_t)(((d0.N + 2 * d0.Zc) + 7) & ~7);
+        g.waves = (uint16_t)((g.count * d0.Zc + 63) / 64);
+        groups.push_back(g);
         i = j;
     }
-    order = idx;
-    std::stable_sort(tiles.begin(), tiles.end(), [](const TileDesc &a, const TileDesc &b) {
+    // workgroups: per base graph, groups by decreasing waves into the open
+    // workgroup with the least room that holds them (kTileWaves waves and
+    // kMaxTileCodeblocks CRC slots each)
+    std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) {
         if (a.bgn != b.bgn) return a.bgn < b.bgn;
-        return a.count * a.Zc > b.count * b.Zc;
+        return a.waves > b.waves;
     });
+    struct Wg {
+        int bgn, waves = 0, slots = 0;
+        std::vector<int> members;
+    };
+    std::vector<Wg> wgs;
+    std::vector<int> open[kTileWaves];   // open workgroups by free waves - 1
+    int cur_bgn = 0;
+    for (int gi = 0; gi < (int)groups.size(); ++gi) {
+        const Group &g = groups[(size_t)gi];
+        if (g.bgn != cur_bgn) {
+            for (auto &o : open) o.clear();
+            cur_bgn = g.bgn;
+        }
+        int chosen = -1;
+        for (int f = g.waves; f <= kTileWaves && chosen < 0; ++f) {
+            auto &lst = open[f - 1];
+            for (size_t q = 0; q < lst.size(); ++q)
+                if (wgs[(size_t)lst[q]].slots + g.count <= kMaxTileCodeblocks) {
+                    chosen = lst[q];
+                    lst.erase(lst.begin() + (long)q);
+                    break;
+                }
+        }
+        if (chosen < 0) {
+            chosen = (int)wgs.size();
+            wgs.push_back(Wg());
+            wgs.back().bgn = g.bgn;
+        }
+        Wg &w = wgs[(size_t)chosen];
+        w.members.push_back(gi);
+        w.waves += g.waves;
+        w.slots += g.count;
+        const int free_waves = kTileWaves - w.waves;
+        if (free_waves > 0 && w.slots < kMaxTileCodeblocks) open[free_waves - 1].push_back(chosen);
+    }
+    tiles.assign(wgs.size() * kTileWaves, TileDesc());
+    lds_entries = 0;
+    for (size_t wi = 0; wi < wgs.size(); ++wi) {
+        const Wg &w = wgs[wi];
+        TileDesc *t = &tiles[wi * kTileWaves];
+        int wave = 0, slot = 0, lds = 0;
+        for (int gi : w.members) {
+            const Group &g = groups[(size_t)gi];
+            TileDesc d;
+            d.first = g.first;
+            d.etab_off = etab_off[g.bgn - 1][g.Zc];
+            d.lds_base = (uint32_t)lds;
+            d.count = g.count;
+            d.bgn = g.bgn;
+            d.Zc = g.Zc;
+            d.lds_stride = g.lds_stride;
+            d.lane_base = (uint16_t)(wave * 64);
+            d.slot_base = (uint16_t)slot;
+            for (int v = wave; v < wave + g.waves; ++v) t[v] = d;
+            wave += g.waves;
+            slot += g.count;
+            lds += g.count * g.lds_stride;
+        }
+        // waves outside any group run the first group's rows with no
+        // codeblock: their lanes address that group's first LQ region, as a
+        // group's lanes past its last codeblock do
+        for (int v = wave; v < kTileWaves; ++v) {
+            t[v] = t[0];
+            t[v].count = 0;
+            t[v].lane_base = (uint16_t)(v * 64);
+        }
+        lds_entries = std::max(lds_entries, lds);
+    }
+    order = idx;
 }
 
 void edge_table(int bgn, int zc, int ils, uint32_t *out) {

@@ -73,10 +73,13 @@ void make_scale_set(const ldpc_decoder_cfg_t &c, ScaleSet &s);
 // (the reference reads past the table); true for other algos.
 bool scale_iterations_ok(const ScaleSet &s, int bgn, int max_iterations);
 CbDesc codeblock_desc(const Params &p, uint32_t etab_off);
-// Groups codeblock descriptors by (BG, Zc) into packed tiles of up to
-// k_of(bgn, zc) codeblocks; order receives descriptor indices tile by tile,
-// lds_entries the largest tile's LQ footprint.  Tiles of base graph 1 (more
-// rows) come first, then fuller tiles, so the longest work starts earliest.
+// Packs codeblock descriptors into tile workgroups: codeblocks of one
+// (BG, Zc) form groups of whole waves (the fewest waves, up to 6, that the
+// codeblocks fill to 7/8, and at most k_of(bgn, zc) codeblocks), and groups
+// of one base graph are packed by decreasing size into workgroups of
+// kTileWaves waves.  tiles receives kTileWaves descriptors per workgroup,
+// order the descriptor indices group by group, lds_entries the largest
+// workgroup's LQ footprint.  Base graph 1's workgroups come first.
 void build_tiles(const std::vector<CbDesc> &descs, const uint32_t (*etab_off)[kMaxZc + 1],
                  int (*k_of)(int bgn, int zc), std::vector<TileDesc> &tiles,
                  std::vector<uint32_t> &order, int &lds_entries);

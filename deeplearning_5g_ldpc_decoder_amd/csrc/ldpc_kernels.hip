@@ -45,9 +45,12 @@
 #ifdef NRLDPC_CPU_EMU
 #define NRLDPC_OPAQUE_LANE(x) (void)(x)
 #define NRLDPC_OPAQUE_UNIFORM(x) (void)(x)
+#define NRLDPC_WAVE_UNIFORM(x) (x)
 #else
 #define NRLDPC_OPAQUE_LANE(x) asm volatile("" : "+v"(x))
 #define NRLDPC_OPAQUE_UNIFORM(x) asm volatile("" : "+s"(x))
+// a value equal on every lane of the wave, moved to a scalar register
+#define NRLDPC_WAVE_UNIFORM(x) __builtin_amdgcn_readfirstlane(x)
 #endif
 
 namespace nrldpc {
@@ -1408,7 +1411,7 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
 // right after each CRC barrier, when every lane has read it.
 // ---------------------------------------------------------------------------
 template <int BG, bool kGeneral, bool kTable>
-__device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int cbl,
+__device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int slot,
                              bool member, const CrcCtx &crc, const uint32_t *tab,
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
                              int *status, int *iters) {
@@ -1437,37 +1440,43 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
                 const uint32_t byte = hard_byte(lq, k);
                 r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
             }
-            if (r) atomicXor(&bank[b][cbl], crc_mulmod_fast(r, crc.mult, c));
+            if (r) atomicXor(&bank[b][slot], crc_mulmod_fast(r, crc.mult, c));
         }
         __syncthreads();
         if (!done) {
             *iters = it + 1;
-            if (bank[b][cbl] == 0u) {
+            if (bank[b][slot] == 0u) {
                 done = true;
                 *status = 0;
             }
         }
-        if (member && zx == 0) bank[b ^ 1][cbl] = 0u;
+        if (member && zx == 0) bank[b ^ 1][slot] = 0u;
         tile_active = __syncthreads_or(!done);
     }
 }
 
+// One workgroup of kTileThreads lanes per tile; each wave reads the
+// descriptor of its own group (TileDesc).  Every group of a workgroup has the
+// same base graph, so all waves run the same rows and barriers.
 template <bool kGeneral, bool kTable>
-__global__ void __launch_bounds__(kMaxThreads)
+__global__ void __launch_bounds__(kTileThreads)
 ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p) {
     NRLDPC_DYNAMIC_LDS(smem);
     __shared__ uint32_t crc_tab[4][256];
     __shared__ uint32_t bank[2][kMaxTileCodeblocks];
 
-    const TileDesc td = tiles[blockIdx.x];
-    const int Zc = td.Zc, lane = threadIdx.x, nt = blockDim.x;
-    const int cbl = lane / Zc;
+    const int lane = threadIdx.x, nt = blockDim.x;
+    const int wave = NRLDPC_WAVE_UNIFORM(lane >> 6);
+    const TileDesc td = tiles[blockIdx.x * kTileWaves + wave];
+    const int Zc = td.Zc, local = lane - td.lane_base;
+    const int cbl = local / Zc;
     const bool member = cbl < td.count;
-    const int zx = lane - cbl * Zc;
+    const int zx = local - cbl * Zc;
+    const int slot = member ? td.slot_base + cbl : 0;
     const int cbi = member ? (int)order[td.first + cbl] : 0;
     const CbDesc d = o.desc[cbi];
     const int n = d.N + 2 * Zc;
-    int16_t *lq = reinterpret_cast<int16_t *>(smem) + (member ? cbl : 0) * td.lds_stride;
+    int16_t *lq = reinterpret_cast<int16_t *>(smem) + td.lds_base + (member ? cbl : 0) * td.lds_stride;
 
     // CRC tables of all four polynomials (codeblocks may differ in crctype)
     for (int i = lane; i < 4 * 256; i += nt) crc_tab[i >> 8][i & 255] = o.crc[(i >> 8) * kCrcTabWords + (i & 255)];
@@ -1493,10 +1502,10 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     const uint32_t *tab = crc_tab[member ? d.crctype : 0];
     if (p.max_it > 0) {
         if (td.bgn == 1)
-            run_tile_i16<1, kGeneral, kTable>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p,
+            run_tile_i16<1, kGeneral, kTable>(lq, etab, zx, Zc, slot, member, crc, tab, bank, p,
                                               &status, &iters);
         else
-            run_tile_i16<2, kGeneral, kTable>(lq, etab, zx, Zc, cbl, member, crc, tab, bank, p,
+            run_tile_i16<2, kGeneral, kTable>(lq, etab, zx, Zc, slot, member, crc, tab, bank, p,
                                               &status, &iters);
     }
     if (!member) return;
@@ -1658,8 +1667,8 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     if (e != hipSuccess) return e;
     e = nrldpc_configure_bp();
     if (e != hipSuccess) return e;
-    // tiles hold at most 384 lanes' worth of beliefs (~56 KB) next to 5.6 KB
-    // of static LDS
+    // a tile workgroup holds at most 512 lanes' worth of beliefs (~70 KB for
+    // base graph 1) next to 6 KB of static LDS
     const int tile_lds = 96 * 1024;
     for (int g = 0; g < 2 && e == hipSuccess; ++g)
         for (int t = 0; t < 2 && e == hipSuccess; ++t)
@@ -1668,7 +1677,7 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     return e;
 }
 
-// Packed-tile launch (algos 1-3): ntiles workgroups of kMaxThreads lanes.
+// Packed-tile launch (algos 1-3): ntiles workgroups of kTileThreads lanes.
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
                                           const nrldpc::TileDesc *tiles, const uint32_t *order,
                                           const nrldpc::DecParams *p, int ntiles,
@@ -1678,7 +1687,8 @@ extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
     if (algo != 1 && (p->scale_iters < 1 || !p->scale[0] || !p->scale[1]))
         return hipErrorInvalidValue;
     const size_t lds = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
-    const dim3 grid(ntiles), block(kMaxThreads);
+    if ((size_t)lds_entries * sizeof(int16_t) > 90 * 1024) return hipErrorInvalidValue;
+    const dim3 grid(ntiles), block(kTileThreads);
     DecOut ov = *o;
     DecParams pv = *p;
     void *args[] = {&ov, &tiles, &order, &pv};

@@ -449,7 +449,7 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
         std::vector<TileDesc> tiles;
         std::vector<uint32_t> order;
         build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, pl->tile_lds);
-        pl->ntiles = (int)tiles.size();
+        pl->ntiles = (int)(tiles.size() / kTileWaves);
         if (hipMalloc(&pl->d_tiles, sizeof(TileDesc) * tiles.size()) != hipSuccess ||
             hipMemcpy(pl->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
                       hipMemcpyHostToDevice) != hipSuccess ||
@@ -548,7 +548,8 @@ struct ldpc_mi355_batch {
 namespace {
 size_t batch_stage_bytes(int capacity) {
     const size_t d = ((sizeof(CbDesc) * (size_t)capacity) + 255) & ~(size_t)255;
-    const size_t t = ((sizeof(TileDesc) * (size_t)capacity) + 255) & ~(size_t)255;
+    // at most one tile workgroup per codeblock
+    const size_t t = ((sizeof(TileDesc) * kTileWaves * (size_t)capacity) + 255) & ~(size_t)255;
     return d + t + sizeof(uint32_t) * (size_t)capacity;
 }
 }  // namespace
@@ -619,7 +620,8 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
     }
     const size_t doff = 0;
     const size_t toff = ((sizeof(CbDesc) * (size_t)b->capacity) + 255) & ~(size_t)255;
-    const size_t ooff = toff + (((sizeof(TileDesc) * (size_t)b->capacity) + 255) & ~(size_t)255);
+    const size_t ooff =
+        toff + (((sizeof(TileDesc) * kTileWaves * (size_t)b->capacity) + 255) & ~(size_t)255);
     memcpy(b->h_stage + doff, L.descs.data(), sizeof(CbDesc) * (size_t)n);
     size_t bytes = sizeof(CbDesc) * (size_t)n;
     int tile_lds = 0;
@@ -627,7 +629,7 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
         std::vector<TileDesc> tiles;
         std::vector<uint32_t> order;
         build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, tile_lds);
-        b->ntiles = (int)tiles.size();
+        b->ntiles = (int)(tiles.size() / kTileWaves);
         memcpy(b->h_stage + toff, tiles.data(), sizeof(TileDesc) * tiles.size());
         memcpy(b->h_stage + ooff, order.data(), sizeof(uint32_t) * order.size());
         bytes = ooff + sizeof(uint32_t) * order.size();

@@ -20,6 +20,8 @@ def lib():
         L.emu_decode_batch.restype = ctypes.c_int
         L.emu_decode_tiles.argtypes = [ctypes.c_int] + [P] * 12 + [ctypes.c_int]
         L.emu_decode_tiles.restype = ctypes.c_int
+        L.emu_plan_tiles.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P, P]
+        L.emu_plan_tiles.restype = ctypes.c_int
         I, F = ctypes.c_int, ctypes.c_float
         L.emu_unrolled_forward.argtypes = [I, I, I, I, I, F, I, I, I] + [P] * 6
         L.emu_unrolled_forward.restype = I
@@ -179,3 +181,24 @@ def force_general(g):
 def use_fixed(on):
     """Whether uniform batches run the lifting-size specialized kernel."""
     lib().emu_use_fixed(int(bool(on)))
+
+
+TILE_DESC = np.dtype([("first", "<u4"), ("etab_off", "<u4"), ("lds_base", "<u4"),
+                      ("count", "<u2"), ("bgn", "<u2"), ("Zc", "<u2"), ("lds_stride", "<u2"),
+                      ("lane_base", "<u2"), ("slot_base", "<u2")])
+TILE_WAVES = 8
+
+
+def plan_tiles(cfgs, k=0):
+    """The tile plan of a batch of configs: (descriptors [workgroups, 8] as
+    TILE_DESC, order, largest LQ footprint in entries)."""
+    cf = np.ascontiguousarray(np.array(cfgs, np.int32).reshape(-1, 7))
+    n = len(cf)
+    cap = TILE_WAVES * max(n, 1)
+    tiles = np.zeros(cap, TILE_DESC)
+    order = np.zeros(max(n, 1), np.uint32)
+    lds = ctypes.c_int(0)
+    rc = lib().emu_plan_tiles(n, _p(cf), k, _p(tiles), cap, _p(order), ctypes.byref(lds))
+    if rc < 0:
+        raise RuntimeError(f"emu_plan_tiles returned {rc}")
+    return tiles[:rc * TILE_WAVES].reshape(rc, TILE_WAVES), order[:n], lds.value

@@ -200,8 +200,32 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     }
 }
 
+// The tile plan of a batch (build_tiles, as plans and batch contexts build
+// it) with k codeblocks per group at most (0 = as many as fit): up to
+// max_entries TileDesc (kTileWaves per workgroup) into tiles, n indices into
+// order, the largest LQ footprint into *lds_entries.  Returns the number of
+// workgroups, -1 when max_entries is too small, or another negative code.
+extern "C" int emu_plan_tiles(int n, const int32_t *cfgs, int k, TileDesc *tiles_out,
+                              int max_entries, uint32_t *order_out, int *lds_entries) {
+    std::vector<uint32_t> llr_off((size_t)n, 0u), hard_off((size_t)n), lq_off((size_t)n);
+    const int32_t row[7] = {1, 8, 22936, 0, 1280, 32767, -32767};
+    Batch b;
+    if (int rc = make_batch(n, cfgs, llr_off.data(), row, nullptr, false, hard_off.data(),
+                            lq_off.data(), b))
+        return rc < 0 ? rc : -rc;
+    std::vector<TileDesc> tiles;
+    std::vector<uint32_t> order;
+    g_fixed_k = k;
+    build_tiles(b.descs, tables().off, fixed_k, tiles, order, *lds_entries);
+    if ((int)tiles.size() > max_entries) return -1;
+    memcpy(tiles_out, tiles.data(), sizeof(TileDesc) * tiles.size());
+    memcpy(order_out, order.data(), sizeof(uint32_t) * order.size());
+    return (int)(tiles.size() / kTileWaves);
+}
+
 // Same batch through the packed-tile kernel (algos 1-3) with k codeblocks
-// per tile (0 = as many as fit).  Returns the number of tiles, or < 0.
+// per group at most (0 = as many as fit).  Returns the number of tile
+// workgroups, or < 0.
 extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_off,
                                 const int32_t *row, const int16_t *llr, int16_t *soft_pool,
                                 const int32_t *soft_slot, uint8_t *hard, int32_t *status,
@@ -222,17 +246,17 @@ extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_
     const uint32_t *ord = order.data();
     const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
     const bool t = row[0] != 1;
-    const unsigned nt = (unsigned)tiles.size();
+    const unsigned nt = (unsigned)(tiles.size() / kTileWaves);
     const size_t lds = (size_t)lds_entries * 2;
     if (!g && !t)
-        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p); });
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p); });
     else if (g && !t)
-        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p); });
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p); });
     else if (!g && t)
-        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<false, true>(o, td, ord, p); });
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true>(o, td, ord, p); });
     else
-        emu_launch(nt, kMaxThreads, lds, [&] { ldpc_tile_kernel<true, true>(o, td, ord, p); });
-    return (int)tiles.size();
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<true, true>(o, td, ord, p); });
+    return (int)nt;
 }
 
 // The library's process-wide algo-2 / algo-3 tables (this build's copy).

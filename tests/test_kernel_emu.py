@@ -246,9 +246,10 @@ def _all_zc_batch(oracle_mod, rng, per_zc=2, zcs=ALL_ZC):
 
 @pytest.mark.parametrize("tile_k", [0, 3])
 def test_emu_packed_tiles_lifting_sizes(oracle_mod, tile_k):
-    """Packed tiles (several codeblocks of one BG/Zc per 384-lane workgroup,
-    per-codeblock CRC and early stop) over every reachable lifting size,
-    bit-exact with the oracle per codeblock."""
+    """Packed tiles (groups of codeblocks of one BG/Zc on whole waves, groups
+    of several lifting sizes per 512-lane workgroup, per-codeblock CRC and
+    early stop) over every reachable lifting size, bit-exact with the oracle
+    per codeblock."""
     rng = np.random.default_rng(31 + tile_k)
     cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=3, zcs=SOME_ZC)
     assert len(cfgs) >= 50
@@ -462,3 +463,55 @@ def test_emu_rate_recovery_matches_reference():
         row = [1, 0, 22936, 0, filler, 32767, -32767]
         emu.decode_batch([cfg], row, [llr], soft_pool=pool, soft_slots=[0])
         assert np.array_equal(pool[:len(want)], want), cfg
+
+
+def test_tile_plan_invariants(oracle_mod):
+    """The tile planner on 2048 mixed configs (the mixed-size workload's
+    generator): every codeblock in exactly one group; a group's codeblocks
+    share (BG, Zc) and sit on whole waves of one workgroup without overlap;
+    all eight entries of a workgroup carry its base graph (so every wave runs
+    the same rows and barriers); CRC slots and LDS regions of a workgroup are
+    disjoint and within kMaxTileCodeblocks / the reported footprint; and
+    workgroups do hold groups of different lifting sizes."""
+    from deeplearning_5g_ldpc_decoder_amd.workloads import draw_config
+    rng = np.random.default_rng(12)
+    cfgs = [draw_config(rng)[0] for _ in range(2048)]
+    tiles, order, lds = emu.plan_tiles(cfgs)
+    codes = [oracle_mod.code(*c) for c in cfgs]
+    seen = np.zeros(len(cfgs), int)
+    mixed_wgs = 0
+    for wg in tiles:
+        assert len(set(int(t["bgn"]) for t in wg)) == 1
+        groups = {}
+        for v, t in enumerate(wg):
+            if t["count"] == 0:   # a wave outside any group: the first group's rows
+                assert t["lane_base"] == 64 * v and v > 0
+                assert (t["Zc"], t["etab_off"], t["lds_base"]) == (wg[0]["Zc"], wg[0]["etab_off"],
+                                                                   wg[0]["lds_base"])
+                continue
+            key = (int(t["first"]), int(t["lane_base"]))
+            groups.setdefault(key, []).append(v)
+        slots, regions, zcs = [], [], set()
+        for (first, lane_base), waves in groups.items():
+            t = wg[waves[0]]
+            assert all(wg[v] == t for v in waves)
+            assert lane_base == 64 * waves[0] and waves == list(range(waves[0], waves[-1] + 1))
+            cnt, zc = int(t["count"]), int(t["Zc"])
+            assert cnt * zc <= 64 * len(waves) < cnt * zc + 64
+            zcs.add(zc)
+            for i in range(cnt):
+                c = codes[order[first + i]]
+                assert (c.bgn, c.Zc) == (int(t["bgn"]), zc)
+                assert int(t["lds_stride"]) >= c.N + 2 * c.Zc
+                seen[order[first + i]] += 1
+            slots.append((int(t["slot_base"]), cnt))
+            regions.append((int(t["lds_base"]), cnt * int(t["lds_stride"])))
+        for lst, cap in ((slots, 256), (regions, lds)):
+            lst.sort()
+            for (a, n), (b, _) in zip(lst, lst[1:]):
+                assert a + n <= b
+            assert not lst or lst[-1][0] + lst[-1][1] <= cap
+        mixed_wgs += len(zcs) > 1
+    assert np.all(seen == 1)
+    assert mixed_wgs > 0
+    assert lds * 2 <= 90 * 1024
