@@ -194,6 +194,8 @@ def bench_mixed(args):
             res[name]["pipelined"] = {"inflight": k, "gbps": info * args.steps / pwall / 1e9,
                                       "ms_per_step": pwall * 1e3 / args.steps,
                                       "outputs_equal_single_stream": same}
+        if name == "packed":
+            res[name]["graph"] = mixed_graph_leg(plan, d_llr, out, info, args)
         plan.close()
         log(f"{name}: {res[name]}")
     pipe = res["packed"].get("pipelined")
@@ -208,6 +210,41 @@ def bench_mixed(args):
             "packed": res["packed"], "one_codeblock_per_workgroup": res["plain"],
             "packing_speedup": res["packed"]["gbps"] / res["plain"]["gbps"]}
     print(json.dumps(line))
+
+
+def mixed_graph_leg(plan, d_llr, ref, info, args):
+    """The packed plan's launches captured in one HIP graph (a graph of
+    `per_graph` back-to-back launches, replayed until --steps launches ran),
+    timed with events on the replay stream; every replay's outputs are the
+    eager ones."""
+    import torch
+    per_graph = 10
+    s = torch.cuda.Stream()
+    gout = plan.alloc_outputs("cuda")
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(per_graph):
+            plan.launch(d_llr, gout, stream=s.cuda_stream)
+    reps = max(1, args.steps // per_graph)
+    same = True
+    for _ in range(2):          # replays checked against the eager outputs
+        for f in ("status", "iters", "hard"):
+            gout[f].zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        same &= all(np.array_equal(gout[f].cpu().numpy(), ref[f].cpu().numpy())
+                    for f in ("status", "iters", "hard"))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(s):
+        e0.record(s)
+        for _ in range(reps):
+            g.replay()
+        e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / (reps * per_graph)
+    return {"launches_per_graph": per_graph, "replays": reps, "ms_per_launch": ms,
+            "gbps": info / (ms * 1e-3) / 1e9, "outputs_equal_eager": bool(same)}
 
 
 def bench_train(args):
