@@ -1089,7 +1089,8 @@ __device__ __forceinline__ void gpair_store(const GenLane &g, const PackedRowSta
     if constexpr (2 * I + 1 < DEG) *(int16_t *)(g.lqb + rs.ob[I]) = (int16_t)(v >> 16);
 }
 
-template <int BG, int R, int N, int... I>
+// kSimple as in packed_finalize (DecParams::simple).
+template <int BG, int R, bool kSimple, int N, int... I>
 __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N],
                                             etab_cptr etab, const DecParams &p, int alpha,
                                             int beta, std::integer_sequence<int, I...>) {
@@ -1101,12 +1102,13 @@ __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N]
     PackedRowState<pairs> rs;
     (gpair_load<e0 + 2 * I, I, deg, pairs>(g, etab, rs), ...);
     (packed_pair_in<I, I, deg, pairs>(msg, w0 + I, rs), ...);
-    const PackedScale f = packed_finalize<false>(rs, p, alpha, beta);
-    (gpair_store<I, deg, pairs>(g, rs, packed_pair_update<I, deg, pairs, false>(msg, w0 + I, rs, f)),
+    const PackedScale f = packed_finalize<kSimple>(rs, p, alpha, beta);
+    (gpair_store<I, deg, pairs>(g, rs,
+                                packed_pair_update<I, deg, pairs, kSimple>(msg, w0 + I, rs, f)),
      ...);
 }
 
-template <int BG, int R, bool kTable, int N>
+template <int BG, int R, bool kTable, bool kSimple, int N>
 __device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], etab_cptr etab,
                                        bool run, const DecParams &p, const uint32_t *ab) {
     if constexpr (R < Graph<BG>::kRows) {
@@ -1118,10 +1120,10 @@ __device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], eta
         }
         constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
         if (run)
-            gpacked_row<BG, R>(g, msg, etab, p, alpha, beta,
-                               std::make_integer_sequence<int, (deg + 1) / 2>{});
+            gpacked_row<BG, R, kSimple>(g, msg, etab, p, alpha, beta,
+                                        std::make_integer_sequence<int, (deg + 1) / 2>{});
         if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
-        gprows<BG, R + 1, kTable>(g, msg, etab, run, p, ab);
+        gprows<BG, R + 1, kTable, kSimple>(g, msg, etab, run, p, ab);
     }
 }
 
@@ -1139,14 +1141,14 @@ __device__ __forceinline__ GenLane gen_lane(int16_t *lq, int zx, int Zc) {
 // One iteration's rows, with the lane's offsets and the edge table made
 // opaque first: otherwise their loop-invariant offsets would be hoisted out
 // of the iteration loop into hundreds of registers.
-template <int BG, bool kTable, int N>
+template <int BG, bool kTable, bool kSimple, int N>
 __device__ __forceinline__ void gprows_iteration(GenLane g, uint32_t (&msg)[N], const uint32_t *etab,
                                                  bool run, const DecParams &p, const uint32_t *ab) {
     NRLDPC_OPAQUE_LANE(g.zx2);
     NRLDPC_OPAQUE_LANE(g.zz);
     etab_cptr ce = (etab_cptr)etab;
     NRLDPC_OPAQUE_UNIFORM(ce);
-    gprows<BG, 0, kTable>(g, msg, ce, run, p, ab);
+    gprows<BG, 0, kTable, kSimple>(g, msg, ce, run, p, ab);
 }
 
 template <int BG>
@@ -1173,7 +1175,8 @@ __device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const
             rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, d.Zc, zx < d.Zc, p,
                                               scale_row<BG, kTable>(p, it));
         else
-            gprows_iteration<BG, kTable>(g, msg, etab, zx < d.Zc, p, scale_row<BG, kTable>(p, it));
+            gprows_iteration<BG, kTable, false>(g, msg, etab, zx < d.Zc, p,
+                                                scale_row<BG, kTable>(p, it));
         ++it;
         if (block_crc(lq, crc, tab, slots) == 0u) {
             *status = 0;
@@ -1410,7 +1413,7 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
 // of two banks (bank = iteration parity); the bank not in use is cleared
 // right after each CRC barrier, when every lane has read it.
 // ---------------------------------------------------------------------------
-template <int BG, bool kGeneral, bool kTable>
+template <int BG, bool kGeneral, bool kTable, bool kSimple>
 __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int slot,
                              bool member, const CrcCtx &crc, const uint32_t *tab,
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
@@ -1432,7 +1435,8 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
             rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, Zc, !done, p,
                                               scale_row<BG, kTable>(p, it));
         else
-            gprows_iteration<BG, kTable>(g, msg, etab, !done, p, scale_row<BG, kTable>(p, it));
+            gprows_iteration<BG, kTable, kSimple>(g, msg, etab, !done, p,
+                                                  scale_row<BG, kTable>(p, it));
         const int b = it & 1;
         if (!done) {
             uint32_t r = 0;
@@ -1458,7 +1462,7 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
 // One workgroup of kTileThreads lanes per tile; each wave reads the
 // descriptor of its own group (TileDesc).  Every group of a workgroup has the
 // same base graph, so all waves run the same rows and barriers.
-template <bool kGeneral, bool kTable>
+template <bool kGeneral, bool kTable, bool kSimple = false>
 __global__ void __launch_bounds__(kTileThreads)
 ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p) {
     NRLDPC_DYNAMIC_LDS(smem);
@@ -1502,11 +1506,11 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     const uint32_t *tab = crc_tab[member ? d.crctype : 0];
     if (p.max_it > 0) {
         if (td.bgn == 1)
-            run_tile_i16<1, kGeneral, kTable>(lq, etab, zx, Zc, slot, member, crc, tab, bank, p,
-                                              &status, &iters);
+            run_tile_i16<1, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, crc, tab,
+                                                       bank, p, &status, &iters);
         else
-            run_tile_i16<2, kGeneral, kTable>(lq, etab, zx, Zc, slot, member, crc, tab, bank, p,
-                                              &status, &iters);
+            run_tile_i16<2, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, crc, tab,
+                                                       bank, p, &status, &iters);
     }
     if (!member) return;
     if (p.max_it > 0)
@@ -1578,12 +1582,16 @@ std::pair<const void *, int> fixed_kernel(bool general, bool table, bool simple,
     return fixed_kernel_any(general, table, simple, uniform);
 }
 
-const void *tile_kernel(bool general, bool table) {
-    if (table)
-        return general ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, true>)
-                       : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, true>);
-    return general ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, false>)
-                   : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, false>);
+// Packed-tile kernel of (general, table, simple); simple applies to fast rows.
+const void *tile_kernel(bool general, bool table, bool simple) {
+    if (general)
+        return table ? reinterpret_cast<const void *>(&ldpc_tile_kernel<true, true>)
+                     : reinterpret_cast<const void *>(&ldpc_tile_kernel<true, false>);
+    if (simple)
+        return table ? reinterpret_cast<const void *>(&ldpc_tile_kernel<false, true, true>)
+                     : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, false, true>);
+    return table ? reinterpret_cast<const void *>(&ldpc_tile_kernel<false, true>)
+                 : reinterpret_cast<const void *>(&ldpc_tile_kernel<false, false>);
 }
 }  // namespace
 
@@ -1672,8 +1680,9 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     const int tile_lds = 96 * 1024;
     for (int g = 0; g < 2 && e == hipSuccess; ++g)
         for (int t = 0; t < 2 && e == hipSuccess; ++t)
-            e = hipFuncSetAttribute(tile_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    tile_lds);
+            for (int m = 0; m < 2 && e == hipSuccess; ++m)
+                e = hipFuncSetAttribute(tile_kernel(g, t, m),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
     return e;
 }
 
@@ -1693,7 +1702,8 @@ extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
     DecParams pv = *p;
     void *args[] = {&ov, &tiles, &order, &pv};
     const hipError_t e =
-        hipLaunchKernel(tile_kernel(pv.general != 0, algo != 1), grid, block, args, lds, stream);
+        hipLaunchKernel(tile_kernel(pv.general != 0, algo != 1, pv.simple != 0), grid, block, args,
+                        lds, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

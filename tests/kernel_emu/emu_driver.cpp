@@ -248,7 +248,12 @@ extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_
     const bool t = row[0] != 1;
     const unsigned nt = (unsigned)(tiles.size() / kTileWaves);
     const size_t lds = (size_t)lds_entries * 2;
-    if (!g && !t)
+    const bool simple = p.simple != 0;
+    if (!g && !t && simple)
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false, true>(o, td, ord, p); });
+    else if (!g && t && simple)
+        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true, true>(o, td, ord, p); });
+    else if (!g && !t)
         emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p); });
     else if (g && !t)
         emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p); });
