@@ -54,13 +54,16 @@ def load_c_library(lib_path=LIB_PATH, encode_symbol="c_ldpc_5g_encode",
     chan = lib if channel_lib_path is None else ctypes.CDLL(channel_lib_path)
     u8 = ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
     i16 = ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
-    enc = getattr(lib, encode_symbol)
+    # lib[name] is a function object of its own (getattr returns the one the
+    # CDLL caches): the ndpointer argument types stay on these bindings and do
+    # not leak to other users of the package's shared library handle
+    enc = lib[encode_symbol]
     enc.argtypes = [u8, u8, ctypes.POINTER(LdpcInConfig)]
     enc.restype = ctypes.c_int32
-    dec = getattr(lib, decode_symbol)
+    dec = lib[decode_symbol]
     dec.argtypes = [i16, u8, ctypes.POINTER(LdpcInConfig), ctypes.POINTER(DecoderConfig)]
     dec.restype = ctypes.c_int32
-    ch = chan.tx_rx_process
+    ch = chan["tx_rx_process"]
     ch.argtypes = [u8, ctypes.c_size_t, ctypes.c_uint16, ctypes.c_float, i16]
     ch.restype = ctypes.c_int32
     return _BoundLibrary(lib, enc, dec, ch, chan)

@@ -224,7 +224,7 @@ def test_module_matches_reference_model(k):
     assert np.allclose(q.detach().cpu().numpy(), f["lq"], rtol=1e-5, atol=1e-4)
     for t in range(len(its)):
         assert np.allclose(its[t].detach().cpu().numpy(), f["its"][t], rtol=1e-5, atol=1e-4)
-    assert abs(float(loss) - float(f["loss"])) <= 1e-5 * max(abs(float(f["loss"])), 1.0)
+    assert abs(float(loss.detach()) - float(f["loss"])) <= 1e-5 * max(abs(float(f["loss"])), 1.0)
     assert _close(m.raw_alpha.grad.cpu().numpy(), f["g_raw_alpha"], 1e-4)
     assert _close(m.raw_beta.grad.cpu().numpy(), f["g_raw_beta"], 1e-4)
     assert _close(x.grad.cpu().numpy(), f["g_llr"], 1e-4)

@@ -120,3 +120,17 @@ def test_run_simulation_pickle_matches_reference_format(tmp_path, monkeypatch):
         assert np.all(bler[1] == 1.0)
         assert bler[0].shape == (2,)
     assert stub.calls[:2] == [1, 4]
+
+
+def test_load_c_library_leaves_shared_bindings_alone():
+    """load_c_library binds the harness's ndpointer argument types on function
+    objects of its own: the package's shared library handle keeps its
+    pointer-typed bindings for every other caller."""
+    from deeplearning_5g_ldpc_decoder_amd import _lib
+    L = _lib.load()
+    before = {n: list(getattr(L, n).argtypes) for n in
+              ("c_ldpc_5g_decode", "c_ldpc_5g_encode", "tx_rx_process")}
+    lib = S.load_c_library()
+    assert lib.ldpc_5g_decode.argtypes[0] is not ctypes.c_void_p
+    for n, types in before.items():
+        assert list(getattr(L, n).argtypes) == types, n
