@@ -457,6 +457,30 @@ void edge_table(int bgn, int zc, int ils, uint32_t *out) {
         out[e] = (uint32_t)(col[e] * zc) | ((uint32_t)(sh[e] % zc) << 16);
 }
 
+int edge_block_words(int bgn) {
+    const int rows = bgn == 1 ? NR_BG1_ROWS : NR_BG2_ROWS;
+    const uint16_t *start = bgn == 1 ? kBG1RowStart : kBG2RowStart;
+    int pairs = 0;
+    for (int r = 0; r < rows; ++r) pairs += (start[r + 1] - start[r] + 1) / 2;
+    return (bgn == 1 ? NR_BG1_EDGES : NR_BG2_EDGES) + 3 * pairs;
+}
+
+void edge_block(int bgn, int zc, int ils, uint32_t *out) {
+    edge_table(bgn, zc, ils, out);
+    const int ne = bgn == 1 ? NR_BG1_EDGES : NR_BG2_EDGES;
+    const int rows = bgn == 1 ? NR_BG1_ROWS : NR_BG2_ROWS;
+    const uint16_t *start = bgn == 1 ? kBG1RowStart : kBG2RowStart;
+    uint32_t *pt = out + ne;
+    for (int r = 0; r < rows; ++r)
+        for (int e = start[r]; e < start[r + 1]; e += 2) {
+            const uint32_t a = out[e];
+            const uint32_t b = e + 1 < start[r + 1] ? out[e + 1] : 0u;
+            *pt++ = 2u * (a >> 16) | (2u * (b >> 16)) << 16;
+            *pt++ = 2u * (a & 0xFFFFu);
+            *pt++ = 2u * (b & 0xFFFFu);
+        }
+}
+
 }  // namespace nrldpc
 
 using namespace nrldpc;

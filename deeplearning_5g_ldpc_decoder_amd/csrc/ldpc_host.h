@@ -27,6 +27,14 @@ const uint16_t *all_lifting_sizes(int *count);
 // Device edge table of (bgn, zc): per edge (column * zc) | (V_ij mod zc) << 16.
 void edge_table(int bgn, int zc, int ils, uint32_t *out);
 
+// Words of one (bgn, zc) table block: the edge table (NR_BG*_EDGES words),
+// then the pair table of the run-time packed rows, 3 words per row pair in
+// row order (edges e0 + 2i, e0 + 2i + 1 of a row; an odd last edge alone):
+// (2 * shift_a) | (2 * shift_b) << 16, 2 * column_a * zc, 2 * column_b * zc
+// (byte units of int16 beliefs; the b words 0 for a lone edge).
+int edge_block_words(int bgn);
+void edge_block(int bgn, int zc, int ils, uint32_t *out);
+
 // Device CRC tables (DecOut::crc): 4 * kCrcTabWords words.
 void crc_device_tables(uint32_t *out);
 

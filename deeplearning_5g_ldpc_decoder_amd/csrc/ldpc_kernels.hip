@@ -1042,10 +1042,11 @@ __device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
 // of any lifting size per workgroup) and the packed tiles, fast rows only.
 // The arithmetic is the specialized kernel's (packed_pair_in,
 // packed_finalize with the general scale, packed_pair_update); offsets come
-// from the edge table, etab[e] = column * Zc | shift << 16 (one scalar load
-// per edge).  lqb is the lane's codeblock beliefs in LDS, zx2 = 2 * zx,
-// zz = zx2 in both halves, z2 = 2 * Zc and wrap = -2 * Zc in both int16
-// halves.
+// from the pair table that follows the edge table of the (BG, Zc) block:
+// per row pair, (2 * shift_a) | (2 * shift_b) << 16, then the two columns'
+// byte offsets (three scalar loads, no scalar decoding).  lqb is the lane's
+// codeblock beliefs in LDS, zx2 = 2 * zx, zz = zx2 in both halves,
+// z2 = 2 * Zc and wrap = -2 * Zc in both int16 halves.
 // ---------------------------------------------------------------------------
 struct GenLane {
     char *lqb;
@@ -1060,23 +1061,22 @@ typedef const uint32_t *etab_cptr;
 typedef const __attribute__((address_space(4))) uint32_t *etab_cptr;
 #endif
 
-template <int E, int I, int DEG, int PAIRS>
-__device__ __forceinline__ void gpair_load(const GenLane &g, etab_cptr etab,
+// Pair I of a row from its three pair-table words pw[0..2].
+template <int I, int DEG, int PAIRS>
+__device__ __forceinline__ void gpair_load(const GenLane &g, const uint32_t *pw,
                                            PackedRowState<PAIRS> &rs) {
-    const uint32_t ta = etab[E];
-    const uint32_t sha = (ta >> 16) * 2u, cola = (ta & 0xFFFFu) * 2u;
+    const uint32_t sh = pw[0], cola = pw[1];
     if constexpr (2 * I + 1 < DEG) {
         // both offsets at once (see packed_pair_addr): halves below 4 * Zc
-        const uint32_t tb = etab[E + 1];
-        const uint32_t shb = (tb >> 16) * 2u, colb = (tb & 0xFFFFu) * 2u;
-        const uint32_t t = g.zz + (sha | (shb << 16));
+        const uint32_t colb = pw[2];
+        const uint32_t t = g.zz + sh;
         const uint32_t u = pk_addw(t, g.wrap);
         rs.oa[I] = cola + min(t & 0xFFFFu, u & 0xFFFFu);
         rs.ob[I] = colb + min(t >> 16, u >> 16);
         rs.qa[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.oa[I]);
         rs.qb[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.ob[I]);
     } else {
-        const uint32_t c = g.zx2 + sha;
+        const uint32_t c = g.zx2 + sh;   // a lone edge: the high half is 0
         rs.oa[I] = cola + min(c, c - g.z2);
         rs.qa[I] = (uint32_t)(uint16_t)*(const int16_t *)(g.lqb + rs.oa[I]);
     }
@@ -1089,10 +1089,29 @@ __device__ __forceinline__ void gpair_store(const GenLane &g, const PackedRowSta
     if constexpr (2 * I + 1 < DEG) *(int16_t *)(g.lqb + rs.ob[I]) = (int16_t)(v >> 16);
 }
 
-// kSimple as in packed_finalize (DecParams::simple).
+// Pair-table words of barrier group G (rows first[G] .. first[G+1] - 1),
+// held in scalar registers.  A group's words are loaded while the previous
+// group runs, so the rows after a barrier find their offsets' scalar
+// operands at hand instead of waiting on a scalar load.
+template <int BG, int G>
+struct GroupWords {
+    static constexpr int kP0 = packed_word0<BG>(kGroupPlan<BG>.first[G]);
+    static constexpr int kN = 3 * (packed_word0<BG>(kGroupPlan<BG>.first[G + 1]) - kP0);
+    uint32_t w[kN];
+};
+
+template <int BG, int G>
+__device__ __forceinline__ GroupWords<BG, G> load_group_words(etab_cptr ptab) {
+    GroupWords<BG, G> gw;
+#pragma unroll
+    for (int i = 0; i < GroupWords<BG, G>::kN; ++i) gw.w[i] = ptab[3 * GroupWords<BG, G>::kP0 + i];
+    return gw;
+}
+
+// kSimple as in packed_finalize (DecParams::simple).  pw: the row's pair words.
 template <int BG, int R, bool kSimple, int N, int... I>
 __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N],
-                                            etab_cptr etab, const DecParams &p, int alpha,
+                                            const uint32_t *pw, const DecParams &p, int alpha,
                                             int beta, std::integer_sequence<int, I...>) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
@@ -1100,7 +1119,7 @@ __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N]
     constexpr int pairs = (int)sizeof...(I);
     static_assert(pairs >= 2, "packed rows need degree >= 3");
     PackedRowState<pairs> rs;
-    (gpair_load<e0 + 2 * I, I, deg, pairs>(g, etab, rs), ...);
+    (gpair_load<I, deg, pairs>(g, pw + 3 * I, rs), ...);
     (packed_pair_in<I, I, deg, pairs>(msg, w0 + I, rs), ...);
     const PackedScale f = packed_finalize<kSimple>(rs, p, alpha, beta);
     (gpair_store<I, deg, pairs>(g, rs,
@@ -1108,10 +1127,12 @@ __device__ __forceinline__ void gpacked_row(const GenLane &g, uint32_t (&msg)[N]
      ...);
 }
 
-template <int BG, int R, bool kTable, bool kSimple, int N>
-__device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], etab_cptr etab,
-                                       bool run, const DecParams &p, const uint32_t *ab) {
-    if constexpr (R < Graph<BG>::kRows) {
+// Rows R .. REnd - 1 of group G (column-disjoint: no barrier between them).
+template <int BG, int G, int R, int REnd, bool kTable, bool kSimple, int N>
+__device__ __forceinline__ void gpgroup_rows(const GenLane &g, uint32_t (&msg)[N],
+                                             const GroupWords<BG, G> &gw, const DecParams &p,
+                                             const uint32_t *ab) {
+    if constexpr (R < REnd) {
         int alpha = p.alpha, beta = p.beta;
         if constexpr (kTable) {
             const uint32_t w = ab[R];
@@ -1119,11 +1140,30 @@ __device__ __forceinline__ void gprows(const GenLane &g, uint32_t (&msg)[N], eta
             beta = (int)(int16_t)(w >> 16);
         }
         constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
-        if (run)
-            gpacked_row<BG, R, kSimple>(g, msg, etab, p, alpha, beta,
-                                        std::make_integer_sequence<int, (deg + 1) / 2>{});
-        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
-        gprows<BG, R + 1, kTable, kSimple>(g, msg, etab, run, p, ab);
+        constexpr int off = 3 * (packed_word0<BG>(R) - GroupWords<BG, G>::kP0);
+        gpacked_row<BG, R, kSimple>(g, msg, gw.w + off, p, alpha, beta,
+                                    std::make_integer_sequence<int, (deg + 1) / 2>{});
+        gpgroup_rows<BG, G, R + 1, REnd, kTable, kSimple>(g, msg, gw, p, ab);
+    }
+}
+
+// Group G of one iteration, then (recursively) the groups after it; gw holds
+// group G's words.  Lanes with run = false only take part in the barriers.
+template <int BG, int G, bool kTable, bool kSimple, int N>
+__device__ __forceinline__ void gpgroups(const GenLane &g, uint32_t (&msg)[N], etab_cptr ptab,
+                                         bool run, const DecParams &p, const uint32_t *ab,
+                                         GroupWords<BG, G> &gw) {
+    constexpr auto &gp = kGroupPlan<BG>;
+    if constexpr (G + 1 < gp.groups) {
+        GroupWords<BG, G + 1> next = load_group_words<BG, G + 1>(ptab);
+        if (run) gpgroup_rows<BG, G, gp.first[G], gp.first[G + 1], kTable, kSimple>(g, msg, gw, p, ab);
+#pragma unroll
+        for (int i = 0; i < GroupWords<BG, G + 1>::kN; ++i) NRLDPC_OPAQUE_UNIFORM(next.w[i]);
+        __syncthreads();
+        gpgroups<BG, G + 1, kTable, kSimple>(g, msg, ptab, run, p, ab, next);
+    } else {
+        if (run) gpgroup_rows<BG, G, gp.first[G], gp.first[G + 1], kTable, kSimple>(g, msg, gw, p, ab);
+        __syncthreads();
     }
 }
 
@@ -1148,7 +1188,9 @@ __device__ __forceinline__ void gprows_iteration(GenLane g, uint32_t (&msg)[N], 
     NRLDPC_OPAQUE_LANE(g.zz);
     etab_cptr ce = (etab_cptr)etab;
     NRLDPC_OPAQUE_UNIFORM(ce);
-    gprows<BG, 0, kTable, kSimple>(g, msg, ce, run, p, ab);
+    const etab_cptr ptab = ce + Graph<BG>::kEdges;
+    GroupWords<BG, 0> first = load_group_words<BG, 0>(ptab);
+    gpgroups<BG, 0, kTable, kSimple>(g, msg, ptab, run, p, ab, first);
 }
 
 template <int BG>

@@ -78,12 +78,12 @@ struct Runtime {
         const uint16_t *zs = all_lifting_sizes(&nz);
         std::vector<uint32_t> tab;
         for (int bg = 1; bg <= 2; ++bg) {
-            const int ne = bg == 1 ? BG1_MAX_NON_NULL_ELEMENTS : BG2_MAX_NON_NULL_ELEMENTS;
+            const int ne = edge_block_words(bg);
             for (int i = 0; i < nz; ++i) {
                 const int zc = zs[i];
                 etab_off[bg - 1][zc] = (uint32_t)tab.size();
                 tab.resize(tab.size() + ne);
-                edge_table(bg, zc, lifting_set(zc), tab.data() + etab_off[bg - 1][zc]);
+                edge_block(bg, zc, lifting_set(zc), tab.data() + etab_off[bg - 1][zc]);
             }
         }
         crc_off = (uint32_t)tab.size();

@@ -19,11 +19,11 @@ struct EdgeTables {
         int nz = 0;
         const uint16_t *zs = all_lifting_sizes(&nz);
         for (int bg = 1; bg <= 2; ++bg) {
-            const int ne = bg == 1 ? NR_BG1_EDGES : NR_BG2_EDGES;
+            const int ne = edge_block_words(bg);
             for (int i = 0; i < nz; ++i) {
                 off[bg - 1][zs[i]] = (uint32_t)tab.size();
                 tab.resize(tab.size() + ne);
-                edge_table(bg, zs[i], lifting_set(zs[i]), tab.data() + off[bg - 1][zs[i]]);
+                edge_block(bg, zs[i], lifting_set(zs[i]), tab.data() + off[bg - 1][zs[i]]);
             }
         }
     }
