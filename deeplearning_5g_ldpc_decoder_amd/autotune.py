@@ -1,13 +1,17 @@
 """Autotuner for the packed-tile mapping of algo-1 batches.
 
 For each (base graph, lifting size) it times decode launches with k
-codeblocks per 384-lane workgroup, k in {1, 2, 4, ..., as many as fit}, on a
-batch large enough to keep every CU busy (tiles >= 2 per CU), and keeps the k
-with the highest codeblock throughput.  The table goes to tile_table.json
-beside this file; BatchPlan applies it through
-ldpc_mi355_set_tile_codeblocks before building plans.
+codeblocks per group, k in {1, 2, 4, ..., as many as fit}, on a batch large
+enough to keep every CU busy (tiles >= 2 per CU), and keeps the k with the
+highest codeblock throughput.  Then it times the tile schedule on a
+mixed-size batch: one workgroup per tile against a persistent grid that pulls
+the tiles from a device queue, and keeps the faster.  The table goes to
+tile_table.json beside this file; BatchPlan applies it (k through
+ldpc_mi355_set_tile_codeblocks, the schedule as its default) before building
+plans.
 
     python -m deeplearning_5g_ldpc_decoder_amd.autotune [--zc 2 3 ...] [--bg 1 2]
+    python -m deeplearning_5g_ldpc_decoder_amd.autotune --schedule-only
 """
 import argparse
 import json
@@ -61,7 +65,8 @@ def tune_class(torch, bgn, zc, reps=4, distinct=16, cus=256):
         L.ldpc_mi355_set_tile_codeblocks(bgn, zc, k)
         n = max(2 * cus * k, 512)
         offs = [(i % distinct) * E for i in range(n)]
-        plan = BatchPlan([cfg] * n, ROW, llr_offsets=offs, flags=BatchPlan.PACK)
+        plan = BatchPlan([cfg] * n, ROW, llr_offsets=offs, flags=BatchPlan.PACK,
+                         schedule="oneshot")
         dt = _time_plan(torch, plan, d_llr, reps)
         plan.close()
         res[k] = n / dt
@@ -71,14 +76,47 @@ def tune_class(torch, bgn, zc, reps=4, distinct=16, cus=256):
                 codeblocks_per_s={str(k): float(v) for k, v in res.items()})
 
 
+def tune_schedule(torch, n=4096, reps=20):
+    """One-shot (a workgroup per tile) against persistent (a resident grid
+    pulling tiles from a device queue) on a mixed-size batch of n codeblocks,
+    with the k table already applied; outputs must agree."""
+    from .batch import BatchPlan
+    from .workloads import mixed_batch
+    cfgs, llrs, _ = mixed_batch(n, seed=5)
+    d_llr = torch.from_numpy(np.concatenate(llrs)).cuda()
+    res, outs = {}, {}
+    for name in ("oneshot", "persistent"):
+        plan = BatchPlan(cfgs, ROW, flags=BatchPlan.PACK, schedule=name)
+        res[name] = _time_plan(torch, plan, d_llr, reps)
+        outs[name] = {k: v.cpu().numpy() for k, v in plan.decode(d_llr).items()}
+        plan.close()
+    same = all(np.array_equal(outs["oneshot"][k], outs["persistent"][k]) for k in outs["oneshot"])
+    if not same:
+        raise RuntimeError("persistent and one-shot tile launches disagree")
+    return dict(persistent=bool(res["persistent"] < res["oneshot"]), codeblocks=n,
+                oneshot_ms=res["oneshot"] * 1e3, persistent_ms=res["persistent"] * 1e3)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--bg", type=int, nargs="*", default=[1, 2])
     ap.add_argument("--zc", type=int, nargs="*", default=list(LIFTING_SIZES))
     ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--schedule-only", action="store_true",
+                    help="keep the k table in --out and re-time only the tile schedule")
     args = ap.parse_args(argv)
     import torch
     assert torch.cuda.is_available(), "autotuning needs the GPU"
+    if args.schedule_only:
+        from .batch import apply_tuning
+        with open(args.out) as f:
+            table = json.load(f)
+        apply_tuning(args.out)
+        table["schedule"] = tune_schedule(torch)
+        print("autotune: schedule", table["schedule"], flush=True)
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1)
+        return 0
     table = {"row": ROW, "device": torch.cuda.get_device_name(0), "classes": {}}
     t0 = time.time()
     for bgn in args.bg:
@@ -90,6 +128,12 @@ def main(argv=None):
             print(f"autotune: BG{bgn} Zc={zc}: k={r['k']} "
                   f"({r['codeblocks_per_s'][str(r['k'])]:.3g} cb/s)", flush=True)
     table["seconds"] = time.time() - t0
+    from .batch import apply_tuning
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1)
+    apply_tuning(args.out)
+    table["schedule"] = tune_schedule(torch)
+    print("autotune: schedule", table["schedule"], flush=True)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
     print("autotune: wrote", args.out, flush=True)

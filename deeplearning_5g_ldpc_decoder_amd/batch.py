@@ -23,13 +23,17 @@ def _decoder(row):
 
 
 _tuning_applied = False
+# tile schedule of plans that do not choose one: persistent (a resident grid
+# pulling tiles from a device queue) when the autotuner measured it faster
+_persistent_default = False
 
 
 def apply_tuning(path=None):
     """Load the autotuned codeblocks-per-workgroup table (tile_table.json)
-    into the library; plans built afterwards use it.  Missing table: the
-    library's default (as many codeblocks as fit in 384 lanes)."""
-    global _tuning_applied
+    into the library and its tile schedule as the plans' default; plans
+    built afterwards use them.  Missing table: the library's default (as many
+    codeblocks as fit in 384 lanes, one workgroup per tile)."""
+    global _tuning_applied, _persistent_default
     import json
     import os
     if path is None:
@@ -39,6 +43,7 @@ def apply_tuning(path=None):
         return 0
     with open(path) as f:
         table = json.load(f)
+    _persistent_default = bool(table.get("schedule", {}).get("persistent", False))
     L = _lib.load()
     n = 0
     for key, v in table.get("classes", {}).items():
@@ -55,17 +60,27 @@ class BatchPlan:
     llr_offsets: element offset of each codeblock's E LLRs in the LLR buffer
     (default: packed back to back).  soft_slots: HARQ soft-buffer slot per
     codeblock or -1 (default: none).  want_lq: also return posteriors.
+    schedule: "oneshot" (one workgroup per packed tile), "persistent" (a
+    resident grid pulling the tiles from a device queue) or None (the
+    autotuned default from tile_table.json).
     """
 
     PACK = 2        # plan flag: pack codeblocks of one (BG, Zc) per workgroup
     NO_PACK = 4     # plan flag: one codeblock per workgroup
     GENERIC = 8     # plan flag: no kernel specialized for a single (BG, Zc)
+    PERSISTENT = 16  # plan flag: packed tiles pulled from a device queue by a resident grid
 
     def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False,
-                 flags=0):
+                 flags=0, schedule=None):
         L = _lib.load()
         if not _tuning_applied:
             apply_tuning()
+        if schedule is None:
+            schedule = "persistent" if _persistent_default else "oneshot"
+        if schedule not in ("oneshot", "persistent"):
+            raise ValueError(f"unknown tile schedule {schedule!r}")
+        if schedule == "persistent":
+            flags |= self.PERSISTENT
         n = len(cfgs)
         if n == 0:
             raise ValueError("empty batch")

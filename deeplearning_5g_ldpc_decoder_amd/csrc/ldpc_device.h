@@ -80,6 +80,14 @@ struct TileDesc {
 };
 static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 
+// Device queue of a persistent tile launch: ctr[0] is the next tile index,
+// ctr[1] counts workgroups that have left; both 0 between launches.  A null
+// ctr launches one workgroup per tile instead.
+struct TileQueue {
+    uint32_t *ctr;
+    int32_t ntiles;
+};
+
 // Per-codeblock result flags (status) follow c_ldpc_5g_decode: 0 CRC pass,
 // 1 not converged.
 struct DecOut {

@@ -32,6 +32,7 @@
 #define NRLDPC_DYNAMIC_LDS(name) extern __shared__ __align__(16) unsigned char name[]
 #endif
 
+#include <algorithm>
 #include <utility>
 
 #include "ldpc_device.h"
@@ -1501,19 +1502,19 @@ __device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, 
     }
 }
 
-// One workgroup of kTileThreads lanes per tile; each wave reads the
-// descriptor of its own group (TileDesc).  Every group of a workgroup has the
-// same base graph, so all waves run the same rows and barriers.
-template <bool kGeneral, bool kTable, bool kSimple = false>
-__global__ void __launch_bounds__(kTileThreads)
-ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p) {
-    NRLDPC_DYNAMIC_LDS(smem);
-    __shared__ uint32_t crc_tab[4][256];
-    __shared__ uint32_t bank[2][kMaxTileCodeblocks];
-
+// One tile (workgroup descriptor `tile`) on the kTileThreads lanes of this
+// workgroup; each wave reads the descriptor of its own group (TileDesc).
+// Every group of a tile has the same base graph, so all waves run the same
+// rows and barriers.  crc_tab holds the four CRC tables already.
+template <bool kGeneral, bool kTable, bool kSimple>
+__device__ __forceinline__ void tile_body(const DecOut &o, const TileDesc *tiles,
+                                          const uint32_t *order, const DecParams &p,
+                                          uint32_t tile, unsigned char *smem,
+                                          uint32_t (*crc_tab)[256],
+                                          uint32_t (*bank)[kMaxTileCodeblocks]) {
     const int lane = threadIdx.x, nt = blockDim.x;
     const int wave = NRLDPC_WAVE_UNIFORM(lane >> 6);
-    const TileDesc td = tiles[blockIdx.x * kTileWaves + wave];
+    const TileDesc td = tiles[(size_t)tile * kTileWaves + wave];
     const int Zc = td.Zc, local = lane - td.lane_base;
     const int cbl = local / Zc;
     const bool member = cbl < td.count;
@@ -1524,8 +1525,6 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     const int n = d.N + 2 * Zc;
     int16_t *lq = reinterpret_cast<int16_t *>(smem) + td.lds_base + (member ? cbl : 0) * td.lds_stride;
 
-    // CRC tables of all four polynomials (codeblocks may differ in crctype)
-    for (int i = lane; i < 4 * 256; i += nt) crc_tab[i >> 8][i & 255] = o.crc[(i >> 8) * kCrcTabWords + (i & 255)];
     for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
         bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
     CrcCtx crc;
@@ -1564,6 +1563,47 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
     if (d.lq_off != kNone) {
         int16_t *out = reinterpret_cast<int16_t *>(o.lq) + d.lq_off;
         for (int i = zx; i < n; i += Zc) out[i] = lq[i];
+    }
+}
+
+// Packed-tile kernel.  With q.ctr null, workgroup b decodes tile b (one
+// workgroup per tile).  Otherwise the launch is persistent: a grid of at most
+// one workgroup per resident slot pulls tile indices from the device queue
+// counter q.ctr[0] until they run past q.ntiles, and the last workgroup to
+// leave (counted in q.ctr[1]) sets both counters back to 0 for the next
+// launch that uses them.
+template <bool kGeneral, bool kTable, bool kSimple = false>
+__global__ void __launch_bounds__(kTileThreads)
+ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p,
+                 TileQueue q) {
+    NRLDPC_DYNAMIC_LDS(smem);
+    __shared__ uint32_t crc_tab[4][256];
+    __shared__ uint32_t bank[2][kMaxTileCodeblocks];
+    __shared__ uint32_t next_tile;
+
+    const int lane = threadIdx.x, nt = blockDim.x;
+    // CRC tables of all four polynomials (codeblocks may differ in crctype);
+    // the recovery's barriers publish them
+    for (int i = lane; i < 4 * 256; i += nt) crc_tab[i >> 8][i & 255] = o.crc[(i >> 8) * kCrcTabWords + (i & 255)];
+    if (!q.ctr) {
+        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, blockIdx.x, smem, crc_tab, bank);
+        return;
+    }
+    for (;;) {
+        if (lane == 0) next_tile = atomicAdd(q.ctr, 1u);
+        __syncthreads();
+        const uint32_t tile = next_tile;
+        if (tile >= (uint32_t)q.ntiles) break;   // uniform over the workgroup
+        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, tile, smem, crc_tab, bank);
+        // every lane is done with this tile's LDS (and with next_tile)
+        __syncthreads();
+    }
+    if (lane == 0) {
+        __threadfence();   // this workgroup's last draw precedes its exit count
+        if (atomicAdd(q.ctr + 1, 1u) == gridDim.x - 1) {
+            atomicExch(q.ctr, 0u);
+            atomicExch(q.ctr + 1, 0u);
+        }
     }
 }
 
@@ -1728,24 +1768,37 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     return e;
 }
 
-// Packed-tile launch (algos 1-3): ntiles workgroups of kTileThreads lanes.
+// Packed-tile launch (algos 1-3): ntiles workgroups of kTileThreads lanes,
+// or, with queue non-null (two zeroed words), a persistent grid of at most
+// one workgroup per resident slot that pulls the ntiles tiles from it.
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
                                           const nrldpc::TileDesc *tiles, const uint32_t *order,
                                           const nrldpc::DecParams *p, int ntiles,
-                                          int lds_entries, hipStream_t stream) {
+                                          int lds_entries, uint32_t *queue, hipStream_t stream) {
     using namespace nrldpc;
     if (algo < 1 || algo > 3) return hipErrorInvalidValue;
     if (algo != 1 && (p->scale_iters < 1 || !p->scale[0] || !p->scale[1]))
         return hipErrorInvalidValue;
     const size_t lds = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
     if ((size_t)lds_entries * sizeof(int16_t) > 90 * 1024) return hipErrorInvalidValue;
-    const dim3 grid(ntiles), block(kTileThreads);
+    const void *fn = tile_kernel(p->general != 0, algo != 1, p->simple != 0);
+    int grid = ntiles;
+    if (queue) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTileThreads, lds);
+        if (e != hipSuccess) return e;
+        grid = std::min(ntiles, std::max(1, cus * per_cu));
+    }
     DecOut ov = *o;
     DecParams pv = *p;
-    void *args[] = {&ov, &tiles, &order, &pv};
-    const hipError_t e =
-        hipLaunchKernel(tile_kernel(pv.general != 0, algo != 1, pv.simple != 0), grid, block, args,
-                        lds, stream);
+    TileQueue q;
+    q.ctr = queue;
+    q.ntiles = ntiles;
+    void *args[] = {&ov, &tiles, &order, &pv, &q};
+    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(kTileThreads), args, lds, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

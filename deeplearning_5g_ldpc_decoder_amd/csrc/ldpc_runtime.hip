@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -21,7 +22,7 @@ extern "C" hipError_t nrldpc_configure_kernels();
 extern "C" hipError_t nrldpc_launch_synth(int bgn, const SynthArgs *a, hipStream_t s);
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const DecOut *o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams *p, int ntiles,
-                                          int lds_entries, hipStream_t stream);
+                                          int lds_entries, uint32_t *queue, hipStream_t stream);
 
 namespace {
 
@@ -298,6 +299,12 @@ struct ldpc_mi355_plan {
     int ntiles = 0, tile_lds = 0;
     TileDesc *d_tiles = nullptr;
     uint32_t *d_order = nullptr;
+    // persistent tile launches (flags bit 4): kQueueRing queues of two words,
+    // zeroed at creation and rearmed by each launch's last workgroup; launch
+    // k uses queue k mod kQueueRing
+    static constexpr int kQueueRing = 64;
+    uint32_t *d_queue = nullptr;
+    mutable std::atomic<uint32_t> launches{0};
 };
 
 namespace {
@@ -455,7 +462,13 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
                       hipMemcpyHostToDevice) != hipSuccess ||
             hipMalloc(&pl->d_order, sizeof(uint32_t) * order.size()) != hipSuccess ||
             hipMemcpy(pl->d_order, order.data(), sizeof(uint32_t) * order.size(),
-                      hipMemcpyHostToDevice) != hipSuccess) {
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            ((flags & 16u) &&
+             (hipMalloc(&pl->d_queue, 2 * sizeof(uint32_t) * ldpc_mi355_plan::kQueueRing) !=
+                  hipSuccess ||
+              hipMemset(pl->d_queue, 0, 2 * sizeof(uint32_t) * ldpc_mi355_plan::kQueueRing) !=
+                  hipSuccess ||
+              hipDeviceSynchronize() != hipSuccess))) {
             fprintf(stderr, "libnrldpc_mi355: plan allocation failed\n");
             ldpc_mi355_plan_destroy(pl);
             return -1;
@@ -502,9 +515,13 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     // NULL is HIP's default (null) stream, as everywhere in HIP -- it is also
     // the handle PyTorch reports for its default stream.
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint32_t *queue = nullptr;
+    if (plan->d_queue)
+        queue = plan->d_queue +
+                2 * (plan->launches.fetch_add(1u) % (uint32_t)ldpc_mi355_plan::kQueueRing);
     if (plan->ntiles > 0)
         HIP_OK(nrldpc_launch_tiles(plan->algo, &o, plan->d_tiles, plan->d_order, &plan->dp,
-                                   plan->ntiles, plan->tile_lds, s));
+                                   plan->ntiles, plan->tile_lds, queue, s));
     else
         HIP_OK(nrldpc_launch_decode(plan->algo, &o, &plan->dp, plan->n, plan->threads,
                                     plan->max_soft, plan->uniform, s));
@@ -517,6 +534,7 @@ extern "C" void ldpc_mi355_plan_destroy(ldpc_mi355_plan_t *plan) {
     if (plan->d_work) (void)hipFree(plan->d_work);
     if (plan->d_tiles) (void)hipFree(plan->d_tiles);
     if (plan->d_order) (void)hipFree(plan->d_order);
+    if (plan->d_queue) (void)hipFree(plan->d_queue);
     if (plan->d_scale) (void)hipFree(plan->d_scale);
     delete plan;
 }
@@ -651,7 +669,7 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
         HIP_OK(nrldpc_launch_tiles(b->algo, &o,
                                    reinterpret_cast<const TileDesc *>(b->d_stage + toff),
                                    reinterpret_cast<const uint32_t *>(b->d_stage + ooff),
-                                   &b->dp, b->ntiles, tile_lds, s));
+                                   &b->dp, b->ntiles, tile_lds, nullptr, s));
     else
         HIP_OK(nrldpc_launch_decode(b->algo, &o, &b->dp, n, threads_for(L.max_zc), L.max_soft,
                                     (b->flags & 8u) ? 0u : L.uniform, s));

@@ -241,7 +241,11 @@ typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
  * Without bits 1/2, int16 plans pack whenever some lifting size leaves room
  * for two or more codeblocks per workgroup.  bit 3 = never use a kernel
  * specialized for the batch's single (base graph, lifting size) (int16
- * algos; used when all codeblocks share one, e.g. BG1 Zc=384).  Returns 0, -2 if any codeblock
+ * algos; used when all codeblocks share one, e.g. BG1 Zc=384).  bit 4 =
+ * persistent packed tiles: a grid of at most one workgroup per resident slot
+ * pulls the tiles from a device queue (the plan holds 64 queues, used in
+ * turn by successive launches, so at most 64 launches of one plan may be in
+ * flight at once).  Returns 0, -2 if any codeblock
  * config is invalid or, for algo 2, needs more iterations than its base
  * graph's trained table holds (index in *bad_index when non-NULL), -3 for
  * an unsupported algo, -1 on a GPU error. */

@@ -41,6 +41,8 @@ def lib():
         L.emu_force_general.restype = None
         L.emu_use_fixed.argtypes = [I]
         L.emu_use_fixed.restype = None
+        L.emu_set_persistent_tiles.argtypes = [I]
+        L.emu_set_persistent_tiles.restype = None
         _lib = L
     return _lib
 
@@ -176,6 +178,12 @@ def row_general(row):
 def force_general(g):
     """Force the int16 row form: True exact, False fast, None the library's."""
     lib().emu_force_general(-1 if g is None else int(bool(g)))
+
+
+def persistent_tiles(grid):
+    """Tile launches pull their tiles from a device queue with at most
+    `grid` workgroups (0: one workgroup per tile)."""
+    lib().emu_set_persistent_tiles(int(grid))
 
 
 def use_fixed(on):

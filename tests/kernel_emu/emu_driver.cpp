@@ -223,6 +223,10 @@ extern "C" int emu_plan_tiles(int n, const int32_t *cfgs, int k, TileDesc *tiles
     return (int)(tiles.size() / kTileWaves);
 }
 
+// > 0: tile launches are persistent with this many workgroups at most.
+int g_persistent_grid = 0;
+extern "C" void emu_set_persistent_tiles(int grid) { g_persistent_grid = grid; }
+
 // Same batch through the packed-tile kernel (algos 1-3) with k codeblocks
 // per group at most (0 = as many as fit).  Returns the number of tile
 // workgroups, or < 0.
@@ -249,18 +253,26 @@ extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_
     const unsigned nt = (unsigned)(tiles.size() / kTileWaves);
     const size_t lds = (size_t)lds_entries * 2;
     const bool simple = p.simple != 0;
+    // persistent launch: g_persistent_grid workgroups pull the tiles from a queue
+    uint32_t ctr[2] = {0u, 0u};
+    TileQueue q;
+    q.ctr = g_persistent_grid > 0 ? ctr : nullptr;
+    q.ntiles = (int)nt;
+    const unsigned grid = g_persistent_grid > 0 ? std::min(nt, (unsigned)g_persistent_grid) : nt;
     if (!g && !t && simple)
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false, true>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false, true>(o, td, ord, p, q); });
     else if (!g && t && simple)
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true, true>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true, true>(o, td, ord, p, q); });
     else if (!g && !t)
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false>(o, td, ord, p, q); });
     else if (g && !t)
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<true, false>(o, td, ord, p, q); });
     else if (!g && t)
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<false, true>(o, td, ord, p, q); });
     else
-        emu_launch(nt, kTileThreads, lds, [&] { ldpc_tile_kernel<true, true>(o, td, ord, p); });
+        emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<true, true>(o, td, ord, p, q); });
+    // the last workgroup out rearms the queue
+    if (q.ctr && (ctr[0] != 0u || ctr[1] != 0u)) return -5;
     return (int)nt;
 }
 

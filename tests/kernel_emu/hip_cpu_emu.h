@@ -35,7 +35,7 @@ struct float4 {
 };
 
 inline thread_local EmuDim3 threadIdx;
-inline EmuDim3 blockIdx, blockDim;
+inline EmuDim3 blockIdx, blockDim, gridDim;
 
 struct EmuBlock {
     std::unique_ptr<std::barrier<>> bar;
@@ -71,6 +71,13 @@ inline int __syncthreads_or(int pred) {
 inline uint32_t atomicXor(uint32_t *addr, uint32_t v) {
     return __atomic_fetch_xor(addr, v, __ATOMIC_SEQ_CST);
 }
+inline uint32_t atomicAdd(uint32_t *addr, uint32_t v) {
+    return __atomic_fetch_add(addr, v, __ATOMIC_SEQ_CST);
+}
+inline uint32_t atomicExch(uint32_t *addr, uint32_t v) {
+    return __atomic_exchange_n(addr, v, __ATOMIC_SEQ_CST);
+}
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 
 inline uint32_t __shfl_xor(uint32_t v, int mask, int width) {
     (void)width;
@@ -96,6 +103,8 @@ inline void emu_launch(unsigned grid, unsigned block, size_t lds_bytes,
                        const std::function<void()> &fn) {
     blockDim.x = block;
     blockDim.y = blockDim.z = 1;
+    gridDim.x = grid;
+    gridDim.y = gridDim.z = 1;
     for (unsigned b = 0; b < grid; ++b) {
         blockIdx.x = b;
         g_emu_block.bar = std::make_unique<std::barrier<>>((std::ptrdiff_t)block);

@@ -33,10 +33,11 @@ def _check(oracle_mod, plan, out, cfgs, llrs, row):
             assert np.array_equal(lq[i], q), (i, cfg)
 
 
-@pytest.mark.parametrize("flags_k", [(2, 0), (2, 4), (4, 0)])
+@pytest.mark.parametrize("flags_k", [(2, 0), (2, 4), (4, 0), (18, 0)])
 def test_all_lifting_sizes_packed_and_plain(torch_dev, oracle_mod, flags_k):
     """Every reachable (BG, Zc), rates 0.25..0.85, Qm 2..8, rv 0..3 in one
-    plan: packed tiles (default and k = 4 per tile) and one codeblock per
+    plan: packed tiles (default and k = 4 per tile, one workgroup per tile or
+    a persistent grid pulling tiles from a device queue) and one codeblock per
     workgroup, all bit-exact with the oracle."""
     from deeplearning_5g_ldpc_decoder_amd import _lib
     from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
@@ -49,8 +50,9 @@ def test_all_lifting_sizes_packed_and_plain(torch_dev, oracle_mod, flags_k):
         for bgn in (1, 2):
             L.ldpc_mi355_set_tile_codeblocks(bgn, zc, k)
     try:
-        plan = BatchPlan(cfgs, ROW1, want_lq=True, flags=flags)
-        assert (plan.tiles > 0) == (flags == 2)
+        plan = BatchPlan(cfgs, ROW1, want_lq=True, flags=flags,
+                         schedule="persistent" if flags & 16 else "oneshot")
+        assert (plan.tiles > 0) == bool(flags & 2)
         out = plan.decode(torch_dev.from_numpy(np.concatenate(llrs)).cuda())
         _check(oracle_mod, plan, out, cfgs, llrs, ROW1)
         plan.close()
@@ -78,15 +80,19 @@ def test_packed_general_rows(torch_dev, oracle_mod):
         plan.close()
 
 
-def test_hip_graph_capture_replay(torch_dev, oracle_mod):
+@pytest.mark.parametrize("flags", [0, 18])
+def test_hip_graph_capture_replay(torch_dev, oracle_mod, flags):
     """A packed plan captured in a HIP graph: every replay, with eager work
-    and synchronizations in between, equals the eager launch."""
+    and synchronizations in between, equals the eager launch.  flags 18: the
+    persistent tile launch, whose device queue the last workgroup of each
+    replay rearms."""
     torch = torch_dev
     from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
     rng = np.random.default_rng(12)
     cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=ALL_ZC[::2])
     d_llr = torch.from_numpy(np.concatenate(llrs)).cuda()
-    plan = BatchPlan(cfgs, ROW1, want_lq=True)
+    plan = BatchPlan(cfgs, ROW1, want_lq=True, flags=flags,
+                     schedule="persistent" if flags & 16 else "oneshot")
     eager = plan.decode(d_llr)
     ref = {k: v.cpu().numpy() for k, v in eager.items()}
     out = plan.alloc_outputs("cuda")
@@ -139,12 +145,13 @@ def test_copy_spans_round_trip(torch_dev):
     assert not b[sum(sizes):].any()
 
 
-@pytest.mark.parametrize("uniform", [True, False])
-def test_overlapping_launches_of_one_plan(torch_dev, oracle_mod, uniform):
+@pytest.mark.parametrize("uniform,flags", [(True, 0), (False, 0), (False, 18)])
+def test_overlapping_launches_of_one_plan(torch_dev, oracle_mod, uniform, flags):
     """Launches of one plan in flight on two streams, each with its own
     outputs (as the throughput bench runs them): every output set equals the
     single-stream decode.  uniform: BG1 Zc=384 codeblocks only (the
-    specialized kernel); otherwise a mixed batch (packed tiles)."""
+    specialized kernel); otherwise a mixed batch (packed tiles; flags 18:
+    persistent tile launches, each on its own device queue)."""
     torch = torch_dev
     from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
     rng = np.random.default_rng(21)
@@ -154,7 +161,8 @@ def test_overlapping_launches_of_one_plan(torch_dev, oracle_mod, uniform):
     else:
         cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=1, zcs=ALL_ZC[1::3])
     d_llr = torch.from_numpy(np.concatenate(llrs)).cuda()
-    plan = BatchPlan(cfgs, ROW1, want_lq=True)
+    plan = BatchPlan(cfgs, ROW1, want_lq=True, flags=flags,
+                     schedule="persistent" if flags & 16 else "oneshot")
     ref = {k: v.cpu().numpy() for k, v in plan.decode(d_llr).items()}
     streams = [torch.cuda.Stream() for _ in range(2)]
     outs = [plan.alloc_outputs("cuda") for _ in range(2)]

@@ -263,6 +263,29 @@ def test_emu_packed_tiles_lifting_sizes(oracle_mod, tile_k):
         assert np.array_equal(out["lq"], q), cfg
 
 
+@pytest.mark.parametrize("grid", [1, 3])
+def test_emu_persistent_tiles_match_oracle(oracle_mod, grid):
+    """Persistent tile launch: `grid` workgroups pull every tile from the
+    device queue (each workgroup decodes several tiles in turn, reusing its
+    LDS and CRC banks), bit-exact with the oracle per codeblock; the last
+    workgroup out leaves the queue counters at zero (checked by the driver,
+    which fails the launch otherwise)."""
+    rng = np.random.default_rng(77 + grid)
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=2, zcs=SOME_ZC)
+    row = [1, 8, 22936, 0, 1280, 32767, -32767]
+    emu.persistent_tiles(grid)
+    try:
+        outs = emu.decode_batch(cfgs, row, llrs, tile_k=0)
+    finally:
+        emu.persistent_tiles(0)
+    for cfg, llr, out in zip(cfgs, llrs, outs):
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+        assert (out["status"], out["iters"]) == (s, k), cfg
+        assert np.array_equal(out["hard"], h), cfg
+        assert np.array_equal(out["lq"], q), cfg
+
+
 def test_emu_packed_tiles_general_row(oracle_mod):
     """Packed tiles with a decoder row that needs the exact general path
     (alpha negative as int16, min_llr > max_llr)."""
