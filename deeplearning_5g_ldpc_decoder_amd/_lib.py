@@ -23,7 +23,7 @@ EXPORTED_FUNCTIONS = (
     "ldpc_mi355_sync", "ldpc_mi355_soft_slot_elems", "ldpc_mi355_version",
     "ldpc_mi355_algo_available", "ldpc_mi355_plan_tiles", "ldpc_mi355_set_tile_codeblocks",
     "ldpc_mi355_synth_batch", "ldpc_mi355_synth_float_batch", "ldpc_mi355_synth_device",
-    "ldpc_mi355_unrolled_create", "ldpc_mi355_unrolled_destroy", "ldpc_mi355_unrolled_dims",
+    "ldpc_mi355_synth_device_noise", "ldpc_mi355_unrolled_create", "ldpc_mi355_unrolled_destroy", "ldpc_mi355_unrolled_dims",
     "ldpc_mi355_unrolled_set_codeblocks_per_workgroup", "ldpc_mi355_unrolled_record_bytes",
     "ldpc_mi355_unrolled_scratch_bytes", "ldpc_mi355_unrolled_forward",
     "ldpc_mi355_unrolled_backward",
@@ -203,6 +203,9 @@ def load():
     L.ldpc_mi355_synth_device.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32, P,
                                           ctypes.c_uint64] + [P] * 8
     L.ldpc_mi355_synth_device.restype = ctypes.c_int32
+    L.ldpc_mi355_synth_device_noise.argtypes = [ctypes.POINTER(LdpcInConfig), ctypes.c_int32, P,
+                                                P, ctypes.c_uint64] + [P] * 8
+    L.ldpc_mi355_synth_device_noise.restype = ctypes.c_int32
     L.ldpc_mi355_unrolled_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(LdpcInConfig),
                                              ctypes.c_int32, ctypes.c_float, ctypes.c_int32]
     L.ldpc_mi355_unrolled_create.restype = ctypes.c_int32

@@ -219,11 +219,14 @@ def synth_batch(cfg, n, noise_power_db, seed=0, threads=8, want_payload=True):
     return llr, pay
 
 
-def synth_device(cfg, n, noise_power_db, seed=0, payload=None, outputs=("llr",), stream=None):
+def synth_device(cfg, n, noise_power_db, seed=0, payload=None, outputs=("llr",), stream=None,
+                 noise=None):
     """Device synthesis of n codeblocks of config cfg (ldpc_mi355_synth_device).
     noise_power_db: scalar or n values.  outputs: any of "bits", "cw", "rm",
-    "llr" (int16 E per codeblock), "llr_f32", "beliefs".  Returns a dict of
-    torch device tensors shaped [n, ...]."""
+    "llr" (int16 E per codeblock), "llr_f32", "beliefs".  noise: optional
+    (noise_var[n], samples[n, E/Qm, 2]) given channel noise
+    (ldpc_mi355_synth_device_noise; noise_power_db is then ignored).
+    Returns a dict of torch device tensors shaped [n, ...]."""
     import torch
     L = _lib.load()
     ic = LdpcInConfig(*[int(v) for v in cfg])
@@ -247,11 +250,25 @@ def synth_device(cfg, n, noise_power_db, seed=0, payload=None, outputs=("llr",),
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     g = lambda k: ptr(out[k]) if k in out else None  # noqa: E731
-    st = L.ldpc_mi355_synth_device(ctypes.byref(ic), n, ptr(nz), ctypes.c_uint64(seed),
-                                   ptr(pay) if pay is not None else None, g("bits"), g("cw"),
-                                   g("rm"), g("llr"), g("llr_f32"), g("beliefs"),
-                                   ctypes.c_void_p(stream))
+    if noise is None:
+        st = L.ldpc_mi355_synth_device(ctypes.byref(ic), n, ptr(nz), ctypes.c_uint64(seed),
+                                       ptr(pay) if pay is not None else None, g("bits"), g("cw"),
+                                       g("rm"), g("llr"), g("llr_f32"), g("beliefs"),
+                                       ctypes.c_void_p(stream))
+        keep = (nz, pay)
+    else:
+        nv = torch.as_tensor(np.ascontiguousarray(noise[0], np.float32).reshape(n), device=dev)
+        smp = np.ascontiguousarray(noise[1], np.float32)
+        if smp.size != n * (ic.E // ic.Qm) * 2:
+            raise ValueError("noise samples must be n * (E / Qm) * 2 floats")
+        smp = torch.as_tensor(smp.reshape(-1), device=dev)
+        st = L.ldpc_mi355_synth_device_noise(ctypes.byref(ic), n, ptr(nv), ptr(smp),
+                                             ctypes.c_uint64(seed),
+                                             ptr(pay) if pay is not None else None, g("bits"),
+                                             g("cw"), g("rm"), g("llr"), g("llr_f32"),
+                                             g("beliefs"), ctypes.c_void_p(stream))
+        keep = (nv, smp, pay)
     if st != 0:
         raise RuntimeError(f"ldpc_mi355_synth_device failed ({st})")
-    out["_keep"] = (nz, pay)    # inputs stay alive until the caller drops the dict
+    out["_keep"] = keep    # inputs stay alive until the caller drops the dict
     return out

@@ -114,6 +114,12 @@ struct SynthArgs {
     float *llr_f32;             // n*E float LLRs, or null (needed for beliefs)
     float *beliefs;             // n*(N+2Zc) float beliefs, or null
     const uint32_t *etab;       // edge table of (BG, Zc)
+    // Given channel noise (both or neither): noise_var n values, the channel's
+    // noise variance per codeblock; noise n*(E/Qm)*2 values, the noise added
+    // to each symbol's real and imaginary part.  Null: noise_db and the
+    // counter-based generator.
+    const float *noise_var;
+    const float *noise;
     uint64_t seed;
     int n, A, B, K, N, E, Zc, Qm, k0, crctype;
 };

@@ -738,12 +738,12 @@ extern "C" int32_t ldpc_mi355_copy_spans(void *const *dst, const void *const *sr
 // ---------------------------------------------------------------------------
 // Device input synthesis (one config, n codeblocks, one launch)
 // ---------------------------------------------------------------------------
-extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t n,
-                                           const float *noise_power_db, uint64_t seed,
-                                           const uint8_t *payload_bits, uint8_t *bits_out,
-                                           uint8_t *codeword_out, uint8_t *rm_bits_out,
-                                           int16_t *llr_out, float *llr_f32_out,
-                                           float *beliefs_out, void *stream) {
+namespace {
+int32_t synth_device(const ldpc_in_config_t *cfg, int32_t n, const float *noise_power_db,
+                     const float *noise_var, const float *noise, uint64_t seed,
+                     const uint8_t *payload_bits, uint8_t *bits_out, uint8_t *codeword_out,
+                     uint8_t *rm_bits_out, int16_t *llr_out, float *llr_f32_out,
+                     float *beliefs_out, void *stream) {
     if (!cfg || n < 0) return -4;
     Params p;
     if (const char *why = derive(*cfg, p)) {
@@ -751,11 +751,13 @@ extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t 
         return -2;
     }
     if (n == 0) return 0;
-    if (!noise_power_db || (beliefs_out && !llr_f32_out)) return -4;
+    if ((!noise_power_db && !noise_var) || (beliefs_out && !llr_f32_out)) return -4;
     Runtime &r = rt();
     if (r.ensure() != 0) return -1;
     SynthArgs a{};
     a.noise_db = noise_power_db;
+    a.noise_var = noise_var;
+    a.noise = noise;
     a.payload = payload_bits;
     a.bits_out = bits_out;
     a.cw_out = codeword_out;
@@ -777,6 +779,30 @@ extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t 
     a.crctype = p.crctype;
     HIP_OK(nrldpc_launch_synth(p.bgn, &a, reinterpret_cast<hipStream_t>(stream)));
     return 0;
+}
+}  // namespace
+
+extern "C" int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t n,
+                                           const float *noise_power_db, uint64_t seed,
+                                           const uint8_t *payload_bits, uint8_t *bits_out,
+                                           uint8_t *codeword_out, uint8_t *rm_bits_out,
+                                           int16_t *llr_out, float *llr_f32_out,
+                                           float *beliefs_out, void *stream) {
+    if (!noise_power_db) return -4;
+    return synth_device(cfg, n, noise_power_db, nullptr, nullptr, seed, payload_bits, bits_out,
+                        codeword_out, rm_bits_out, llr_out, llr_f32_out, beliefs_out, stream);
+}
+
+extern "C" int32_t ldpc_mi355_synth_device_noise(const ldpc_in_config_t *cfg, int32_t n,
+                                                 const float *noise_var, const float *noise,
+                                                 uint64_t seed, const uint8_t *payload_bits,
+                                                 uint8_t *bits_out, uint8_t *codeword_out,
+                                                 uint8_t *rm_bits_out, int16_t *llr_out,
+                                                 float *llr_f32_out, float *beliefs_out,
+                                                 void *stream) {
+    if (!noise_var || !noise) return -4;
+    return synth_device(cfg, n, nullptr, noise_var, noise, seed, payload_bits, bits_out,
+                        codeword_out, rm_bits_out, llr_out, llr_f32_out, beliefs_out, stream);
 }
 
 // ---------------------------------------------------------------------------

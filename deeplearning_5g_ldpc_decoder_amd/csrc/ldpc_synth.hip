@@ -14,9 +14,12 @@
 //             the selection is the (a0 + k) mod Navail-th non-filler position
 //             of the circular buffer (a0 = rank of k0), interleaved in Qm rows
 //   channel   Gray QAM (TS 38.211 5.1) + complex AWGN, max-log LLR per bit
-//             (d1^2 - d0^2) / noise_var over the axis levels
-//             (c_lib/tx_rx_process.c:68-324); int16 output quantized x128,
-//             rounded half away from zero, clipped to +-10000 (:27-39)
+//             (c_lib/tx_rx_process.c:68-324) in the reference's piecewise
+//             linear form and float operation order (kDemapSegs), so given
+//             the same noise the LLRs are the reference's bit for bit; int16
+//             output quantized x128, rounded half away from zero, clipped to
+//             +-10000 (:27-39).  The noise is the counter-based generator's,
+//             or the caller's samples (SynthArgs::noise)
 //   beliefs   (optional) the float rate recovery of the trainable decoder's
 //             inputs (py5gphy/ldpc/nr_ldpc_raterecover.py:15-68): repeats
 //             summed, filler positions 10 * max|LLR|, 2Zc leading zeros
@@ -61,6 +64,61 @@ __device__ __forceinline__ float pam_level(const int *b, int nb) {
     if (nb == 3) return (float)((1 - 2 * b[0]) * (4 - (1 - 2 * b[1]) * (2 - (1 - 2 * b[2]))));
     return (float)((1 - 2 * b[0]) *
                    (8 - (1 - 2 * b[1]) * (4 - (1 - 2 * b[2]) * (2 - (1 - 2 * b[3])))));
+}
+
+// Max-log LLR of bit k of one Gray-PAM axis with 2^nb levels as a function of
+// the received coordinate r, as the reference evaluates it
+// (c_lib/tx_rx_process.c:101-232): the first segment with r < T*A gives
+//     sgn * ((C*A) * (r + O*A)) / noise_var      (r alone when O = 0),
+// each product and sum rounded as written there.  Rows: nb = 2, 3, 4 and
+// k = 0..nb-1; a row ends at the segment with T = kDemapOpen.
+struct DemapSeg {
+    int8_t T, C, sgn, O;
+};
+constexpr int8_t kDemapOpen = 127;
+constexpr DemapSeg kDemapSegs[] = {
+    // nb = 2 (16QAM), k = 0, 1
+    {-2, 8, 1, 1}, {2, 4, 1, 0}, {kDemapOpen, 8, 1, -1},
+    {0, 4, 1, 2}, {kDemapOpen, 4, -1, -2},
+    // nb = 3 (64QAM), k = 0, 1, 2
+    {-6, 16, 1, 3}, {-4, 12, 1, 2}, {-2, 8, 1, 1}, {2, 4, 1, 0}, {4, 8, 1, -1}, {6, 12, 1, -2},
+    {kDemapOpen, 16, 1, -3},
+    {-6, 8, 1, 5}, {-2, 4, 1, 4}, {0, 8, 1, 3}, {2, 8, -1, -3}, {6, 4, -1, -4},
+    {kDemapOpen, 8, -1, -5},
+    {-4, 4, 1, 6}, {0, 4, -1, 2}, {4, 4, 1, -2}, {kDemapOpen, 4, -1, -6},
+    // nb = 4 (256QAM), k = 0 .. 3
+    {-14, 32, 1, 7}, {-12, 28, 1, 6}, {-10, 24, 1, 5}, {-8, 20, 1, 4}, {-6, 16, 1, 3},
+    {-4, 12, 1, 2}, {-2, 8, 1, 1}, {2, 4, 1, 0}, {4, 8, 1, -1}, {6, 12, 1, -2}, {8, 16, 1, -3},
+    {10, 20, 1, -4}, {12, 24, 1, -5}, {14, 28, 1, -6}, {kDemapOpen, 32, 1, -7},
+    {-14, 16, 1, 11}, {-12, 12, 1, 10}, {-10, 8, 1, 9}, {-6, 4, 1, 8}, {-4, 8, 1, 7},
+    {-2, 12, 1, 6}, {0, 16, 1, 5}, {2, 16, -1, -5}, {4, 12, -1, -6}, {6, 8, -1, -7},
+    {10, 4, -1, -8}, {12, 8, -1, -9}, {14, 12, -1, -10}, {kDemapOpen, 16, -1, -11},
+    {-14, 8, 1, 13}, {-10, 4, 1, 12}, {-8, 8, 1, 11}, {-6, 8, -1, 5}, {-2, 4, -1, 4},
+    {0, 8, -1, 3}, {2, 8, 1, -3}, {6, 4, 1, -4}, {8, 8, 1, -5}, {10, 8, -1, -11},
+    {14, 4, -1, -12}, {kDemapOpen, 8, -1, -13},
+    {-12, 4, 1, 14}, {-8, 4, -1, 10}, {-4, 4, 1, 6}, {0, 4, -1, 2}, {4, 4, 1, -2},
+    {8, 4, -1, -6}, {12, 4, 1, -10}, {kDemapOpen, 4, -1, -14},
+};
+
+// First segment of each row: rows (2, 0), (2, 1), (3, 0) .. (3, 2), (4, 0) .. (4, 3).
+struct DemapRows {
+    int first[9];
+    constexpr DemapRows() : first() {
+        int row = 0;
+        first[0] = 0;
+        for (int i = 0; row < 8; ++i)
+            if (kDemapSegs[i].T == kDemapOpen) first[++row] = i + 1;
+    }
+};
+inline constexpr DemapRows kDemapRows{};
+
+__device__ __forceinline__ float demap_pam(float r, int nb, int k, float A, float nv) {
+    int i = kDemapRows.first[(nb == 2 ? 0 : (nb == 3 ? 2 : 5)) + k];
+    while (kDemapSegs[i].T != kDemapOpen && !(r < (float)kDemapSegs[i].T * A)) ++i;
+    const DemapSeg g = kDemapSegs[i];
+    const float x = g.O == 0 ? r : r + (float)g.O * A;
+    const float v = ((float)g.C * A) * x / nv;
+    return g.sgn < 0 ? -v : v;
 }
 
 __device__ __forceinline__ int16_t quantize_llr(float llr) {
@@ -189,12 +247,13 @@ __global__ void __launch_bounds__(kSynthThreads) ldpc_synth_kernel(SynthArgs a) 
     const int S = E / Qm;
     const int lo = B - 2 * Zc, hi = K - 2 * Zc, F = hi - lo, navail = N - F;
     const int a0 = a.k0 < lo ? a.k0 : (a.k0 < hi ? lo : a.k0 - F);
-    const float nv = powf(10.0f, a.noise_db[cb] / 10.0f);
-    const float sigma = sqrtf(nv * 0.5f);
+    const float nv = a.noise_var ? a.noise_var[cb] : powf(10.0f, a.noise_db[cb] / 10.0f);
+    const float sigma = sqrtf(nv / 2.0f);
     const int nb = Qm == 1 ? 1 : Qm / 2;
-    const float amp = Qm <= 2 ? 0.70710678118654752f
-                              : (Qm == 4 ? 0.31622776601683794f
-                                         : (Qm == 6 ? 0.15430334996209191f : 0.07669649888473704f));
+    // the reference's 1 / sqrt(2), 1 / sqrt(10), ... rounded as it computes them
+    const float amp = Qm <= 2 ? 1.0f / sqrtf(2.0f)
+                              : (Qm == 4 ? 1.0f / sqrtf(10.0f)
+                                         : (Qm == 6 ? 1.0f / sqrtf(42.0f) : 1.0f / sqrtf(170.0f)));
     float local_max = 0.0f;
     for (int j = t; j < S; j += nt) {
         int bits[8];
@@ -205,37 +264,33 @@ __global__ void __launch_bounds__(kSynthThreads) ldpc_synth_kernel(SynthArgs a) 
             bits[i] = cw[pos + 2 * Zc];
             if (a.rm_out) a.rm_out[(size_t)cb * E + (size_t)j * Qm + i] = (uint8_t)bits[i];
         }
-        const uint64_t r = rnd64(a.seed, cb, 1, (uint32_t)j);
-        const float u1 = unit_open(r), u2 = unit_open(r << 24);
-        const float rad = sqrtf(-2.0f * logf(u1));
-        const float na = rad * cosf(6.283185307179586f * u2);
-        const float nb_ = rad * sinf(6.283185307179586f * u2);
+        float na, nb_;
+        if (a.noise) {
+            na = a.noise[((size_t)cb * S + j) * 2];
+            nb_ = a.noise[((size_t)cb * S + j) * 2 + 1];
+        } else {
+            const uint64_t r = rnd64(a.seed, cb, 1, (uint32_t)j);
+            const float u1 = unit_open(r), u2 = unit_open(r << 24);
+            const float rad = sqrtf(-2.0f * logf(u1));
+            na = sigma * (rad * cosf(6.283185307179586f * u2));
+            nb_ = sigma * (rad * sinf(6.283185307179586f * u2));
+        }
         float llr[8];
         if (Qm == 1) {
             const float s = (float)(1 - 2 * bits[0]) * amp;
-            llr[0] = 4.0f * ((s + sigma * na) + (s + sigma * nb_)) * amp / nv;
+            llr[0] = 4.0f * ((s + na) + (s + nb_)) * amp / nv;
+        } else if (Qm == 2) {
+            llr[0] = 4.0f * ((float)(1 - 2 * bits[0]) * amp + na) * amp / nv;
+            llr[1] = 4.0f * ((float)(1 - 2 * bits[1]) * amp + nb_) * amp / nv;
         } else {
             int ab[2][4];
             for (int k = 0; k < nb; ++k) {
                 ab[0][k] = bits[2 * k];
                 ab[1][k] = bits[2 * k + 1];
             }
-            const float rx[2] = {pam_level(ab[0], nb) * amp + sigma * na,
-                                 pam_level(ab[1], nb) * amp + sigma * nb_};
-            const int L = 1 << nb;
+            const float rx[2] = {pam_level(ab[0], nb) * amp + na, pam_level(ab[1], nb) * amp + nb_};
             for (int ax = 0; ax < 2; ++ax)
-                for (int k = 0; k < nb; ++k) {
-                    float d0 = 3.0e38f, d1 = 3.0e38f;
-                    for (int v = 0; v < L; ++v) {
-                        int lb[4];
-                        for (int m = 0; m < nb; ++m) lb[m] = (v >> (nb - 1 - m)) & 1;
-                        const float d = rx[ax] - pam_level(lb, nb) * amp;
-                        const float dd = d * d;
-                        if (lb[k]) d1 = dd < d1 ? dd : d1;
-                        else d0 = dd < d0 ? dd : d0;
-                    }
-                    llr[2 * k + ax] = (d1 - d0) / nv;
-                }
+                for (int k = 0; k < nb; ++k) llr[2 * k + ax] = demap_pam(rx[ax], nb, k, amp, nv);
         }
         for (int i = 0; i < Qm; ++i) {
             const size_t o = (size_t)cb * E + (size_t)j * Qm + i;

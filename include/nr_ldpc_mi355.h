@@ -224,6 +224,23 @@ int32_t ldpc_mi355_synth_device(const ldpc_in_config_t *cfg, int32_t n,
                                 uint8_t *codeword_out, uint8_t *rm_bits_out, int16_t *llr_out,
                                 float *llr_f32_out, float *beliefs_out, void *stream);
 
+/* ldpc_mi355_synth_device with the channel noise given instead of drawn
+ * (replaces the noise of the reference's tx_rx_process,
+ * c_lib/tx_rx_process.c:286-324, whose samples come from the C library's
+ * rand()): noise_var (device, n floats) is each codeblock's noise variance
+ * (the reference's powf(10, noise_power_db / 10)), noise (device,
+ * n*(E/Qm)*2 floats) the values added to each symbol's real and imaginary
+ * part (the reference's noise_sigma * randn() in call order).  Modulation,
+ * max-log LLRs and quantization are the reference's operations, so the same
+ * noise gives the reference's LLRs bit for bit.  0, -2, -4 (a noise pointer
+ * NULL), -1. */
+int32_t ldpc_mi355_synth_device_noise(const ldpc_in_config_t *cfg, int32_t n,
+                                      const float *noise_var, const float *noise, uint64_t seed,
+                                      const uint8_t *payload_bits, uint8_t *bits_out,
+                                      uint8_t *codeword_out, uint8_t *rm_bits_out,
+                                      int16_t *llr_out, float *llr_f32_out, float *beliefs_out,
+                                      void *stream);
+
 /* One codeblock of a batch. */
 typedef struct {
     ldpc_in_config_t cfg;  /* HARQ_on: combine into the soft buffer slot */

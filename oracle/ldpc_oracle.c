@@ -663,3 +663,131 @@ double orc_decode_batch_i16(const int16_t *llr, uint8_t *hard, int32_t *status,
     pthread_mutex_destroy(&job.mu);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+/* ------------------------------------------------------------------ */
+/* Test channel (c_lib/tx_rx_process.c:20-324).                       */
+/* ------------------------------------------------------------------ */
+
+/* One N(0, 1) draw from two rand() calls (Box-Muller, :59-65). */
+static float orc_gauss(void)
+{
+    const float u1 = ((float)rand() + 1.0f) / ((float)RAND_MAX + 2.0f);
+    const float u2 = ((float)rand() + 1.0f) / ((float)RAND_MAX + 2.0f);
+    return sqrtf(-2.0f * logf(u1)) * cosf(6.28318530717958647692f * u2);
+}
+
+/* Max-log LLR of one axis bit as the reference writes it (:101-232):
+ * breakpoints bp[0..n-2] in units of A; segment i (r below bp[i] * A, the
+ * last one above all) is sgn[i] * ((mul[i] * A) * (r + off[i] * A)) / nv,
+ * with r alone for off 0. */
+typedef struct {
+    int n;
+    int bp[15], mul[15], sgn[15], off[15];
+} orc_axis_bit_t;
+
+static const orc_axis_bit_t kQam16[2] = {
+    {3, {-2, 2}, {8, 4, 8}, {1, 1, 1}, {1, 0, -1}},
+    {2, {0}, {4, 4}, {1, -1}, {2, -2}},
+};
+static const orc_axis_bit_t kQam64[3] = {
+    {7, {-6, -4, -2, 2, 4, 6}, {16, 12, 8, 4, 8, 12, 16}, {1, 1, 1, 1, 1, 1, 1},
+     {3, 2, 1, 0, -1, -2, -3}},
+    {6, {-6, -2, 0, 2, 6}, {8, 4, 8, 8, 4, 8}, {1, 1, 1, -1, -1, -1}, {5, 4, 3, -3, -4, -5}},
+    {4, {-4, 0, 4}, {4, 4, 4, 4}, {1, -1, 1, -1}, {6, 2, -2, -6}},
+};
+static const orc_axis_bit_t kQam256[4] = {
+    {15, {-14, -12, -10, -8, -6, -4, -2, 2, 4, 6, 8, 10, 12, 14},
+     {32, 28, 24, 20, 16, 12, 8, 4, 8, 12, 16, 20, 24, 28, 32},
+     {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+     {7, 6, 5, 4, 3, 2, 1, 0, -1, -2, -3, -4, -5, -6, -7}},
+    {14, {-14, -12, -10, -6, -4, -2, 0, 2, 4, 6, 10, 12, 14},
+     {16, 12, 8, 4, 8, 12, 16, 16, 12, 8, 4, 8, 12, 16},
+     {1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1},
+     {11, 10, 9, 8, 7, 6, 5, -5, -6, -7, -8, -9, -10, -11}},
+    {12, {-14, -10, -8, -6, -2, 0, 2, 6, 8, 10, 14},
+     {8, 4, 8, 8, 4, 8, 8, 4, 8, 8, 4, 8},
+     {1, 1, 1, -1, -1, -1, 1, 1, 1, -1, -1, -1},
+     {13, 12, 11, 5, 4, 3, -3, -4, -5, -11, -12, -13}},
+    {8, {-12, -8, -4, 0, 4, 8, 12}, {4, 4, 4, 4, 4, 4, 4, 4},
+     {1, -1, 1, -1, 1, -1, 1, -1}, {14, 10, 6, 2, -2, -6, -10, -14}},
+};
+
+static float orc_axis_llr(const orc_axis_bit_t *b, float r, float A, float nv)
+{
+    int i = 0;
+    while (i < b->n - 1 && !(r < (float)b->bp[i] * A)) i++;
+    const float x = b->off[i] == 0 ? r : r + (float)b->off[i] * A;
+    const float v = ((float)b->mul[i] * A) * x / nv;
+    return b->sgn[i] < 0 ? -v : v;
+}
+
+/* Gray PAM level of one axis from its nb bits (MSB first), an exact small
+ * integer (TS 38.211 5.1; :68-99). */
+static int orc_pam_level(const uint8_t *axis_bits, int nb)
+{
+    int lvl = 0;
+    for (int m = nb - 1; m >= 0; m--) {
+        const int s = axis_bits[m] ? -1 : 1;
+        lvl = (m == nb - 1) ? s : s * ((1 << (nb - 1 - m)) - lvl);
+    }
+    return lvl;
+}
+
+int orc_tx_rx(const uint8_t *bits, int nbits, int Qm, float noise_db, unsigned seed,
+              float *noise_out, float *noise_var_out, float *llr_out, int16_t *q_out)
+{
+    if (!bits || !q_out || nbits <= 0) return -1;
+    if (!(Qm == 1 || Qm == 2 || Qm == 4 || Qm == 6 || Qm == 8) || nbits % Qm) return -1;
+    srand(seed);
+    const float nv = powf(10.0f, noise_db / 10.0f);
+    const float sigma = sqrtf(nv / 2.0f);
+    if (noise_var_out) *noise_var_out = nv;
+    const float A = Qm <= 2 ? 1.0f / sqrtf(2.0f)
+                  : Qm == 4 ? 1.0f / sqrtf(10.0f)
+                  : Qm == 6 ? 1.0f / sqrtf(42.0f) : 1.0f / sqrtf(170.0f);
+    const orc_axis_bit_t *tab = Qm == 4 ? kQam16 : Qm == 6 ? kQam64 : kQam256;
+    const int nb = Qm / 2;
+    for (int i = 0; i < nbits / Qm; i++) {
+        const uint8_t *sb = bits + (size_t)i * Qm;
+        float re, im;
+        if (Qm == 1) {
+            re = im = (sb[0] ? -1.0f : 1.0f) * A;
+        } else {
+            uint8_t ax[2][4];
+            for (int k = 0; k < nb; k++) {
+                ax[0][k] = sb[2 * k];
+                ax[1][k] = sb[2 * k + 1];
+            }
+            re = (float)orc_pam_level(ax[0], nb) * A;
+            im = (float)orc_pam_level(ax[1], nb) * A;
+        }
+        const float n_re = sigma * orc_gauss();
+        const float n_im = sigma * orc_gauss();
+        re += n_re;
+        im += n_im;
+        if (noise_out) {
+            noise_out[2 * i] = n_re;
+            noise_out[2 * i + 1] = n_im;
+        }
+        float llr[8];
+        if (Qm == 1) {
+            llr[0] = 4.0f * (re + im) * A / nv;
+        } else if (Qm == 2) {
+            llr[0] = 4.0f * re * A / nv;
+            llr[1] = 4.0f * im * A / nv;
+        } else {
+            for (int k = 0; k < nb; k++) {
+                llr[2 * k] = orc_axis_llr(&tab[k], re, A, nv);
+                llr[2 * k + 1] = orc_axis_llr(&tab[k], im, A, nv);
+            }
+        }
+        for (int j = 0; j < Qm; j++) {
+            const float scaled = llr[j] * 128.0f;
+            int32_t q = (int32_t)(scaled >= 0.0f ? scaled + 0.5f : scaled - 0.5f);
+            q = q > 10000 ? 10000 : (q < -10000 ? -10000 : q);
+            q_out[(size_t)i * Qm + j] = (int16_t)q;
+            if (llr_out) llr_out[(size_t)i * Qm + j] = llr[j];
+        }
+    }
+    return 0;
+}

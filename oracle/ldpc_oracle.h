@@ -146,6 +146,20 @@ void orc_rate_match(const orc_code_t *c, const uint8_t *cw, uint8_t *rm_bits);
 /* 1 when H * cw = 0 over GF(2) for every lifted check. */
 int orc_syndrome_ok(const orc_code_t *c, const uint8_t *cw);
 
+/* The reference's test channel with its own noise stream
+ * (c_lib/tx_rx_process.c:286-324): srand(seed), then per symbol Gray QAM
+ * (:68-99), noise_sigma * randn() added to the real, then the imaginary
+ * part (randn = Box-Muller of two rand() draws, :59-65), piecewise max-log
+ * LLRs (:101-284) and x128 quantization (:27-39), every float operation as
+ * the reference orders it.  bits: nbits (one per byte; nbits % Qm == 0).
+ * noise_out (NULL or 2 per symbol): the values added; noise_var_out (NULL or
+ * one value); llr_out (NULL or nbits floats); q_out: nbits int16.  Uses the
+ * C library's global rand() state (not reentrant).  0, or -1 for invalid
+ * arguments.  Pinned by tests/golden/channel_vectors.npz: the
+ * from-source reference's tx_rx_process after the same srand(seed). */
+int orc_tx_rx(const uint8_t *bits, int nbits, int Qm, float noise_db, unsigned seed,
+              float *noise_out, float *noise_var_out, float *llr_out, int16_t *q_out);
+
 /* CPU baseline: decode n codeblocks of one config with algo 1 on P POSIX
  * threads (one codeblock per task).  llr: n*E int16, hard: n*B/8 bytes,
  * status/iters: n each.  Returns elapsed seconds (monotonic clock). */

@@ -51,6 +51,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.c_void_p
         L.orc_code_init.argtypes = [ctypes.POINTER(OrcCode)] + [ctypes.c_int] * 7
+        L.orc_tx_rx.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint,
+                                P, P, P, P]
         L.orc_params.argtypes = [ctypes.c_int] * 7 + [P]
         L.orc_crc.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_crc.restype = ctypes.c_uint32
@@ -293,6 +295,22 @@ def _constellation(Qm):
             pts[v] = (s[0] * (8 - s[2] * (4 - s[4] * (2 - s[6]))) +
                       1j * s[1] * (8 - s[3] * (4 - s[5] * (2 - s[7])))) / np.sqrt(170)
     return pts
+
+
+def tx_rx(bits, Qm, noise_power_db, seed):
+    """The reference's test channel with its own noise stream after
+    srand(seed) (orc_tx_rx, c_lib/tx_rx_process.c:286-324).  Returns
+    (noise [E/Qm, 2] as added, noise variance, float LLRs, int16 LLRs)."""
+    bits = np.ascontiguousarray(bits, np.uint8)
+    E = len(bits)
+    noise = np.zeros((E // Qm, 2), np.float32)
+    nv = ctypes.c_float()
+    llr = np.zeros(E, np.float32)
+    q = np.zeros(E, np.int16)
+    if lib().orc_tx_rx(_p(bits), E, Qm, ctypes.c_float(noise_power_db), ctypes.c_uint(seed),
+                       _p(noise), ctypes.byref(nv), _p(llr), _p(q)) != 0:
+        raise ValueError("orc_tx_rx: invalid arguments")
+    return noise, float(nv.value), llr, q
 
 
 def channel_llr(bits, Qm, noise_power_db, rng):

@@ -31,6 +31,8 @@ def lib():
         L.emu_unrolled_backward.restype = I
         L.emu_synth.argtypes = [I] * 7 + [P, ctypes.c_uint64] + [P] * 7
         L.emu_synth.restype = I
+        L.emu_synth_noise.argtypes = [I] * 7 + [P, P, P, ctypes.c_uint64] + [P] * 7
+        L.emu_synth_noise.restype = I
         L.emu_set_trained_tables.argtypes = [I, P, P, I]
         L.emu_set_trained_tables.restype = I
         L.emu_set_degree_alpha.argtypes = [P]
@@ -132,8 +134,9 @@ def unrolled(cfg, T, llr, alpha, beta, clip=None, detach=False, cbs=0, g_lq=None
     return out
 
 
-def synth(cfg, n, noise_db, seed=0, payload=None, beliefs=False):
+def synth(cfg, n, noise_db, seed=0, payload=None, beliefs=False, noise=None):
     """Input-synthesis kernel on the CPU.  cfg = (bgn, A, crctype, E, rv, Qm).
+    noise: optional (noise_var[n], samples[n, E/Qm, 2]) given channel noise.
     Returns dict(bits, cw, rm, llr, llr_f32[, beliefs])."""
     from oracle import oracle as O
     bgn, A, crctype, E, rv, Qm = cfg[:6]
@@ -146,9 +149,14 @@ def synth(cfg, n, noise_db, seed=0, payload=None, beliefs=False):
                rm=np.zeros((n, E), np.uint8), llr=np.zeros((n, E), np.int16),
                llr_f32=np.zeros((n, E), np.float32))
     bel = np.zeros((n, nt), np.float32) if beliefs else None
-    rc = lib().emu_synth(bgn, A, crctype, E, rv, Qm, n, _p(nz), ctypes.c_uint64(seed), _p(pay),
-                         _p(out["bits"]), _p(out["cw"]), _p(out["rm"]), _p(out["llr"]),
-                         _p(out["llr_f32"]), _p(bel))
+    nv = smp = None
+    if noise is not None:
+        nv = np.ascontiguousarray(noise[0], np.float32).reshape(n)
+        smp = np.ascontiguousarray(noise[1], np.float32).reshape(-1)
+        assert smp.size == n * (E // Qm) * 2
+    rc = lib().emu_synth_noise(bgn, A, crctype, E, rv, Qm, n, _p(nz), _p(nv), _p(smp),
+                               ctypes.c_uint64(seed), _p(pay), _p(out["bits"]), _p(out["cw"]),
+                               _p(out["rm"]), _p(out["llr"]), _p(out["llr_f32"]), _p(bel))
     assert rc == 0, rc
     if beliefs:
         out["beliefs"] = bel

@@ -3,7 +3,10 @@ channel (deeplearning_5g_ldpc_decoder_amd/csrc/ldpc_synth.hip), and of the float
 rate recovery of the trainable decoder's inputs
 (py5gphy/ldpc/nr_ldpc_raterecover.py:15-68), for tests.  The channel follows
 c_lib/tx_rx_process.c (Gray QAM, max-log LLR = (d1^2 - d0^2) / noise_var,
-x128 quantization rounded half away from zero, clipped to +-10000)."""
+x128 quantization rounded half away from zero, clipped to +-10000); the
+kernel evaluates the same max-log values in the reference's piecewise form,
+so the two agree to float rounding here, and the kernel's exact float form
+is pinned against the reference itself in tests/test_channel.py."""
 import numpy as np
 
 M64 = np.uint64(0xFFFFFFFFFFFFFFFF)

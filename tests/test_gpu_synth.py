@@ -83,3 +83,63 @@ def test_synth_device_bler_matches_host_synthesis():
     _say(f"BLER device-synthesized {e1 / n:.4f}, host-synthesized {e2 / n:.4f}")
     assert 0.02 < e2 / n < 0.98
     assert lo1 <= hi2 and lo2 <= hi1
+
+
+def _channel_cases():
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from test_channel import CASES
+    return CASES
+
+
+@pytest.mark.parametrize("case", _channel_cases(), ids=lambda c: f"Qm{c['cfg'][5]}-E{c['cfg'][3]}")
+def test_synth_device_given_noise_matches_reference(oracle_mod, case):
+    """ldpc_mi355_synth_device_noise on the GPU, given the noise the
+    reference's tx_rx_process drew after srand(seed): its int16 LLRs are the
+    reference's (tests/golden/channel_vectors.npz, written from the
+    from-source reference) and its float LLRs the oracle's, bit for bit."""
+    from deeplearning_5g_ldpc_decoder_amd.batch import synth_device
+    cfg = case["cfg"]
+    _say(f"synth_device_noise cfg={cfg}")
+    out = synth_device(cfg, 1, 0.0, payload=case["payload"][None, :], outputs=("rm", "llr", "llr_f32"),
+                       noise=(np.array([case["noise_var"]]), case["noise"]))
+    torch.cuda.synchronize()
+    c = oracle_mod.code(*cfg)
+    rm = oracle_mod.rate_match(c, oracle_mod.encode(c, oracle_mod.attach_crc(case["payload"], cfg[2])))
+    assert np.array_equal(out["rm"][0].cpu().numpy(), rm)
+    assert np.array_equal(out["llr"][0].cpu().numpy(), case["llr_ref"])
+    assert np.array_equal(out["llr_f32"][0].cpu().numpy(), case["llr_f32"])
+
+
+def test_device_noise_statistics():
+    """The device's own noise (counter-based generator, Box-Muller) against
+    the closed-form statistics of complex AWGN with variance noise_var / 2
+    per dimension, recovered exactly from QPSK float LLRs
+    (llr = 4 * A * y / noise_var): mean, variance, kurtosis, a
+    Kolmogorov-Smirnov distance to N(0, noise_var / 2), and no correlation
+    between real and imaginary parts, neighbouring symbols or codeblocks.
+    Bounds are 6 standard errors (KS: the 1e-6 critical value)."""
+    from scipy import stats
+
+    from deeplearning_5g_ldpc_decoder_amd.batch import synth_device
+    cfg = (1, 8424, 2, 25344, 0, 2, 0)
+    n, ndb = 16, 1.5
+    _say(f"device noise statistics cfg={cfg} n={n}")
+    out = synth_device(cfg, n, ndb, seed=77, outputs=("rm", "llr_f32"))
+    torch.cuda.synchronize()
+    rm = out["rm"].cpu().numpy().astype(np.float64)
+    llr = out["llr_f32"].cpu().numpy().astype(np.float64)
+    nv = float(np.float32(10.0) ** np.float32(ndb / 10.0))
+    A = 1.0 / np.sqrt(2.0)
+    noise = llr * nv / (4.0 * A) - (1.0 - 2.0 * rm) * A      # [n, E]: re, im alternate
+    sd = np.sqrt(nv / 2.0)
+    x = noise.reshape(-1) / sd
+    N = x.size
+    assert abs(x.mean()) < 6.0 / np.sqrt(N)
+    assert abs(x.var() - 1.0) < 6.0 * np.sqrt(2.0 / N)
+    assert abs(stats.kurtosis(x, fisher=False) - 3.0) < 6.0 * np.sqrt(24.0 / N)
+    assert stats.kstest(x, "norm").statistic < 2.63 / np.sqrt(N)
+    re, im = noise[:, 0::2] / sd, noise[:, 1::2] / sd
+    lim = 6.0 / np.sqrt(re.size)
+    assert abs(np.mean(re * im)) < lim
+    assert abs(np.mean(re[:, 1:] * re[:, :-1])) < 6.0 / np.sqrt(re[:, 1:].size)
+    assert abs(np.mean(re[1:] * re[:-1])) < 6.0 / np.sqrt(re[1:].size)
