@@ -1247,24 +1247,36 @@ __device__ __forceinline__ float flip_sign(float v, uint32_t bits, int j) {
     return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) ^ ((bits << (31 - j)) & 0x80000000u));
 }
 
-template <int BG, int R>
-__device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st, etab_cptr etab,
-                                          int zx, int Zc, const DecParams &p) {
+// FP32 belief at byte offset a from a generic or an LDS base pointer.
+__device__ __forceinline__ float f32_at(const char *b, uint32_t a) {
+    return *reinterpret_cast<const float *>(b + a);
+}
+__device__ __forceinline__ void f32_put(char *b, uint32_t a, float v) {
+    *reinterpret_cast<float *>(b + a) = v;
+}
+#ifndef NRLDPC_CPU_EMU
+typedef __attribute__((address_space(3))) float lds_f32;
+__device__ __forceinline__ float f32_at(const lds_char *b, uint32_t a) {
+    return *(const lds_f32 *)(b + a);
+}
+__device__ __forceinline__ void f32_put(lds_char *b, uint32_t a, float v) { *(lds_f32 *)(b + a) = v; }
+#endif
+
+// The row update on beliefs at byte offsets addr[j] from lqb (a generic or
+// an LDS pointer to char).
+template <int BG, int R, typename CharPtr>
+__device__ __forceinline__ void row_f32ms_at(CharPtr lqb, const uint32_t *addr, StateF32<BG> &st,
+                                             const DecParams &p) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     const float o1 = st.n1[R], o2 = st.n2[R];
     const uint32_t sb = st.sgn[R];
     const int oidx = (int)(sb >> 24);
-    int addr[deg];
     float rel[deg];
 #pragma unroll
     for (int j = 0; j < deg; ++j) {
-        const uint32_t t = etab[e0 + j];
-        int c = zx + (int)(t >> 16);
-        c = (c >= Zc) ? c - Zc : c;
-        addr[j] = (int)(t & 0xFFFFu) + c;
         const float ov = (j == oidx) ? o2 : o1;
-        rel[j] = lq[addr[j]] - flip_sign(ov, sb, j);
+        rel[j] = f32_at(lqb, addr[j]) - flip_sign(ov, sb, j);
     }
     float m1 = 3.402823466e+38f, m2 = 3.402823466e+38f;
     int idx = 0;
@@ -1295,11 +1307,28 @@ __device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &
         float v = rel[j] + flip_sign(nv, bits, j);
         if (v > p.fmax) v = p.fmax;
         if (v < p.fmin) v = p.fmin;
-        lq[addr[j]] = v;
+        f32_put(lqb, addr[j], v);
     }
     st.n1[R] = n1;
     st.n2[R] = n2;
     st.sgn[R] = bits | ((uint32_t)idx << 24);
+}
+
+// Row R with run-time shifts from the edge table.
+template <int BG, int R>
+__device__ __forceinline__ void row_f32ms(float *__restrict__ lq, StateF32<BG> &st, etab_cptr etab,
+                                          int zx, int Zc, const DecParams &p) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    uint32_t addr[deg];
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const uint32_t t = etab[e0 + j];
+        int c = zx + (int)(t >> 16);
+        c = (c >= Zc) ? c - Zc : c;
+        addr[j] = 4u * ((t & 0xFFFFu) + (uint32_t)c);
+    }
+    row_f32ms_at<BG, R>(reinterpret_cast<char *>(lq), addr, st, p);
 }
 
 // Rows with the barrier after each group of column-disjoint rows only
@@ -1342,6 +1371,67 @@ __device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const
     return it;
 }
 
+
+// FP32 min-sum with compile-time shifts (every codeblock of the launch has
+// base graph BG and lifting size ZC): row_f32ms_at on LDS beliefs at LDS
+// address 0, each edge's byte offset a constant column part (the DS
+// instruction's immediate) plus the lane's wrapped shift.
+template <int BG, int ZC, int E>
+__device__ __forceinline__ uint32_t fixed_f32_off(uint32_t zx4) {
+    constexpr uint32_t col = 2u * FixedCode<BG, ZC>::col_bytes(E);
+    constexpr uint32_t sh = 2u * FixedCode<BG, ZC>::shift_bytes(E);
+    if constexpr (sh == 0)
+        return col + zx4;
+    else
+        return col + min(zx4 + sh, zx4 + sh - 4u * (uint32_t)ZC);
+}
+
+template <int BG, int ZC, int R, int... J>
+__device__ __forceinline__ void fixed_row_f32ms(lds_char *lqb, StateF32<BG> &st, uint32_t zx4,
+                                                const DecParams &p,
+                                                std::integer_sequence<int, J...>) {
+    constexpr int e0 = Graph<BG>::start(R);
+    const uint32_t addr[] = {fixed_f32_off<BG, ZC, e0 + J>(zx4)...};
+    row_f32ms_at<BG, R>(lqb, addr, st, p);
+}
+
+template <int BG, int ZC, int R>
+__device__ __forceinline__ void fixed_rows_f32ms(lds_char *lqb, StateF32<BG> &st, uint32_t zx4,
+                                                 bool run, const DecParams &p) {
+    if constexpr (R < Graph<BG>::kRows) {
+        constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+        if (run) fixed_row_f32ms<BG, ZC, R>(lqb, st, zx4, p, std::make_integer_sequence<int, deg>{});
+        if constexpr (kRowGroups<BG>.barrier_after[R]) __syncthreads();
+        fixed_rows_f32ms<BG, ZC, R + 1>(lqb, st, zx4, run, p);
+    }
+}
+
+template <int BG, int ZC>
+__device__ int run_fixed_f32ms(float *lq, const DecParams &p, const CrcCtx &crc,
+                               const uint32_t *tab, uint32_t *slots, int *status) {
+    StateF32<BG> st;
+#pragma unroll
+    for (int r = 0; r < Graph<BG>::kRows; ++r) {
+        st.n1[r] = 0.0f;
+        st.n2[r] = 0.0f;
+        st.sgn[r] = 255u << 24;
+    }
+    const int zx = threadIdx.x;
+    uint32_t zx4 = 4u * (uint32_t)zx;
+    lds_char *lqb = fixed_lds_base(reinterpret_cast<int16_t *>(lq));
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        NRLDPC_OPAQUE_LANE(zx4);
+        fixed_rows_f32ms<BG, ZC, 0>(lqb, st, zx4, zx < ZC, p);
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -1444,6 +1534,50 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     }
     if (d.lq_off != kNone) {
         int16_t *out = reinterpret_cast<int16_t *>(o.lq) + d.lq_off;
+        for (int i = t; i < n; i += nt) out[i] = lq[i];
+    }
+}
+
+// Lifting-size specialized FP32 min-sum kernel (algo 4): one codeblock of
+// base graph BG and lifting size ZC per workgroup, beliefs as floats at LDS
+// address 0 (all of its LDS is dynamic), then the CRC table and slots.
+template <int BG, int ZC>
+struct FixedLdsF32 {
+    static constexpr int kBeliefs = Graph<BG>::kCols * ZC;           // N + 2Zc
+    static constexpr int kTabOff = ((kBeliefs * 4 + 15) / 16) * 16;
+    static constexpr int kSlotsOff = kTabOff + 256 * 4;
+    static constexpr int kBytes = kSlotsOff + (kMaxThreads / 64) * 4;
+};
+
+template <int BG, int ZC>
+__global__ void __launch_bounds__(kMaxThreads)
+ldpc_decode_fixed_f32_kernel(DecOut o, DecParams p) {
+    using LdsL = FixedLdsF32<BG, ZC>;
+    NRLDPC_DYNAMIC_LDS(smem);
+    float *lq = reinterpret_cast<float *>(smem);
+    uint32_t *crc_tab = reinterpret_cast<uint32_t *>(smem + LdsL::kTabOff);
+    uint32_t *slots = reinterpret_cast<uint32_t *>(smem + LdsL::kSlotsOff);
+
+    const CbDesc d = o.desc[blockIdx.x];
+    constexpr int n = LdsL::kBeliefs;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int16_t *llr = o.llr + d.llr_off;
+    int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+    recover_fill<float>(lq, d, llr, soft, p.filler, t, nt, n);
+    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
+    __syncthreads();
+
+    int status = 1, it = 0;
+    if (p.max_it > 0) {
+        it = run_fixed_f32ms<BG, ZC>(lq, p, crc, crc_tab, slots, &status);
+        for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
+    }
+    if (t == 0) {
+        o.status[blockIdx.x] = status;
+        o.iters[blockIdx.x] = it;
+    }
+    if (d.lq_off != kNone) {
+        float *out = reinterpret_cast<float *>(o.lq) + d.lq_off;
         for (int i = t; i < n; i += nt) out[i] = lq[i];
     }
 }
@@ -1609,25 +1743,16 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
 
 }  // namespace nrldpc
 
-#ifndef NRLDPC_CPU_EMU
+#if !defined(NRLDPC_CPU_EMU) && defined(NRLDPC_FIXED_TU)
+// ---------------------------------------------------------------------------
+// The lifting-size specialized kernels, compiled in their own translation
+// unit (ldpc_fixed.hip includes this file with NRLDPC_FIXED_TU) so the
+// library's two largest groups of kernels build in parallel.
+// ---------------------------------------------------------------------------
 namespace {
 using namespace nrldpc;
 
-// int16 decode kernel of (general, table).
-const void *i16_kernel(bool general, bool table) {
-    if (table)
-        return general ? reinterpret_cast<const void *>(
-                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, true>)
-                       : reinterpret_cast<const void *>(
-                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, true>);
-    return general ? reinterpret_cast<const void *>(
-                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, false>)
-                   : reinterpret_cast<const void *>(
-                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>);
-}
-
-// Lifting-size specialized kernels (ldpc_decode_fixed_kernel): BG1 at the
-// largest lifting size.
+// Every (general, table, simple) variant of ldpc_decode_fixed_kernel<BG, ZC>.
 template <int BG, int ZC>
 const void *fixed_kernel_of(bool general, bool table, bool simple) {
     if (general)
@@ -1642,26 +1767,125 @@ const void *fixed_kernel_of(bool general, bool table, bool simple) {
                  : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false>);
 }
 
-constexpr uint32_t fixed_pair(int bg, int zc) { return ((uint32_t)bg << 16) | (uint32_t)zc; }
-constexpr uint32_t kFixedPairs[] = {fixed_pair(1, 384)};
+// Only the fast rows without tables (algo 1 with a fast-row decoder row);
+// other variants run the generic kernel.
+template <int BG, int ZC>
+const void *fixed_kernel_fast(bool general, bool table, bool simple) {
+    if (general || table) return nullptr;
+    return simple ? reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false, true>)
+                  : reinterpret_cast<const void *>(&ldpc_decode_fixed_kernel<BG, ZC, false, false>);
+}
 
-// Set by nrldpc_configure_kernels once every specialized kernel is known to
+constexpr uint32_t fixed_pair(int bg, int zc) { return ((uint32_t)bg << 16) | (uint32_t)zc; }
+// BG1 at the largest lifting size (every variant), and the BG2 lifting
+// sizes of the FP32 BLER workload (104 and 208; fast algo-1 rows)
+constexpr uint32_t kFixedPairs[] = {fixed_pair(1, 384), fixed_pair(2, 104), fixed_pair(2, 208)};
+
+// Set by nrldpc_configure_fixed once every specialized kernel is known to
 // have no static LDS (their belief addressing assumes LDS address 0).
 bool g_fixed_ok = false;
 
 std::pair<const void *, int> fixed_kernel_any(bool general, bool table, bool simple,
                                               uint32_t uniform) {
-    if (uniform == fixed_pair(1, 384))
+    switch (uniform) {
+    case fixed_pair(1, 384):
         return {fixed_kernel_of<1, 384>(general, table, simple), FixedLds<1, 384>::kBytes};
-    return {nullptr, 0};
+    case fixed_pair(2, 104):
+        return {fixed_kernel_fast<2, 104>(general, table, simple), FixedLds<2, 104>::kBytes};
+    case fixed_pair(2, 208):
+        return {fixed_kernel_fast<2, 208>(general, table, simple), FixedLds<2, 208>::kBytes};
+    default:
+        return {nullptr, 0};
+    }
 }
 
-// (kernel, dynamic LDS bytes) of the specialized kernel for `uniform`, or
-// (nullptr, 0) when there is none.
-std::pair<const void *, int> fixed_kernel(bool general, bool table, bool simple,
-                                          uint32_t uniform) {
-    if (!g_fixed_ok) return {nullptr, 0};
-    return fixed_kernel_any(general, table, simple, uniform);
+// The FP32 min-sum (algo 4) kernels: the BG2 lifting sizes of the FP32 BLER
+// workload.
+std::pair<const void *, int> fixed_f32_kernel_any(uint32_t uniform) {
+    switch (uniform) {
+    case fixed_pair(2, 104):
+        return {reinterpret_cast<const void *>(&ldpc_decode_fixed_f32_kernel<2, 104>),
+                FixedLdsF32<2, 104>::kBytes};
+    case fixed_pair(2, 208):
+        return {reinterpret_cast<const void *>(&ldpc_decode_fixed_f32_kernel<2, 208>),
+                FixedLdsF32<2, 208>::kBytes};
+    default:
+        return {nullptr, 0};
+    }
+}
+}  // namespace
+
+extern "C" hipError_t nrldpc_configure_fixed() {
+    const int max_lds = 160 * 1024 - 2048;
+    bool fixed_ok = true;
+    for (uint32_t u : kFixedPairs) {
+        const void *fn = fixed_f32_kernel_any(u).first;
+        if (!fn) continue;
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        if (e != hipSuccess) return e;
+        hipFuncAttributes fa;
+        e = hipFuncGetAttributes(&fa, fn);
+        if (e != hipSuccess) return e;
+        if (fa.sharedSizeBytes != 0) fixed_ok = false;
+    }
+    for (uint32_t u : kFixedPairs)
+        for (int g = 0; g < 2; ++g)
+            for (int t = 0; t < 2; ++t)
+                for (int m = 0; m < 2; ++m) {
+                    const void *fn = fixed_kernel_any(g, t, m, u).first;
+                    if (!fn) continue;
+                    hipError_t e =
+                        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+                    if (e != hipSuccess) return e;
+                    hipFuncAttributes fa;
+                    e = hipFuncGetAttributes(&fa, fn);
+                    if (e != hipSuccess) return e;
+                    if (fa.sharedSizeBytes != 0) fixed_ok = false;
+                }
+    g_fixed_ok = fixed_ok;
+    return hipSuccess;
+}
+
+// The specialized kernel for `uniform` ((bgn << 16) | Zc) and the variant
+// (general, table, simple), with its dynamic LDS bytes in *lds_bytes; NULL
+// when there is none (or the kernels were not configured).
+extern "C" const void *nrldpc_fixed_kernel(int general, int table, int simple, uint32_t uniform,
+                                           int *lds_bytes) {
+    if (!g_fixed_ok) return nullptr;
+    const auto fk = fixed_kernel_any(general != 0, table != 0, simple != 0, uniform);
+    if (fk.first && lds_bytes) *lds_bytes = fk.second;
+    return fk.first;
+}
+
+// The specialized FP32 min-sum kernel for `uniform`, as nrldpc_fixed_kernel.
+extern "C" const void *nrldpc_fixed_f32_kernel(uint32_t uniform, int *lds_bytes) {
+    if (!g_fixed_ok) return nullptr;
+    const auto fk = fixed_f32_kernel_any(uniform);
+    if (fk.first && lds_bytes) *lds_bytes = fk.second;
+    return fk.first;
+}
+#endif  // !NRLDPC_CPU_EMU && NRLDPC_FIXED_TU
+
+#if !defined(NRLDPC_CPU_EMU) && !defined(NRLDPC_FIXED_TU)
+extern "C" hipError_t nrldpc_configure_fixed();
+extern "C" const void *nrldpc_fixed_kernel(int general, int table, int simple, uint32_t uniform,
+                                           int *lds_bytes);
+extern "C" const void *nrldpc_fixed_f32_kernel(uint32_t uniform, int *lds_bytes);
+
+namespace {
+using namespace nrldpc;
+
+// int16 decode kernel of (general, table).
+const void *i16_kernel(bool general, bool table) {
+    if (table)
+        return general ? reinterpret_cast<const void *>(
+                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, true>)
+                       : reinterpret_cast<const void *>(
+                             &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, true>);
+    return general ? reinterpret_cast<const void *>(
+                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, true, false>)
+                   : reinterpret_cast<const void *>(
+                         &ldpc_decode_kernel<Algo::kMinSumI16, int16_t, false, false>);
 }
 
 // Packed-tile kernel of (general, table, simple); simple applies to fast rows.
@@ -1706,15 +1930,23 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
             return hipErrorInvalidValue;
         fn = i16_kernel(pv.general != 0, algo != 1);
         if (uniform) {
-            const auto fk = fixed_kernel(pv.general != 0, algo != 1, pv.simple != 0, uniform);
-            if (fk.first) {
-                fn = fk.first;
-                lds = (size_t)fk.second;
+            int fl = 0;
+            if (const void *fk = nrldpc_fixed_kernel(pv.general != 0, algo != 1, pv.simple != 0,
+                                                     uniform, &fl)) {
+                fn = fk;
+                lds = (size_t)fl;
             }
         }
         break;
     case 4:
         fn = reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>);
+        if (uniform) {
+            int fl = 0;
+            if (const void *fk = nrldpc_fixed_f32_kernel(uniform, &fl)) {
+                fn = fk;
+                lds = (size_t)fl;
+            }
+        }
         break;
     case 0:
         return nrldpc_launch_bp(o, p, ncb, threads, max_soft, stream);
@@ -1731,26 +1963,12 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     const int max_lds = 160 * 1024 - 2048;
     hipError_t e = hipSuccess;
     for (int g = 0; g < 2 && e == hipSuccess; ++g)
-        for (int t = 0; t < 2 && e == hipSuccess; ++t) {
+        for (int t = 0; t < 2 && e == hipSuccess; ++t)
             e = hipFuncSetAttribute(i16_kernel(g, t), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     max_lds);
-            for (uint32_t u : kFixedPairs)
-                for (int m = 0; m < 2 && e == hipSuccess; ++m)
-                    e = hipFuncSetAttribute(fixed_kernel_any(g, t, m, u).first,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        }
     if (e != hipSuccess) return e;
-    bool fixed_ok = true;
-    for (int g = 0; g < 2; ++g)
-        for (int t = 0; t < 2; ++t)
-            for (uint32_t u : kFixedPairs)
-                for (int m = 0; m < 2; ++m) {
-                    hipFuncAttributes fa;
-                    e = hipFuncGetAttributes(&fa, fixed_kernel_any(g, t, m, u).first);
-                    if (e != hipSuccess) return e;
-                    if (fa.sharedSizeBytes != 0) fixed_ok = false;
-                }
-    g_fixed_ok = fixed_ok;
+    e = nrldpc_configure_fixed();
+    if (e != hipSuccess) return e;
     e = hipFuncSetAttribute(
         reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumF32, float>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
@@ -1802,4 +2020,4 @@ extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
-#endif  // NRLDPC_CPU_EMU
+#endif  // !NRLDPC_CPU_EMU && !NRLDPC_FIXED_TU

@@ -188,6 +188,16 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
         return 0;
     }
     case 4:
+        if (g_use_fixed && b.uniform == ((2u << 16) | 26u)) {
+            emu_launch(n, threads, (size_t)FixedLdsF32<2, 26>::kBytes,
+                       [&] { ldpc_decode_fixed_f32_kernel<2, 26>(o, p); });
+            return 0;
+        }
+        if (g_use_fixed && b.uniform == ((1u << 16) | 15u)) {
+            emu_launch(n, threads, (size_t)FixedLdsF32<1, 15>::kBytes,
+                       [&] { ldpc_decode_fixed_f32_kernel<1, 15>(o, p); });
+            return 0;
+        }
         emu_launch(n, threads, (size_t)b.max_soft * 4,
                    [&] { ldpc_decode_kernel<Algo::kMinSumF32, float>(o, p); });
         return 0;

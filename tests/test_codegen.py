@@ -20,7 +20,7 @@ from conftest import ROOT
 
 BUILD = os.path.join(ROOT, "deeplearning_5g_ldpc_decoder_amd", "csrc", "_build")
 LLVM = "/opt/rocm/llvm/bin"
-OBJECTS = ["ldpc_kernels", "ldpc_bp", "ldpc_unrolled", "ldpc_synth"]
+OBJECTS = ["ldpc_kernels", "ldpc_fixed", "ldpc_bp", "ldpc_unrolled", "ldpc_synth"]
 
 
 def _device_code(tmp_path, name):

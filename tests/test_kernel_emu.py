@@ -431,6 +431,34 @@ def test_emu_fixed_kernel_matches_generic_and_oracle(oracle_mod, pair, row, tab,
         assert np.array_equal(a["lq"], q), cfg
 
 
+@pytest.mark.parametrize("pair", [(2, 26), (1, 15)])
+@pytest.mark.parametrize("row", [[4, 20, 22936, 0, 1280, 32767, -32767],
+                                 [4, 8, 26000, 300, -900, 2000, -1500]])
+def test_emu_fixed_f32_kernel_matches_generic_and_oracle(oracle_mod, pair, row):
+    """The lifting-size specialized FP32 min-sum kernel (algo 4; compile-time
+    shifts, beliefs at LDS address 0) on uniform batches: bit-exact with the
+    generic FP32 kernel and with the oracle's FP32 decoder (posteriors,
+    status, iterations)."""
+    bgn, zc = pair
+    rng = np.random.default_rng(2000 * bgn + zc + row[1])
+    cfgs, llrs = _all_zc_batch(oracle_mod, rng, per_zc=4, zcs=[zc])
+    cfgs, llrs = zip(*[(c, l) for c, l in zip(cfgs, llrs) if c[0] == bgn])
+    outs = {}
+    try:
+        for fixed in (True, False):
+            emu.use_fixed(fixed)
+            outs[fixed] = emu.decode_batch(list(cfgs), row, list(llrs))
+    finally:
+        emu.use_fixed(True)
+    for cfg, llr, a, b in zip(cfgs, llrs, outs[True], outs[False]):
+        assert (a["status"], a["iters"]) == (b["status"], b["iters"]), cfg
+        assert np.array_equal(a["hard"], b["hard"]) and np.array_equal(a["lq"], b["lq"]), cfg
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+        assert (a["status"], a["iters"]) == (s, k), cfg
+        assert np.array_equal(a["lq"], q), cfg
+
+
 def test_emu_scatter_chunked_and_unaligned(oracle_mod):
     """Rate recovery without repetition reads 16-byte chunks of 8 LLRs when
     the codeblock's LLRs are 16-byte aligned and E is a multiple of 8, and
