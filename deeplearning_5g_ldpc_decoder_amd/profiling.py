@@ -82,22 +82,27 @@ def valu_split(dir_a, bench_a, dir_b, bench_b, kernel="ldpc_decode_fixed_kernel"
             "per_iteration_wave_instr": per_it, "per_codeblock_wave_instr": per_cb}
 
 
-def trace_summary(db_path, top=8):
+def trace_summary(db_path, top=8, by_grid=False):
     """Markdown table of the kernels of a rocprofv3 trace database (rocpd
-    SQLite; durations in ns)."""
+    SQLite; durations in ns).  by_grid: one row per (kernel, workgroups per
+    launch), which tells apart launches of one kernel with different grids
+    (a persistent grid against one workgroup per tile)."""
     import sqlite3
     import statistics
     db = sqlite3.connect(db_path)
-    rows = db.execute("select name, duration from kernels").fetchall()
+    rows = db.execute("select name, duration, grid_x, workgroup_x from kernels").fetchall()
     per = {}
-    for name, dur in rows:
-        per.setdefault(name, []).append(dur / 1e3)
+    for name, dur, gx, wx in rows:
+        key = (name, gx // max(wx, 1) if by_grid else None)
+        per.setdefault(key, []).append(dur / 1e3)
     total = sum(sum(v) for v in per.values())
-    out = ["| kernel | calls | total us | average us | median us | min us | max us | % |",
-           "|---|---|---|---|---|---|---|---|"]
-    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:top]:
+    wg = " workgroups |" if by_grid else ""
+    out = [f"| kernel |{wg} calls | total us | average us | median us | min us | max us | % |",
+           "|---|" + ("---|" if by_grid else "") + "---|---|---|---|---|---|---|"]
+    for (name, g), v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:top]:
         short = name if len(name) <= 110 else name[:107] + "..."
-        out.append(f"| `{short}` | {len(v)} | {sum(v):.1f} | {statistics.mean(v):.2f} | "
+        gcol = f" {g} |" if by_grid else ""
+        out.append(f"| `{short}` |{gcol} {len(v)} | {sum(v):.1f} | {statistics.mean(v):.2f} | "
                    f"{statistics.median(v):.2f} | {min(v):.2f} | {max(v):.2f} | "
                    f"{100 * sum(v) / total:.2f} |")
     return "\n".join(out)
@@ -114,6 +119,8 @@ def main(argv=None):
     k.add_argument("--kernel", default="ldpc_decode")
     t = sub.add_parser("trace-summary")
     t.add_argument("db")
+    t.add_argument("--by-grid", action="store_true",
+                   help="one row per kernel and number of workgroups per launch")
     v = sub.add_parser("valu-split")
     v.add_argument("dir_a")
     v.add_argument("bench_a")
@@ -123,7 +130,7 @@ def main(argv=None):
     if args.cmd == "copy":
         copy_workload(args.mib, args.reps)
     elif args.cmd == "trace-summary":
-        print(trace_summary(args.db))
+        print(trace_summary(args.db, by_grid=args.by_grid))
     elif args.cmd == "counters":
         print(json.dumps(counter_means(args.dir, args.kernel), indent=1))
     else:
