@@ -3,7 +3,10 @@
 For each (base graph, lifting size) it times decode launches with k
 codeblocks per group, k in {1, 2, 4, ..., as many as fit}, on a batch large
 enough to keep every CU busy (tiles >= 2 per CU), and keeps the k with the
-highest codeblock throughput.  Then it times the tile schedule on a
+highest codeblock throughput; for the lifting sizes the lane-per-edge schedule
+takes (one wave per codeblock, a layer's edges one per lane: Zc <= 3 for BG1,
+<= 6 for BG2) it times that schedule too and records the faster as the
+class's schedule.  Then it times the tile schedule on a
 mixed-size batch: one workgroup per tile against a persistent grid that pulls
 the tiles from a device queue, and keeps the faster.  The table goes to
 tile_table.json beside this file; BatchPlan applies it (k through
@@ -12,6 +15,7 @@ plans.
 
     python -m deeplearning_5g_ldpc_decoder_amd.autotune [--zc 2 3 ...] [--bg 1 2]
     python -m deeplearning_5g_ldpc_decoder_amd.autotune --schedule-only
+    python -m deeplearning_5g_ldpc_decoder_amd.autotune --edge-lanes-only
 """
 import argparse
 import json
@@ -25,6 +29,8 @@ from . import _lib
 from .workloads import LIFTING_SIZES, payload_for, waterfall_esn0_db
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
+# lifting sizes the lane-per-edge schedule takes (degree * Zc <= 64 lanes)
+EDGE_LANES_MAX_ZC = {1: 3, 2: 6}
 ROW = [1, 8, 22936, 0, 1280, 32767, -32767]
 
 
@@ -72,8 +78,21 @@ def tune_class(torch, bgn, zc, reps=4, distinct=16, cus=256):
         res[k] = n / dt
     L.ldpc_mi355_set_tile_codeblocks(bgn, zc, 0)
     best = max(res, key=res.get)
-    return dict(k=int(best), payload=A, crc=crc, E=E,
-                codeblocks_per_s={str(k): float(v) for k, v in res.items()})
+    out = dict(k=int(best), payload=A, crc=crc, E=E, schedule="tiles",
+               codeblocks_per_s={str(k): float(v) for k, v in res.items()})
+    if zc <= EDGE_LANES_MAX_ZC[bgn]:
+        # the lane-per-edge schedule: one wave per codeblock, a layer's edges
+        # one per lane
+        n = 8 * cus
+        offs = [(i % distinct) * E for i in range(n)]
+        plan = BatchPlan([cfg] * n, ROW, llr_offsets=offs,
+                         flags=BatchPlan.NO_PACK | BatchPlan.EDGE_LANES, schedule="oneshot")
+        dt = _time_plan(torch, plan, d_llr, reps)
+        plan.close()
+        out["edge_lanes_codeblocks_per_s"] = n / dt
+        if n / dt > res[best]:
+            out["schedule"] = "edge_lanes"
+    return out
 
 
 def tune_schedule(torch, n=4096, reps=20):
@@ -104,9 +123,32 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--schedule-only", action="store_true",
                     help="keep the k table in --out and re-time only the tile schedule")
+    ap.add_argument("--edge-lanes-only", action="store_true",
+                    help="keep the table in --out and re-time only the classes the "
+                         "lane-per-edge schedule takes")
     args = ap.parse_args(argv)
     import torch
     assert torch.cuda.is_available(), "autotuning needs the GPU"
+    if args.edge_lanes_only:
+        from .batch import apply_tuning
+        with open(args.out) as f:
+            table = json.load(f)
+        apply_tuning(args.out)
+        for bgn in (1, 2):
+            for zc in LIFTING_SIZES:
+                if zc > EDGE_LANES_MAX_ZC[bgn]:
+                    continue
+                r = tune_class(torch, bgn, zc)
+                if r is None:
+                    continue
+                table["classes"][f"{bgn}:{zc}"] = r
+                print(f"autotune: BG{bgn} Zc={zc}: k={r['k']} "
+                      f"({r['codeblocks_per_s'][str(r['k'])]:.3g} cb/s), lane-per-edge "
+                      f"{r['edge_lanes_codeblocks_per_s']:.3g} cb/s -> {r['schedule']}",
+                      flush=True)
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1)
+        return 0
     if args.schedule_only:
         from .batch import apply_tuning
         with open(args.out) as f:

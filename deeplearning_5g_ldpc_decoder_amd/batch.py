@@ -26,6 +26,20 @@ _tuning_applied = False
 # tile schedule of plans that do not choose one: persistent (a resident grid
 # pulling tiles from a device queue) when the autotuner measured it faster
 _persistent_default = False
+# (bgn, Zc) classes whose autotuned schedule is lane-per-edge: a plan whose
+# codeblocks all belong to such classes uses it (algo 1)
+_edge_lane_classes = set()
+_zc_cache = {}
+
+
+def _bgn_zc(cfg):
+    key = tuple(int(v) for v in cfg[:6])
+    if key not in _zc_cache:
+        pc = _lib.LdpcConfig()
+        st = _lib.load().gen_ldpc_parameters(ctypes.byref(pc), ctypes.byref(LdpcInConfig(
+            *(key + (0,)))))
+        _zc_cache[key] = (key[0], int(pc.Zc)) if st == 0 else None
+    return _zc_cache[key]
 
 
 def apply_tuning(path=None):
@@ -44,6 +58,10 @@ def apply_tuning(path=None):
     with open(path) as f:
         table = json.load(f)
     _persistent_default = bool(table.get("schedule", {}).get("persistent", False))
+    _edge_lane_classes.clear()
+    for key, v in table.get("classes", {}).items():
+        if v.get("schedule") == "edge_lanes":
+            _edge_lane_classes.add(tuple(int(x) for x in key.split(":")))
     L = _lib.load()
     n = 0
     for key, v in table.get("classes", {}).items():
@@ -69,6 +87,7 @@ class BatchPlan:
     NO_PACK = 4     # plan flag: one codeblock per workgroup
     GENERIC = 8     # plan flag: no kernel specialized for a single (BG, Zc)
     PERSISTENT = 16  # plan flag: packed tiles pulled from a device queue by a resident grid
+    EDGE_LANES = 32  # plan flag: lane-per-edge schedule (small lifting sizes, algo 1 fast rows)
 
     def __init__(self, cfgs, decoder_row, llr_offsets=None, soft_slots=None, want_lq=False,
                  flags=0, schedule=None):
@@ -81,6 +100,9 @@ class BatchPlan:
             raise ValueError(f"unknown tile schedule {schedule!r}")
         if schedule == "persistent":
             flags |= self.PERSISTENT
+        if (_edge_lane_classes and int(decoder_row[0]) == 1 and not flags & self.PACK and
+                all(_bgn_zc(c) in _edge_lane_classes for c in cfgs)):
+            flags |= self.EDGE_LANES
         n = len(cfgs)
         if n == 0:
             raise ValueError("empty batch")

@@ -18,7 +18,7 @@ struct Crc {
     int w;
 };
 
-__device__ inline Crc crc_spec(int type) {
+__device__ __forceinline__ Crc crc_spec(int type) {
     switch (type) {
     case 0: return {0x1021u, 0xFFFFu, 16};
     case 1: return {0x864CFBu, 0xFFFFFFu, 24};
@@ -27,14 +27,14 @@ __device__ inline Crc crc_spec(int type) {
     }
 }
 
-__device__ inline uint32_t crc_xtimes(uint32_t r, const Crc &c) {
+__device__ __forceinline__ uint32_t crc_xtimes(uint32_t r, const Crc &c) {
     const uint32_t top = (r >> (c.w - 1)) & 1u;
     r = (r << 1) & c.mask;
     return top ? r ^ c.poly : r;
 }
 
 // a(x) * b(x) mod g(x) for remainders a, b of degree < w.
-__device__ inline uint32_t crc_mulmod(uint32_t a, uint32_t b, const Crc &c) {
+__device__ __forceinline__ uint32_t crc_mulmod(uint32_t a, uint32_t b, const Crc &c) {
     uint32_t r = 0;
     for (int i = c.w - 1; i >= 0; --i) {
         r = crc_xtimes(r, c);
@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t crc_mulmod_fast(uint32_t a, uint32_t b, cons
 }
 
 // x^(8m) mod g(x)
-__device__ inline uint32_t crc_x8pow(int m, const Crc &c) {
+__device__ __forceinline__ uint32_t crc_x8pow(int m, const Crc &c) {
     uint32_t res = 1u, base = 0x100u;
     while (m) {
         if (m & 1) res = crc_mulmod(res, base, c);
@@ -75,7 +75,7 @@ __device__ inline uint32_t crc_x8pow(int m, const Crc &c) {
     return res;
 }
 
-__device__ inline uint32_t wave_xor(uint32_t v) {
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v ^= __shfl_xor(v, o, 64);
     return v;
@@ -85,7 +85,7 @@ __device__ inline uint32_t wave_xor(uint32_t v) {
 // each row of 16 lanes its XOR in its last lane, two row broadcasts carry
 // rows 0-2 into lane 63, and one readlane returns it (the wave must be
 // whole: DPP reads inactive lanes' registers too).
-__device__ inline uint32_t wave_xor_full(uint32_t v) {
+__device__ __forceinline__ uint32_t wave_xor_full(uint32_t v) {
 #ifdef NRLDPC_CPU_EMU
     return wave_xor(v);
 #else
@@ -101,7 +101,7 @@ __device__ inline uint32_t wave_xor_full(uint32_t v) {
 }
 
 // Hard-decision byte from 8 consecutive beliefs (bit 7 = first belief).
-__device__ inline uint32_t hard_byte(const int16_t *lq, int b) {
+__device__ __forceinline__ uint32_t hard_byte(const int16_t *lq, int b) {
     const uint4 v = *reinterpret_cast<const uint4 *>(lq + 8 * b);
 #ifdef NRLDPC_CPU_EMU
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
@@ -123,7 +123,7 @@ __device__ inline uint32_t hard_byte(const int16_t *lq, int b) {
 #endif
 }
 
-__device__ inline uint32_t hard_byte(const float *lq, int b) {
+__device__ __forceinline__ uint32_t hard_byte(const float *lq, int b) {
     const float4 a = *reinterpret_cast<const float4 *>(lq + 8 * b);
     const float4 c = *reinterpret_cast<const float4 *>(lq + 8 * b + 4);
     const float f[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
@@ -140,7 +140,7 @@ struct CrcCtx {
     int b_lo, b_hi;    // this lane's byte slice [b_lo, b_hi)
 };
 
-__device__ inline CrcCtx crc_setup(int crctype, int nbytes, uint32_t *tab) {
+__device__ __forceinline__ CrcCtx crc_setup(int crctype, int nbytes, uint32_t *tab) {
     CrcCtx ctx;
     ctx.spec = crc_spec(crctype);
     const int t = threadIdx.x, nt = blockDim.x;
@@ -159,7 +159,7 @@ __device__ inline CrcCtx crc_setup(int crctype, int nbytes, uint32_t *tab) {
 // Whole-codeblock CRC of the hard decisions; every lane gets the result.
 // Contains one workgroup barrier.  Caller guarantees lq is settled.
 template <typename T>
-__device__ inline uint32_t block_crc(const T *lq, const CrcCtx &ctx,
+__device__ __forceinline__ uint32_t block_crc(const T *lq, const CrcCtx &ctx,
                                      const uint32_t *tab, uint32_t *slots) {
     const Crc &c = ctx.spec;
     uint32_t r = 0;
@@ -188,7 +188,7 @@ __device__ inline uint32_t block_crc(const T *lq, const CrcCtx &ctx,
 namespace nrldpc {
 // crc_setup with the tables precomputed on the host (DecOut::crc): the byte
 // table is copied to tab (LDS) and the lane's multiplier is one load.
-__device__ inline CrcCtx crc_setup_tab(int crctype, int nbytes, uint32_t *tab,
+__device__ __forceinline__ CrcCtx crc_setup_tab(int crctype, int nbytes, uint32_t *tab,
                                        const uint32_t *crc_tables) {
     CrcCtx ctx;
     ctx.spec = crc_spec(crctype);

@@ -71,7 +71,7 @@ __device__ __forceinline__ uint32_t put_half(uint32_t w, int hi, int v) {
     return hi ? ((w & 0xFFFFu) | ((uint32_t)v << 16)) : ((w & 0xFFFF0000u) | ((uint32_t)v & 0xFFFFu));
 }
 
-__device__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, int filler,
+__device__ __forceinline__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, int filler,
                              int first, int stride, int n) {
     const bool harq = d.harq_on && soft;
     const int chunks = n >> 3;
@@ -102,7 +102,7 @@ __device__ void recover_init(int16_t *lq, const CbDesc &d, const int16_t *soft, 
 // harq: the initial beliefs hold soft-buffer values to add to; otherwise they
 // are 0 and each LLR is stored directly.
 template <int kBatch = kScatterBatch>
-__device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+__device__ __forceinline__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
                                 int first, int stride, bool harq) {
     const RecoverCtx c = recover_ctx(d);
     // floor(j / Qm) = umulhi(j, ceil(2^32 / Qm)), exact for j < 2^16 and Qm <= 8
@@ -138,7 +138,7 @@ __device__ void recover_scatter(int16_t *lq, const CbDesc &d, const int16_t *__r
 constexpr int kScatterChunks = 12;
 
 template <int kChunks = kScatterChunks>
-__device__ void recover_scatter_x8(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+__device__ __forceinline__ void recover_scatter_x8(int16_t *lq, const CbDesc &d, const int16_t *__restrict__ llr,
                                    int first, int stride, bool harq) {
     const RecoverCtx c = recover_ctx(d);
     const uint4 *src = reinterpret_cast<const uint4 *>(llr);
@@ -187,7 +187,7 @@ __device__ void recover_scatter_x8(int16_t *lq, const CbDesc &d, const int16_t *
 // them.  The caller's next barrier orders the soft-buffer write-back before
 // decoding changes the beliefs.
 template <int kBatch = kScatterBatch>
-__device__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, int16_t *soft,
+__device__ __forceinline__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, int16_t *soft,
                             int filler, int first, int stride, int n, bool active) {
     const bool scatter = active && d.E <= d.N - (d.K - d.B);
     if (active) {
@@ -276,7 +276,7 @@ __device__ __forceinline__ uint32_t push_sign(uint32_t acc, int v) {
 // (alpha * max(m - beta, 0) + 2^14) >> 15 as int16 (c_lib/fixedpoint_ldpc_decoder.c:38-46).
 // alpha is in [-32768, 32767] and the offset in [0, 65535], so a 24-bit
 // multiply is exact and the product fits in 32 bits.
-__device__ inline int scale_i16(int m, int alpha, int beta) {
+__device__ __forceinline__ int scale_i16(int m, int alpha, int beta) {
     const int off = max(m - beta, 0);
 #ifdef NRLDPC_CPU_EMU
     const int scaled = alpha * off;
@@ -995,7 +995,7 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
 }
 
 template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple>
-__device__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
+__device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
                              const uint32_t *tab, uint32_t *slots, int *status) {
     // exact rows: one message per edge in global edge order; fast rows:
     // row-paired packed messages
@@ -1198,7 +1198,7 @@ template <int BG>
 constexpr int kPackedWords = packed_word0<BG>(Graph<BG>::kRows);
 
 template <int BG, bool kGeneral, bool kTable>
-__device__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
+__device__ __forceinline__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
                        const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
     const int zx = threadIdx.x;
     int it = 0;
@@ -1345,7 +1345,7 @@ __device__ __forceinline__ void rows_f32ms(float *lq, StateF32<BG> &st, etab_cpt
 }
 
 template <int BG>
-__device__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
+__device__ __forceinline__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
                          const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
     StateF32<BG> st;
 #pragma unroll
@@ -1407,7 +1407,7 @@ __device__ __forceinline__ void fixed_rows_f32ms(lds_char *lqb, StateF32<BG> &st
 }
 
 template <int BG, int ZC>
-__device__ int run_fixed_f32ms(float *lq, const DecParams &p, const CrcCtx &crc,
+__device__ __forceinline__ int run_fixed_f32ms(float *lq, const DecParams &p, const CrcCtx &crc,
                                const uint32_t *tab, uint32_t *slots, int *status) {
     StateF32<BG> st;
 #pragma unroll
@@ -1434,9 +1434,128 @@ __device__ int run_fixed_f32ms(float *lq, const DecParams &p, const CrcCtx &crc,
 }
 
 // ---------------------------------------------------------------------------
+// Lane-per-edge schedule (algo 1, fast rows, small lifting sizes): one wave
+// per codeblock, and each layer (base-graph row) spread over the wave one
+// edge per lane -- lane l holds edge j = l / Zc of lifted check zx = l % Zc
+// (deg * Zc <= 64: Zc <= 3 for BG1, <= 6 for BG2).  A row is one belief
+// load per lane and a tree reduction of (min1, min2, sign parity) over the
+// check's lanes, which sit Zc apart, through LDS; the arithmetic is the
+// packed fast rows' (packed_pair_in / packed_finalize / packed_pair_update),
+// so results are the reference's (c_lib/fixedpoint_ldpc_decoder.c:103-144)
+// whenever DecParams::general is 0.
+// ---------------------------------------------------------------------------
+template <int BG>
+inline constexpr int kEdgeLaneMaxZc =
+    64 / (BG == 1 ? NR_BG1_MAX_DEGREE : NR_BG2_MAX_DEGREE);
+
+// Two smallest magnitudes (15-bit halves) and the sign parity (bit 31) of
+// two partial sets.
+__device__ __forceinline__ uint32_t lpe_combine(uint32_t a, uint32_t b) {
+    const uint32_t a1 = a & 0x7FFFu, a2 = (a >> 16) & 0x7FFFu;
+    const uint32_t b1 = b & 0x7FFFu, b2 = (b >> 16) & 0x7FFFu;
+    const uint32_t m1 = min(a1, b1);
+    const uint32_t m2 = min(max(a1, b1), min(a2, b2));
+    return m1 | (m2 << 16) | ((a ^ b) & 0x80000000u);
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void lpe_row(int16_t *lq, int &msg, int addr, int j, int zx, int Zc,
+                                        uint32_t *keys, const DecParams &p) {
+    constexpr int deg = Graph<BG>::start(R + 1) - Graph<BG>::start(R);
+    const int lane = threadIdx.x;
+    const bool act = j < deg;
+    int rel = 0, a = 0x7FFF;
+    uint32_t sb = 0;
+    if (act) {
+        rel = wrap16((int)lq[addr] - msg);
+        a = rel == -32768 ? 32767 : (rel < 0 ? -rel : rel);
+        sb = rel < 0 ? 1u : 0u;
+    }
+    uint32_t key = (uint32_t)a | (0x7FFFu << 16) | (sb << 31);
+#pragma unroll
+    for (int st = 1; st < deg; st <<= 1) {
+        keys[lane] = key;
+        __syncthreads();
+        if (act && (j & (2 * st - 1)) == 0 && j + st < deg)
+            key = lpe_combine(key, keys[lane + st * Zc]);
+        __syncthreads();
+    }
+    keys[lane] = key;   // lane zx (j = 0) holds its check's result
+    __syncthreads();
+    const uint32_t tot = keys[zx];
+    if (act) {
+        const int m1 = (int)(tot & 0x7FFFu), m2 = (int)((tot >> 16) & 0x7FFFu);
+        int n1, n2;
+        if (p.simple) {
+            n1 = scale_simple(m1, p.alpha);
+            n2 = scale_simple(m2, p.alpha);
+        } else {
+            n1 = scale_i16(min(m1, p.max_llr), p.alpha, p.beta);
+            n2 = scale_i16(min(m2, p.max_llr), p.alpha, p.beta);
+        }
+        const int mag = a == m1 ? n2 : n1;
+        const int m = ((sb ^ (tot >> 31)) & 1u) ? -mag : mag;
+        int v = rel + m;   // the saturating int16 add
+        v = v > 32767 ? 32767 : (v < -32768 ? -32768 : v);
+        v = max(v, p.min_llr);
+        if (!p.simple) v = min(v, p.max_llr);
+        lq[addr] = (int16_t)v;
+        msg = m;
+    }
+    __syncthreads();   // the row's beliefs and keys are read before the next row writes
+}
+
+template <int BG, int R>
+__device__ __forceinline__ void lpe_rows(int16_t *lq, int (&msg)[Graph<BG>::kRows],
+                                         const int (&addr)[Graph<BG>::kRows], int j, int zx,
+                                         int Zc, uint32_t *keys, const DecParams &p) {
+    if constexpr (R < Graph<BG>::kRows) {
+        lpe_row<BG, R>(lq, msg[R], addr[R], j, zx, Zc, keys, p);
+        lpe_rows<BG, R + 1>(lq, msg, addr, j, zx, Zc, keys, p);
+    }
+}
+
+template <int BG>
+__device__ __forceinline__ int run_lpe_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d,
+                           const DecParams &p, const CrcCtx &crc, const uint32_t *tab,
+                           uint32_t *slots, int *status) {
+    __shared__ uint32_t keys[64];
+    constexpr int nR = Graph<BG>::kRows;
+    const int Zc = d.Zc, lane = threadIdx.x;
+    const int j = lane / Zc, zx = lane - j * Zc;
+    // each row's belief index of this lane's edge (lanes past the row's
+    // edges keep 0; they take part in the barriers only)
+    int addr[nR], msg[nR];
+#pragma unroll
+    for (int r = 0; r < nR; ++r) {
+        const int deg = Graph<BG>::start(r + 1) - Graph<BG>::start(r);
+        addr[r] = 0;
+        msg[r] = 0;
+        if (j < deg) {
+            const uint32_t t = etab[Graph<BG>::start(r) + j];
+            int c = zx + (int)(t >> 16);
+            c = c >= Zc ? c - Zc : c;
+            addr[r] = (int)(t & 0xFFFFu) + c;
+        }
+    }
+    int it = 0;
+    *status = 1;
+    while (it < p.max_it) {
+        lpe_rows<BG, 0>(lq, msg, addr, j, zx, Zc, keys, p);
+        ++it;
+        if (block_crc(lq, crc, tab, slots) == 0u) {
+            *status = 0;
+            break;
+        }
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4 };
+// kMinSumI16Edge: algo 1 in the lane-per-edge schedule (run_lpe_i16).
+enum class Algo { kMinSumI16 = 1, kMinSumF32 = 4, kMinSumI16Edge = 5 };
 
 
 // kGeneral selects the exact-for-every-row int16 path (see row_i16) and
@@ -1457,7 +1576,7 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     // fused rate recovery + HARQ combine into LDS; soft buffer write-back
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    if constexpr (A == Algo::kMinSumI16)
+    if constexpr (A == Algo::kMinSumI16 || A == Algo::kMinSumI16Edge)
         recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     else
         recover_fill<T>(lq, d, llr, soft, p.filler, t, nt, n);
@@ -1474,6 +1593,9 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
         } else if constexpr (A == Algo::kMinSumF32) {
             it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
+        } else if constexpr (A == Algo::kMinSumI16Edge) {
+            it = (d.bgn == 1) ? run_lpe_i16<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
+                              : run_lpe_i16<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
         }
         // hard decisions of the final iteration (left untouched when no
         // iteration ran, like the reference)
@@ -1591,7 +1713,7 @@ ldpc_decode_fixed_f32_kernel(DecOut o, DecParams p) {
 // right after each CRC barrier, when every lane has read it.
 // ---------------------------------------------------------------------------
 template <int BG, bool kGeneral, bool kTable, bool kSimple>
-__device__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int slot,
+__device__ __forceinline__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int slot,
                              bool member, const CrcCtx &crc, const uint32_t *tab,
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
                              int *status, int *iters) {
@@ -1984,6 +2106,29 @@ extern "C" hipError_t nrldpc_configure_kernels() {
                 e = hipFuncSetAttribute(tile_kernel(g, t, m),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
     return e;
+}
+
+// Lane-per-edge launch (algo 1, fast rows): one 64-lane workgroup per
+// codeblock; every codeblock must have Zc <= kEdgeLaneMaxZc of its base graph.
+extern "C" hipError_t nrldpc_launch_edge_lanes(const nrldpc::DecOut *o,
+                                               const nrldpc::DecParams *p, int ncb,
+                                               int max_soft, hipStream_t stream) {
+    using namespace nrldpc;
+    if (p->general) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)max_soft * sizeof(int16_t) + 15) & ~(size_t)15;
+    DecOut ov = *o;
+    DecParams pv = *p;
+    void *args[] = {&ov, &pv};
+    const hipError_t e = hipLaunchKernel(
+        reinterpret_cast<const void *>(&ldpc_decode_kernel<Algo::kMinSumI16Edge, int16_t>),
+        dim3(ncb), dim3(64), args, lds, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// Largest lifting size of base graph bgn that the lane-per-edge schedule takes.
+extern "C" int nrldpc_edge_lanes_max_zc(int bgn) {
+    return bgn == 1 ? nrldpc::kEdgeLaneMaxZc<1> : nrldpc::kEdgeLaneMaxZc<2>;
 }
 
 // Packed-tile launch (algos 1-3): ntiles workgroups of kTileThreads lanes,

@@ -58,7 +58,7 @@ __device__ __forceinline__ int llr_index(int k, const RecoverCtx &c) {
 }
 
 template <typename T>
-__device__ void recover_fill(T *lq, const CbDesc &d, const int16_t *__restrict__ llr,
+__device__ __forceinline__ void recover_fill(T *lq, const CbDesc &d, const int16_t *__restrict__ llr,
                              int16_t *soft, int filler, int first, int stride, int n) {
     const RecoverCtx c = recover_ctx(d);
     const bool harq = d.harq_on && soft;

@@ -19,6 +19,9 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const DecOut *o, const DecP
                                            int ncb, int threads, int max_soft, uint32_t uniform,
                                            hipStream_t stream);
 extern "C" hipError_t nrldpc_configure_kernels();
+extern "C" hipError_t nrldpc_launch_edge_lanes(const DecOut *o, const DecParams *p, int ncb,
+                                               int max_soft, hipStream_t stream);
+extern "C" int nrldpc_edge_lanes_max_zc(int bgn);
 extern "C" hipError_t nrldpc_launch_synth(int bgn, const SynthArgs *a, hipStream_t s);
 extern "C" hipError_t nrldpc_launch_tiles(int algo, const DecOut *o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams *p, int ntiles,
@@ -305,6 +308,8 @@ struct ldpc_mi355_plan {
     static constexpr int kQueueRing = 64;
     uint32_t *d_queue = nullptr;
     mutable std::atomic<uint32_t> launches{0};
+    // lane-per-edge schedule (flags bit 5, when every codeblock qualifies)
+    bool edge_lanes = false;
 };
 
 namespace {
@@ -452,7 +457,13 @@ extern "C" int32_t ldpc_mi355_plan_create(ldpc_mi355_plan_t **plan, const ldpc_m
         }
         bind_scale(pl->dp, scale, pl->d_scale);
     }
-    if (want_packing(pl->algo, flags, L.descs)) {
+    // lane-per-edge: algo 1 fast rows, every codeblock small enough for one
+    // wave per layer
+    bool edge = (flags & 32u) && pl->algo == 1 && !pl->dp.general;
+    for (int i = 0; edge && i < n; ++i)
+        edge = L.descs[i].Zc <= nrldpc_edge_lanes_max_zc(L.descs[i].bgn);
+    pl->edge_lanes = edge;
+    if (!edge && want_packing(pl->algo, flags, L.descs)) {
         std::vector<TileDesc> tiles;
         std::vector<uint32_t> order;
         build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, pl->tile_lds);
@@ -519,7 +530,9 @@ extern "C" int32_t ldpc_mi355_plan_decode(const ldpc_mi355_plan_t *plan, const i
     if (plan->d_queue)
         queue = plan->d_queue +
                 2 * (plan->launches.fetch_add(1u) % (uint32_t)ldpc_mi355_plan::kQueueRing);
-    if (plan->ntiles > 0)
+    if (plan->edge_lanes)
+        HIP_OK(nrldpc_launch_edge_lanes(&o, &plan->dp, plan->n, plan->max_soft, s));
+    else if (plan->ntiles > 0)
         HIP_OK(nrldpc_launch_tiles(plan->algo, &o, plan->d_tiles, plan->d_order, &plan->dp,
                                    plan->ntiles, plan->tile_lds, queue, s));
     else

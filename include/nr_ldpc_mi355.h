@@ -262,7 +262,10 @@ typedef struct ldpc_mi355_plan ldpc_mi355_plan_t;
  * persistent packed tiles: a grid of at most one workgroup per resident slot
  * pulls the tiles from a device queue (the plan holds 64 queues, used in
  * turn by successive launches, so at most 64 launches of one plan may be in
- * flight at once).  Returns 0, -2 if any codeblock
+ * flight at once).  bit 5 = lane-per-edge schedule (algo 1 with fast rows,
+ * used when every codeblock has Zc <= 3 for base graph 1, <= 6 for base
+ * graph 2; ignored otherwise): one wave per codeblock, each layer's edges
+ * one per lane.  Returns 0, -2 if any codeblock
  * config is invalid or, for algo 2, needs more iterations than its base
  * graph's trained table holds (index in *bad_index when non-NULL), -3 for
  * an unsupported algo, -1 on a GPU error. */

@@ -45,6 +45,8 @@ def lib():
         L.emu_use_fixed.restype = None
         L.emu_set_persistent_tiles.argtypes = [I]
         L.emu_set_persistent_tiles.restype = None
+        L.emu_set_edge_lanes.argtypes = [I]
+        L.emu_set_edge_lanes.restype = None
         _lib = L
     return _lib
 
@@ -192,6 +194,12 @@ def persistent_tiles(grid):
     """Tile launches pull their tiles from a device queue with at most
     `grid` workgroups (0: one workgroup per tile)."""
     lib().emu_set_persistent_tiles(int(grid))
+
+
+def edge_lanes(on):
+    """Algo-1 fast-row batches (one codeblock per workgroup) in the
+    lane-per-edge schedule."""
+    lib().emu_set_edge_lanes(int(bool(on)))
 
 
 def use_fixed(on):

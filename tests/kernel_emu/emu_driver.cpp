@@ -105,6 +105,7 @@ DecOut outputs(const Batch &b, const int16_t *llr, int16_t *soft_pool, uint8_t *
 }
 
 int g_force_general = -1;   // -1: the library's choice; 0 / 1 forces a path
+int g_edge_lanes = 0;       // 1: algo-1 fast rows in the lane-per-edge schedule
 int g_use_fixed = 1;        // run the lifting-size specialized kernel when one exists
 
 // The specialized kernel of (BG, ZC) for a uniform batch.  The library
@@ -172,6 +173,15 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
         // and the table flag of algos 2/3
         const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
         const bool t = row[0] != 1;
+        if (g_edge_lanes && !g && !t) {
+            // lane-per-edge schedule: one 64-lane workgroup per codeblock
+            for (int i = 0; i < n; ++i)
+                if (b.descs[i].Zc > (b.descs[i].bgn == 1 ? kEdgeLaneMaxZc<1> : kEdgeLaneMaxZc<2>))
+                    return -3;
+            emu_launch(n, 64, lds16,
+                       [&] { ldpc_decode_kernel<Algo::kMinSumI16Edge, int16_t>(o, p); });
+            return 0;
+        }
         if (try_fixed(b.uniform, n, threads, o, p, g, t)) return 0;
         if (!g && !t)
             emu_launch(n, threads, lds16,
@@ -307,3 +317,5 @@ extern "C" int emu_row_general(const int32_t *row) {
                           (int16_t)row[4], (int16_t)row[5], (int16_t)row[6]};
     return decoder_params(dc).general;
 }
+
+extern "C" void emu_set_edge_lanes(int on) { g_edge_lanes = on; }

@@ -212,3 +212,34 @@ def test_specialized_kernels_match_generic(torch_dev, oracle_mod, cfg, row):
         s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr[i]))
         assert (st[i], its[i]) == (s, k), i
         assert np.array_equal(lqs[i], q), i
+
+
+def test_edge_lanes_match_oracle_and_tiles(torch_dev, oracle_mod):
+    """The lane-per-edge schedule (plan flag 32: one wave per codeblock, a
+    layer's edges one per lane) on every lifting size it takes, in one
+    plan: bit-exact with the oracle and with the packed-tile decode of the
+    same batch."""
+    torch = torch_dev
+    from test_kernel_emu import EDGE_ZC
+    from deeplearning_5g_ldpc_decoder_amd.batch import BatchPlan
+    rng = np.random.default_rng(57)
+    cfgs, llrs = [], []
+    for bgn, zcs in EDGE_ZC.items():
+        c_, l_ = _all_zc_batch(oracle_mod, rng, per_zc=6, zcs=zcs)
+        for c, l in zip(c_, l_):
+            if c[0] == bgn:
+                cfgs.append(c)
+                llrs.append(l)
+    d = torch.from_numpy(np.concatenate(llrs)).cuda()
+    for row in (ROW1, [1, 8, 26000, 32, -900, 2000, -1500]):
+        edge = BatchPlan(cfgs, row, want_lq=True, flags=BatchPlan.NO_PACK | BatchPlan.EDGE_LANES,
+                         schedule="oneshot")
+        tiles = BatchPlan(cfgs, row, want_lq=True, flags=BatchPlan.PACK, schedule="oneshot")
+        assert tiles.tiles > 0
+        out = edge.decode(d)
+        ref = tiles.decode(d)
+        for k in ref:
+            assert np.array_equal(out[k].cpu().numpy(), ref[k].cpu().numpy()), (row, k)
+        _check(oracle_mod, edge, out, cfgs, llrs, row)
+        edge.close()
+        tiles.close()

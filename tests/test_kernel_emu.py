@@ -459,6 +459,41 @@ def test_emu_fixed_f32_kernel_matches_generic_and_oracle(oracle_mod, pair, row):
         assert np.array_equal(a["lq"], q), cfg
 
 
+EDGE_ZC = {1: [2, 3], 2: [2, 3, 4, 5, 6]}
+
+
+@pytest.mark.parametrize("row", [[1, 8, 22936, 0, 1280, 32767, -32767],
+                                 [1, 8, 26000, 32, -900, 32767, -32767],
+                                 [1, 8, 22936, 0, 1280, 2000, -1500]])
+def test_emu_edge_lanes_match_oracle(oracle_mod, row):
+    """The lane-per-edge schedule (one wave per codeblock, a layer's edges one
+    per lane, tree reduction of min1/min2/parity over each check's lanes) on
+    every lifting size it takes (BG1 Zc 2-3, BG2 Zc 4 and 6), Qm 2-8, in
+    one batch: bit-exact with the oracle (status, iterations, hard bits,
+    posteriors) for the simple, offset and clipped fast-row decoder rows."""
+    rng = np.random.default_rng(41 + row[3])
+    cfgs, llrs = [], []
+    for bgn, zcs in EDGE_ZC.items():
+        c_, l_ = _all_zc_batch(oracle_mod, rng, per_zc=3, zcs=zcs)
+        for c, l in zip(c_, l_):
+            if c[0] == bgn:
+                cfgs.append(c)
+                llrs.append(l)
+    # every reachable class (BG2 Zc 2, 3 and 5 take no codeblock size)
+    assert {(c[0], oracle_mod.code(*c).Zc) for c in cfgs} == {(1, 2), (1, 3), (2, 4), (2, 6)}
+    emu.edge_lanes(True)
+    try:
+        outs = emu.decode_batch(cfgs, row, llrs)
+    finally:
+        emu.edge_lanes(False)
+    for cfg, llr, out in zip(cfgs, llrs, outs):
+        c = oracle_mod.code(*cfg)
+        s, h, q, k = oracle_mod.decode_soft(c, row, oracle_mod.rate_recover(c, row[4], llr))
+        assert (out["status"], out["iters"]) == (s, k), cfg
+        assert np.array_equal(out["hard"], h), cfg
+        assert np.array_equal(out["lq"], q), cfg
+
+
 def test_emu_scatter_chunked_and_unaligned(oracle_mod):
     """Rate recovery without repetition reads 16-byte chunks of 8 LLRs when
     the codeblock's LLRs are 16-byte aligned and E is a multiple of 8, and
