@@ -209,48 +209,81 @@ __device__ __forceinline__ float seg_sum(float v, int seg) {
     return v;
 }
 
-// One lifted check of row R, reverse.  Called by every lane of the
-// workgroup (the segment sums shuffle across lanes).  lr_in / lr_out are
-// uniform: whether message gradients are read (not in the last iteration)
-// and written (not in the first).
+// One lifted check of row R, reverse, in two phases so that every row of a
+// barrier group issues its loads before any row of the group computes: the
+// row's message-gradient stores go to global memory, which the next row's
+// gradient loads would otherwise wait behind (the rows of a group touch
+// disjoint beliefs and message slots, so their order within the group does
+// not matter).
+template <int DEG>
+struct UbwdLoad {
+    float m1, m2;
+    uint32_t w2, pass;
+    float gin[DEG];     // message gradients of the later iteration
+    int addr[DEG];      // belief indices of the row's edges
+    float g[DEG];       // belief gradients (clamp gating applied in the compute phase,
+                        // so no load result is waited for before the group's last load)
+};
+
 template <int BG, int R>
-__device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const UnrolledArgs &a,
-                                         uetab_ptr etab, const float *ab, int zx, int b,
-                                         int t,
-                                         bool active, bool member, bool lr_in, bool lr_out,
-                                         const uint32_t *__restrict__ rec,
-                                         float *__restrict__ glr) {
+__device__ __forceinline__ void urow_bwd_load(const float *__restrict__ glq, const UnrolledArgs &a,
+                                              uetab_ptr etab, int zx, bool active, bool lr_in,
+                                              const uint32_t *__restrict__ rec,
+                                              const float *__restrict__ glr,
+                                              UbwdLoad<Graph<BG>::start(R + 1) -
+                                                       Graph<BG>::start(R)> &ld) {
+    constexpr int e0 = Graph<BG>::start(R);
+    constexpr int deg = Graph<BG>::start(R + 1) - e0;
+    int Zc = a.Zc;
+    NRLDPC_OPAQUE_S(Zc);
+    NRLDPC_OPAQUE_V(zx);
+    NRLDPC_OPAQUE_S(etab);
+    // every lane loads (inactive ones at lane 0 of their codeblock, results
+    // unused): the load phase stays one straight-line block, so no wait for
+    // its loads is forced at the end of a divergent branch
+    zx = active ? zx : 0;
+    const uint32_t *w = rec + (size_t)R * a.words * Zc + zx;
+    ld.m1 = u2f(w[0]);
+    ld.m2 = u2f(w[Zc]);
+    ld.w2 = w[2 * Zc];
+    ld.pass = a.has_clip ? w[3 * Zc] : 0xFFFFFFFFu;
+    if (lr_in) {
+        const float *lr = glr + (size_t)e0 * Zc + zx;
+#pragma unroll
+        for (int j = 0; j < deg; ++j) ld.gin[j] = lr[j * Zc];
+    }
+#pragma unroll
+    for (int j = 0; j < deg; ++j) {
+        const uint32_t e = etab[e0 + j];
+        int c = zx + (int)(e >> 16);
+        c = (c >= Zc) ? c - Zc : c;
+        ld.addr[j] = (int)(e & 0xFFFFu) + c;
+        ld.g[j] = glq[ld.addr[j]];
+    }
+}
+
+// Called by every lane of the workgroup (the segment sums shuffle across
+// lanes).  lr_in / lr_out are uniform: whether message gradients are read (not
+// in the last iteration) and written (not in the first).
+template <int BG, int R>
+__device__ __forceinline__ void urow_bwd_compute(float *__restrict__ glq, const UnrolledArgs &a,
+                                                 const float *ab, int zx, int b, int t,
+                                                 bool active, bool member, bool lr_in,
+                                                 bool lr_out,
+                                                 const UbwdLoad<Graph<BG>::start(R + 1) -
+                                                                Graph<BG>::start(R)> &ld,
+                                                 float *__restrict__ glr) {
     constexpr int e0 = Graph<BG>::start(R);
     constexpr int deg = Graph<BG>::start(R + 1) - e0;
     constexpr int nR = Graph<BG>::kRows;
     int Zc = a.Zc;
     NRLDPC_OPAQUE_S(Zc);
-    NRLDPC_OPAQUE_V(zx);
-    NRLDPC_OPAQUE_S(etab);
     float ga = 0.0f, gb = 0.0f;
     if (active) {
         const float al = ab[R], be = ab[nR + R];
-        const uint32_t *w = rec + (size_t)R * a.words * Zc + zx;
-        const float m1 = u2f(w[0]), m2 = u2f(w[Zc]);
-        const uint32_t w2 = w[2 * Zc];
-        const uint32_t pass = a.has_clip ? w[3 * Zc] : 0xFFFFFFFFu;
+        const float m1 = ld.m1, m2 = ld.m2;
+        const uint32_t w2 = ld.w2;
         float *lr = glr + (size_t)e0 * Zc + zx;
-        float gin[deg];
-        if (lr_in) {
-#pragma unroll
-            for (int j = 0; j < deg; ++j) gin[j] = lr[j * Zc];
-        }
-        int addr[deg];
-        float gq[deg];
-#pragma unroll
-        for (int j = 0; j < deg; ++j) {
-            const uint32_t e = etab[e0 + j];
-            int c = zx + (int)(e >> 16);
-            c = (c >= Zc) ? c - Zc : c;
-            addr[j] = (int)(e & 0xFFFFu) + c;
-            const float g = glq[addr[j]];
-            gq[j] = ((pass >> j) & 1u) ? g : 0.0f;
-        }
         const uint32_t negs = w2 & 0x7FFFFu;
         const int i1 = (int)((w2 >> 19) & 31u), i2 = (int)((w2 >> 24) & 31u);
         constexpr uint32_t mask = (1u << deg) - 1u;
@@ -258,10 +291,13 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
         const float d1 = m1 - be, d2 = m2 - be;
         const float c1 = (d1 < 0.0f) ? 0.0f : d1, c2 = (d2 < 0.0f) ? 0.0f : d2;
         const bool p1 = d1 >= 0.0f, p2 = d2 >= 0.0f;
+        float gq[deg];
+#pragma unroll
+        for (int j = 0; j < deg; ++j) gq[j] = ((ld.pass >> j) & 1u) ? ld.g[j] : 0.0f;
         float gm1 = 0.0f, gm2 = 0.0f;
 #pragma unroll
         for (int j = 0; j < deg; ++j) {
-            const float gn = lr_in ? gq[j] + gin[j] : gq[j];
+            const float gn = lr_in ? gq[j] + ld.gin[j] : gq[j];
             const float gmag = uflip_sign(gn, bits, j);
             const bool first = (j == i1);
             ga += gmag * (first ? c2 : c1);
@@ -278,7 +314,7 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
         for (int j = 0; j < deg; ++j) {
             const float gabs = (j == i1) ? gm1 * s1 : ((j == i2) ? gm2 * s2 : 0.0f);
             const float gr = a.detach ? 0.0f : gq[j] + gabs;
-            glq[addr[j]] = gr;
+            glq[ld.addr[j]] = gr;
             if (lr_out) lr[j * Zc] = -gr;
         }
     }
@@ -292,20 +328,42 @@ __device__ __forceinline__ void urow_bwd(float *__restrict__ glq, const Unrolled
     }
 }
 
+// First row of the barrier group that holds row R.
+template <int BG>
+__host__ __device__ constexpr int group_first(int r) {
+    while (r > 0 && !kRowGroups<BG>.barrier_after[r - 1]) --r;
+    return r;
+}
+
+// Rows R down to RFirst of one barrier group: this row's loads, then (by
+// recursion) the loads and computes of the rows below it, then this row's
+// compute -- every load of the group is issued before the first compute.
+template <int BG, int R, int RFirst>
+__device__ __forceinline__ void ugroup_bwd(float *glq, const UnrolledArgs &a, uetab_ptr etab,
+                                           const float *ab, int zx, int b, int t, bool active,
+                                           bool member, bool lr_in, bool lr_out,
+                                           const uint32_t *rec, float *glr) {
+    UbwdLoad<Graph<BG>::start(R + 1) - Graph<BG>::start(R)> ld;
+    urow_bwd_load<BG, R>(glq, a, etab, zx, active, lr_in, rec, glr, ld);
+    if constexpr (R > RFirst)
+        ugroup_bwd<BG, R - 1, RFirst>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out,
+                                      rec, glr);
+    urow_bwd_compute<BG, R>(glq, a, ab, zx, b, t, active, member, lr_in, lr_out, ld, glr);
+}
+
+// Rows R down to 0 of one iteration, group by group, a barrier after each.
 template <int BG, int R>
 __device__ __forceinline__ void urows_bwd(float *glq, const UnrolledArgs &a, uetab_ptr etab,
                                           const float *ab, int zx, int b, int t, bool active,
                                           bool member, bool lr_in, bool lr_out,
                                           const uint32_t *rec, float *glr) {
     if constexpr (R >= 0) {
-        urow_bwd<BG, R>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec, glr);
-        // in reverse: a barrier where row R - 1 starts another group
-        if constexpr (R == 0)
-            __syncthreads();
-        else if constexpr (kRowGroups<BG>.barrier_after[R - 1])
-            __syncthreads();
-        urows_bwd<BG, R - 1>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec,
-                             glr);
+        constexpr int first = group_first<BG>(R);
+        ugroup_bwd<BG, R, first>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec,
+                                 glr);
+        __syncthreads();
+        urows_bwd<BG, first - 1>(glq, a, etab, ab, zx, b, t, active, member, lr_in, lr_out, rec,
+                                 glr);
     }
 }
 
@@ -390,7 +448,7 @@ __global__ void __launch_bounds__(kMaxThreads) unrolled_bwd_kernel(UnrolledArgs 
         for (int i = zx; i < n; i += a.lpc) glq[i] = gi ? g[i] + gi[i] : g[i];
     }
     __syncthreads();
-    float *glr = (a.g_lr && member) ? a.g_lr + (size_t)b * nE * a.Zc : nullptr;
+    float *glr = a.g_lr ? a.g_lr + (size_t)(member ? b : a.batch - 1) * nE * a.Zc : nullptr;
     for (int t = a.T - 1; t >= 0; --t) {
         if (a.g_iter && t < a.T - 1) {
             if (member) {
@@ -399,8 +457,9 @@ __global__ void __launch_bounds__(kMaxThreads) unrolled_bwd_kernel(UnrolledArgs 
             }
             __syncthreads();
         }
-        const uint32_t *rec =
-            member ? a.rec + ((size_t)b * a.T + t) * nR * a.words * a.Zc : nullptr;
+        // the last codeblock's record for lanes past the batch (loaded, unused)
+        const int bl = member ? b : a.batch - 1;
+        const uint32_t *rec = a.rec + ((size_t)bl * a.T + t) * nR * a.words * a.Zc;
         const bool lr_on = a.g_lr != nullptr;
         load_ab<BG>(ab, a, t);
         urows_bwd<BG, nR - 1>(glq, a, etab, ab, zx, b, t, active, member, lr_on && t < a.T - 1,
