@@ -649,10 +649,13 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
         report_missing_soft_pool(L.max_slot);
         return -4;
     }
+    // Staging packed to this batch's size (descriptors, tiles, order back to
+    // back): one small copy that the runtime runs as a copy kernel on the
+    // stream.  Copies sized for the whole capacity go to the DMA engine,
+    // which the contexts' streams share and which orders them one after
+    // another behind each other's pending device-to-host copies.
     const size_t doff = 0;
-    const size_t toff = ((sizeof(CbDesc) * (size_t)b->capacity) + 255) & ~(size_t)255;
-    const size_t ooff =
-        toff + (((sizeof(TileDesc) * kTileWaves * (size_t)b->capacity) + 255) & ~(size_t)255);
+    size_t toff = 0, ooff = 0;
     memcpy(b->h_stage + doff, L.descs.data(), sizeof(CbDesc) * (size_t)n);
     size_t bytes = sizeof(CbDesc) * (size_t)n;
     int tile_lds = 0;
@@ -661,9 +664,16 @@ extern "C" int32_t ldpc_mi355_batch_decode(ldpc_mi355_batch_t *b, const ldpc_mi3
         std::vector<uint32_t> order;
         build_tiles(L.descs, r.etab_off, tile_codeblocks, tiles, order, tile_lds);
         b->ntiles = (int)(tiles.size() / kTileWaves);
+        toff = (bytes + 255) & ~(size_t)255;
+        ooff = toff + ((sizeof(TileDesc) * tiles.size() + 255) & ~(size_t)255);
+        bytes = ooff + sizeof(uint32_t) * order.size();
+        if (bytes > b->stage_bytes) {   // at most one workgroup per codeblock
+            fprintf(stderr, "libnrldpc_mi355: batch staging overflow (%zu > %zu bytes)\n", bytes,
+                    b->stage_bytes);
+            return -1;
+        }
         memcpy(b->h_stage + toff, tiles.data(), sizeof(TileDesc) * tiles.size());
         memcpy(b->h_stage + ooff, order.data(), sizeof(uint32_t) * order.size());
-        bytes = ooff + sizeof(uint32_t) * order.size();
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     HIP_OK(hipMemcpyAsync(b->d_stage, b->h_stage, bytes, hipMemcpyHostToDevice, s));

@@ -106,8 +106,11 @@ struct Context {
     ldpc_mi355_batch_t *batch = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    unsigned char *d_out = nullptr;   // status (cap int32), iterations (cap int32), hard bytes
+    // status (n int32), iterations (n int32), hard bytes of the running
+    // batch of n codeblocks; sized for cap codeblocks
+    unsigned char *d_out = nullptr;
     unsigned char *h_out = nullptr;   // pinned copy
+    int n = 0;
     std::vector<ldpc_mi355_cb_t> descs;
     std::vector<int> records;         // records of the running batch
     uint64_t region_off = 0, region_len = 0;
@@ -244,8 +247,8 @@ extern "C" int32_t ldpc_mi355_serve_run(
             }
             const auto tc = std::chrono::steady_clock::now();
             const int32_t *st = reinterpret_cast<const int32_t *>(c.h_out);
-            const int32_t *it = st + cap;
-            const uint8_t *hb = c.h_out + (size_t)8 * cap;
+            const int32_t *it = st + c.n;
+            const uint8_t *hb = c.h_out + (size_t)8 * c.n;
             int k = 0;
             size_t h = 0;
             for (int ri : c.records) {
@@ -384,14 +387,14 @@ extern "C" int32_t ldpc_mi355_serve_run(
             int32_t *d_status = reinterpret_cast<int32_t *>(c->d_out);
             int32_t bad = -1;
             if (ldpc_mi355_batch_decode(c->batch, c->descs.data(), n, llr_base, soft_pool,
-                                        c->d_out + (size_t)8 * cap, d_status, d_status + cap,
+                                        c->d_out + (size_t)8 * n, d_status, d_status + n,
                                         nullptr, c->stream, &bad) != 0) {
                 rc = -1;
                 break;
             }
             uint64_t hard_bytes = 0;
             ldpc_mi355_batch_layout(c->batch, &hard_bytes, nullptr, nullptr, nullptr);
-            if (hipMemcpyAsync(c->h_out, c->d_out, (size_t)8 * cap + hard_bytes,
+            if (hipMemcpyAsync(c->h_out, c->d_out, (size_t)8 * n + hard_bytes,
                                hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipEventRecord(c->done, c->stream) != hipSuccess) {
                 rc = -1;
@@ -399,6 +402,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
             }
             queue.erase(queue.begin(), queue.begin() + (long)take);
             c->busy = true;
+            c->n = n;
             c->region_off = (uint64_t)off;
             c->region_len = got;
             ++inflight;
