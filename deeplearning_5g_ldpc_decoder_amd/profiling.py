@@ -108,6 +108,53 @@ def trace_summary(db_path, top=8, by_grid=False):
     return "\n".join(out)
 
 
+def _kernel_rows(db_path, match):
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    return db.execute("select stream_id, start, end from kernels where name like ? "
+                      "order by start", (f"%{match}%",)).fetchall()
+
+
+def concurrency(rows):
+    """Share of the time between the first start and the last end of rows
+    (stream, start, end) during which k of them run at once, for each k."""
+    events = sorted([(s, 1) for _, s, _ in rows] + [(e, -1) for _, _, e in rows])
+    share, cur, last = {}, 0, events[0][0] if events else 0
+    for t, d in events:
+        share[cur] = share.get(cur, 0) + (t - last)
+        last, cur = t, cur + d
+    total = sum(share.values()) or 1
+    return {k: v / total for k, v in sorted(share.items())}
+
+
+def stream_gaps(db_path, match="ldpc_", min_kernels=20):
+    """Markdown table, one row per stream with at least min_kernels kernels
+    whose name contains match: kernels, median duration, median and 90th
+    percentile idle gap between one such kernel's end and the next one's
+    start on the stream, and the busy share of the stream's span; then the
+    time share of 0, 1, 2, ... such kernels running at once over the trace."""
+    import numpy as np
+    rows = _kernel_rows(db_path, match)
+    per = {}
+    for sid, s, e in rows:
+        per.setdefault(sid, []).append((s, e))
+    out = ["| stream | kernels | median us | median gap us | p90 gap us | busy |",
+           "|---|---|---|---|---|---|"]
+    for sid, v in sorted(per.items()):
+        if len(v) < min_kernels:
+            continue
+        dur = np.array([e - s for s, e in v]) / 1e3
+        gap = np.array([v[i + 1][0] - v[i][1] for i in range(len(v) - 1)]) / 1e3
+        span = (v[-1][1] - v[0][0]) / 1e3
+        out.append(f"| {sid} | {len(v)} | {np.median(dur):.1f} | {np.median(gap):.1f} | "
+                   f"{np.percentile(gap, 90):.1f} | {dur.sum() / span:.2f} |")
+    conc = concurrency(rows)
+    out.append("")
+    out.append("Time share by kernels running at once: " +
+               ", ".join(f"{k}: {v:.3f}" for k, v in conc.items()))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -121,6 +168,11 @@ def main(argv=None):
     t.add_argument("db")
     t.add_argument("--by-grid", action="store_true",
                    help="one row per kernel and number of workgroups per launch")
+    g = sub.add_parser("stream-gaps")
+    g.add_argument("db")
+    g.add_argument("--match", default="ldpc_",
+                   help="kernels whose name contains this string")
+    g.add_argument("--min-kernels", type=int, default=20)
     v = sub.add_parser("valu-split")
     v.add_argument("dir_a")
     v.add_argument("bench_a")
@@ -131,6 +183,8 @@ def main(argv=None):
         copy_workload(args.mib, args.reps)
     elif args.cmd == "trace-summary":
         print(trace_summary(args.db, by_grid=args.by_grid))
+    elif args.cmd == "stream-gaps":
+        print(stream_gaps(args.db, args.match, args.min_kernels))
     elif args.cmd == "counters":
         print(json.dumps(counter_means(args.dir, args.kernel), indent=1))
     else:
