@@ -104,8 +104,8 @@ SERVE_RECORD_DTYPE = np.dtype([
     ("template_id", "<i4"), ("ue", "<i4"), ("proc", "<i4"), ("rv_index", "<i4"),
     ("n_cbs", "<i4"), ("n_pass", "<i4"), ("cb_first", "<i8"), ("hard_first", "<i8"),
     ("batch", "<i4"), ("final_tx", "<i4"), ("arrival_s", "<f8"), ("admitted_s", "<f8"),
-    ("done_s", "<f8")])
-assert SERVE_RECORD_DTYPE.itemsize == 72
+    ("done_s", "<f8"), ("queued_s", "<f8")])
+assert SERVE_RECORD_DTYPE.itemsize == 80
 
 
 _lib = None

@@ -16,15 +16,16 @@
 //     region;
 //   * a batch's LLR spans are copied into its region (host LLRs) or read in
 //     place (device LLRs, as a demapper on the GPU leaves them: no region,
-//     no copy), decoded (the decoder row of the contexts) and its status,
-//     iterations and hard bytes copied back, all asynchronously on the
+//     no copy) and decoded (the decoder row of the contexts), the decode
+//     writing status, iterations and hard bytes straight into the context's
+//     pinned host buffer (no device-to-host copy), asynchronously on the
 //     context's stream; an event per batch is polled for completion;
 //   * a transmission whose codeblocks do not all pass CRC is retransmitted
 //     with the next redundancy version of 0, 2, 3, 1 and soft-combined on
 //     the device into the same slots (HARQ_on = 1); slot (ue, proc, cb) is
 //     ((ue * n_procs) + proc) * max_cbs + cb of the soft-buffer array.
 // Host work per batch is descriptor building, one copy per request (host
-// LLRs only) and three calls into the HIP runtime; nothing is allocated
+// LLRs only), the batch decode and an event record; nothing is allocated
 // inside the loop.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -107,9 +108,10 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     // status (n int32), iterations (n int32), hard bytes of the running
-    // batch of n codeblocks; sized for cap codeblocks
+    // batch of n codeblocks, sized for cap codeblocks: pinned host memory
+    // that the decode writes through its device address (d_out)
+    unsigned char *h_out = nullptr;
     unsigned char *d_out = nullptr;
-    unsigned char *h_out = nullptr;   // pinned copy
     int n = 0;
     std::vector<ldpc_mi355_cb_t> descs;
     std::vector<int> records;         // records of the running batch
@@ -165,8 +167,9 @@ extern "C" int32_t ldpc_mi355_serve_run(
         if (ldpc_mi355_batch_create(&c.batch, cap, decoder_config, 0) != 0 ||
             hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc(&c.d_out, out_bytes) != hipSuccess ||
-            hipHostMalloc(&c.h_out, out_bytes, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc(&c.h_out, out_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipHostGetDevicePointer(reinterpret_cast<void **>(&c.d_out), c.h_out, 0) !=
+                hipSuccess) {
             rc = -1;
             break;
         }
@@ -208,12 +211,15 @@ extern "C" int32_t ldpc_mi355_serve_run(
     std::deque<int> queue;                 // admitted-in-order records
     std::deque<Event> blocked;             // new data waiting for a free HARQ process
     std::vector<char> proc_busy((size_t)cfg->n_ues * cfg->n_procs, 0);
+    // when each HARQ process was last freed (a blocked arrival joins the
+    // decode queue then)
+    std::vector<double> proc_freed((size_t)cfg->n_ues * cfg->n_procs, 0.0);
     int32_t n_rec = 0;
     int64_t n_cb = 0, n_hard = 0;
     int inflight = 0;
     const auto t0 = std::chrono::steady_clock::now();
 
-    auto new_record = [&](const Event &e, int proc, double t_arr) -> int {
+    auto new_record = [&](const Event &e, int proc, double t_arr, double t_queued) -> int {
         if (n_rec >= max_records) return -1;
         const int32_t nc = templates[e.tpl].n_cbs;
         if (n_cb + nc > max_cb_results) return -1;
@@ -228,6 +234,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
         r.hard_first = -1;
         r.batch = -1;
         r.arrival_s = t_arr;
+        r.queued_s = t_queued;
         r.admitted_s = -1.0;
         r.done_s = -1.0;
         n_cb += nc;
@@ -269,8 +276,10 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 if (!r.final_tx)
                     pending.push({now + cfg->harq_rtt_s, seq++, true, r.ue, r.template_id, r.proc,
                                   r.rv_index + 1});
-                else
+                else {
                     proc_busy[(size_t)r.ue * cfg->n_procs + r.proc] = 0;
+                    proc_freed[(size_t)r.ue * cfg->n_procs + r.proc] = now;
+                }
                 k += r.n_cbs;
                 h += (size_t)nb;
             }
@@ -288,7 +297,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
             if (!e.retx) {
                 blocked.push_back(e);
             } else {
-                const int ri = new_record(e, e.proc, e.t);
+                const int ri = new_record(e, e.proc, e.t, e.t);
                 if (ri < 0) {
                     rc = -4;
                     break;
@@ -310,7 +319,9 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 continue;
             }
             proc_busy[(size_t)e.ue * cfg->n_procs + proc] = 1;
-            const int ri = new_record(e, proc, e.t);
+            const double t_queued =
+                std::max(e.t, proc_freed[(size_t)e.ue * cfg->n_procs + proc]);
+            const int ri = new_record(e, proc, e.t, t_queued);
             if (ri < 0) {
                 rc = -4;
                 break;
@@ -392,11 +403,7 @@ extern "C" int32_t ldpc_mi355_serve_run(
                 rc = -1;
                 break;
             }
-            uint64_t hard_bytes = 0;
-            ldpc_mi355_batch_layout(c->batch, &hard_bytes, nullptr, nullptr, nullptr);
-            if (hipMemcpyAsync(c->h_out, c->d_out, (size_t)8 * n + hard_bytes,
-                               hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-                hipEventRecord(c->done, c->stream) != hipSuccess) {
+            if (hipEventRecord(c->done, c->stream) != hipSuccess) {
                 rc = -1;
                 break;
             }
@@ -426,7 +433,6 @@ extern "C" int32_t ldpc_mi355_serve_run(
         if (c.batch) ldpc_mi355_batch_destroy(c.batch);
         if (c.done) (void)hipEventDestroy(c.done);
         if (c.stream) (void)hipStreamDestroy(c.stream);
-        if (c.d_out) (void)hipFree(c.d_out);
         if (c.h_out) (void)hipHostFree(c.h_out);
     }
     if (rc == -5)

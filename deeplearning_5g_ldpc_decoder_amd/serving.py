@@ -5,7 +5,10 @@ Requests are slots of 1-64 codeblocks of mixed base graph / lifting size /
 rate / modulation, each from one UE's HARQ process and carrying a deadline.
 The loop (``Server.run``) runs in real time:
 
-* arrivals join a FIFO queue at their arrival instant (Poisson process);
+* new-data arrivals (Poisson process) wait for a free HARQ process of their
+  UE, then join a FIFO decode queue; retransmissions join it when due.
+  Latency is counted from joining that queue (``Request.latency``) and, as
+  well, from arrival (``Request.latency_from_arrival``);
 * whenever a launch context is free, the queue head is admitted as one batch
   for as long as the batch stays within ``max_batch_codeblocks`` and its
   LLRs fit in the device LLR pool (``DevicePool``, first-fit over one HBM
@@ -149,6 +152,7 @@ class Request:
     deadline: float
     n_cbs: int = 0
     info_bits: int = 0
+    queued: float = -1.0      # joined the decode queue (-1: at arrival)
     admitted: float = -1.0
     done: float = -1.0
     batch_id: int = -1
@@ -161,6 +165,16 @@ class Request:
 
     @property
     def latency(self):
+        """Decode latency: from joining the decode queue to the results on
+        the host.  New data joins it once its UE has a free HARQ process (a
+        UE sends a transport block only on one); a retransmission when it is
+        due."""
+        return self.done - (self.queued if self.queued >= 0 else self.arrival)
+
+    @property
+    def latency_from_arrival(self):
+        """From arrival, including the wait of new data for a free HARQ
+        process of its UE."""
         return self.done - self.arrival
 
     @property
@@ -422,6 +436,7 @@ class ServeStats:
     def summary(self, deadline_s):
         done = [r for r in self.requests if r.done >= 0]
         lat = np.array([r.latency for r in done]) if done else np.array([np.nan])
+        lat_a = np.array([r.latency_from_arrival for r in done]) if done else np.array([np.nan])
         # goodput: payload bits of the codeblocks that passed CRC on the last
         # transmission of their HARQ chain
         info = sum(int(self.cb_info[r.template][r.status == 0].sum())
@@ -433,6 +448,10 @@ class ServeStats:
                            "p99": float(np.percentile(lat, 99) * 1e3),
                            "mean": float(np.mean(lat) * 1e3)},
             "deadline_miss_rate": float(np.mean(lat > deadline_s)),
+            "latency_from_arrival_ms": {"p50": float(np.percentile(lat_a, 50) * 1e3),
+                                        "p99": float(np.percentile(lat_a, 99) * 1e3),
+                                        "mean": float(np.mean(lat_a) * 1e3)},
+            "deadline_miss_rate_from_arrival": float(np.mean(lat_a > deadline_s)),
             "info_gbps": info / self.wall_s / 1e9 if self.wall_s > 0 else 0.0,
             "codeblocks": int(sum(r.n_cbs for r in done)),
             "transmission_fail_rate": float(np.mean([not r.ok for r in done])) if done else 0.0,
@@ -469,6 +488,7 @@ class Server:
         pool = self.ex.pool
         stats = ServeStats(cb_info=pool.cb_info)
         proc_busy = np.zeros((self.n_ues, self.n_procs), bool)
+        proc_freed = np.zeros((self.n_ues, self.n_procs))   # when each process was last freed
         pending = []        # (time, seq, Request) not yet arrived, heap-ordered by time
         import heapq
         seq = 0
@@ -483,10 +503,10 @@ class Server:
         t0 = self.clock()
         inflight = 0
 
-        def make(kind_tpl, ue, proc, rvi, t_arr):
+        def make(kind_tpl, ue, proc, rvi, t_arr, t_queued):
             nonlocal rid
             r = Request(rid=rid, template=kind_tpl, ue=ue, proc=proc, rv_index=rvi,
-                        arrival=t_arr, deadline=t_arr + self.deadline_s,
+                        arrival=t_arr, deadline=t_queued + self.deadline_s, queued=t_queued,
                         n_cbs=len(pool.cfgs[kind_tpl]), info_bits=pool.info_bits[kind_tpl])
             rid += 1
             stats.requests.append(r)
@@ -511,6 +531,7 @@ class Server:
                         seq += 1
                     else:
                         proc_busy[r.ue, r.proc] = False
+                        proc_freed[r.ue, r.proc] = now
             # arrivals
             while pending and pending[0][0] <= now:
                 t_arr, _, ev = heapq.heappop(pending)
@@ -518,13 +539,14 @@ class Server:
                     blocked.append((t_arr, ev[1], ev[2]))
                 else:
                     _, ue, tpl, proc, rvi = ev
-                    queue.append(make(tpl, ue, proc, rvi, t_arr))
+                    queue.append(make(tpl, ue, proc, rvi, t_arr, t_arr))
             still = []
             for t_arr, ue, tpl in blocked:
                 free = np.flatnonzero(~proc_busy[ue])
                 if len(free):
                     proc_busy[ue, free[0]] = True
-                    queue.append(make(tpl, ue, int(free[0]), 0, t_arr))
+                    queue.append(make(tpl, ue, int(free[0]), 0, t_arr,
+                                      max(t_arr, float(proc_freed[ue, free[0]]))))
                 else:
                     still.append((t_arr, ue, tpl))
             stats.max_blocked_arrivals = max(stats.max_blocked_arrivals, len(still))
@@ -654,7 +676,8 @@ class NativeServer:
             a, m = int(r["cb_first"]), int(r["n_cbs"])
             q = Request(rid=i, template=t, ue=int(r["ue"]), proc=int(r["proc"]),
                         rv_index=int(r["rv_index"]), arrival=float(r["arrival_s"]),
-                        deadline=float(r["arrival_s"]) + self.deadline_s, n_cbs=m,
+                        queued=float(r["queued_s"]),
+                        deadline=float(r["queued_s"]) + self.deadline_s, n_cbs=m,
                         info_bits=pool.info_bits[t], admitted=float(r["admitted_s"]),
                         done=float(r["done_s"]), batch_id=int(r["batch"]))
             q.status = st[a:a + m].copy()

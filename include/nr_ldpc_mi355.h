@@ -382,13 +382,20 @@ typedef struct {
 
 /* One transmission, as served.  Codeblock results: status and iterations at
  * [cb_first, cb_first + n_cbs) of the result arrays; hard bytes at byte
- * hard_first of hard_out (-1 when not recorded). */
+ * hard_first of hard_out (-1 when not recorded).  Times in seconds from the
+ * start of the run: arrival_s the arrival (new data) or the due time of a
+ * retransmission (its failed transmission's completion + harq_rtt_s);
+ * queued_s when it joined the decode queue: a retransmission at arrival_s,
+ * new data once its UE had a free HARQ process (the later of arrival_s and
+ * the completion that freed the process); admitted_s when its batch was
+ * launched; done_s when the loop saw the batch complete. */
 typedef struct {
     int32_t template_id, ue, proc, rv_index;
     int32_t n_cbs, n_pass;          /* codeblocks, codeblocks that passed CRC */
     int64_t cb_first, hard_first;
     int32_t batch, final_tx;        /* batch index; 1 = last transmission of its chain */
-    double arrival_s, admitted_s, done_s;   /* seconds from the start of the run */
+    double arrival_s, admitted_s, done_s;
+    double queued_s;
 } ldpc_mi355_serve_record_t;
 
 typedef struct {
@@ -407,7 +414,8 @@ typedef struct {
  * (llr_host) are copied -- with llr_source DEVICE or HOST_MAPPED each
  * batch reads its requests' LLRs where they are, with no region and no
  * copy -- decoded with decoder_config
- * and its results copied back asynchronously on its context's stream; a
+ * and its results written by the decode into the context's pinned host
+ * buffer, asynchronously on its context's stream; a
  * transmission with a failed codeblock is retransmitted with the next rv of
  * 0, 2, 3, 1 after harq_rtt_s and combined into the same soft slots of
  * soft_pool (device, 68*384 int16 per slot).  Fills records (one per
