@@ -20,9 +20,12 @@ namespace nrldpc {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kMaxZc = 384;
 // CRC tables on the device, per CRC type (CRC16, 24A, 24B, 24C): the 256-entry
-// byte table, then x^(8m) mod g for m = 0 .. kMaxCrcBytes.
+// byte table, then x^(8m) mod g for m = 0 .. kMaxCrcBytes, then x^m mod g for
+// m = 0 .. kMaxCrcBits - 1 (the weight of a bit with m bits after it).
 constexpr int kMaxCrcBytes = 8448 / 8;
-constexpr int kCrcTabWords = 256 + kMaxCrcBytes + 1;
+constexpr int kMaxCrcBits = 8448;
+constexpr int kCrcBitOff = 256 + kMaxCrcBytes + 1;
+constexpr int kCrcTabWords = kCrcBitOff + kMaxCrcBits;
 constexpr int kMaxThreads = 384;         // one lane per lifted check
 constexpr int kMaxSoft = 68 * kMaxZc;    // N + 2Zc for BG1 Zc=384
 

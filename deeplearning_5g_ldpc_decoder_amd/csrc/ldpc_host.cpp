@@ -514,6 +514,9 @@ namespace nrldpc {
 // Per CRC type: the byte table, then x^(8m) mod g for m = 0..kMaxCrcBytes
 // (t * x^8 mod g = ((t << 8) mod x^w) ^ tab[t >> (w - 8)]).  A lane whose
 // byte slice has m bytes after it multiplies its partial remainder by entry m.
+// Then x^m mod g for m < kMaxCrcBits: bit i of a B-bit block contributes
+// x^(B - 1 - i) to the block's remainder (times x^w, which is invertible
+// mod g, so a zero test needs no x^w).
 void crc_device_tables(uint32_t *out) {
     const CrcTables &T = crc_tables();
     for (int t = 0; t < 4; ++t) {
@@ -525,6 +528,12 @@ void crc_device_tables(uint32_t *out) {
         for (int m = 0; m <= kMaxCrcBytes; ++m) {
             o[256 + m] = r;
             r = ((r << 8) & mask) ^ T.tab[t][(r >> (w - 8)) & 0xFFu];
+        }
+        // x^m mod g, one bit at a time
+        r = 1u;
+        for (int m = 0; m < kMaxCrcBits; ++m) {
+            o[kCrcBitOff + m] = r;
+            r = ((r >> (w - 1)) & 1u) ? ((r << 1) & mask) ^ T.poly[t] : (r << 1) & mask;
         }
     }
 }

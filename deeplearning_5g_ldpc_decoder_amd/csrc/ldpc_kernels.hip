@@ -994,9 +994,40 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
     }
 }
 
+// Lanes of the specialized kernel: ZC rounded up to whole waves.
+template <int ZC>
+inline constexpr int kFixedLanes = (ZC + 63) / 64 * 64;
+
+// CRC zero test of the hard decisions, linear form: lane zx holds bit
+// i = c * ZC + zx of each systematic column c, whose weight x^(B - 1 - i) mod g
+// (0 past the B-bit block, and for lanes zx >= ZC) sits in LDS at
+// wts[c * lanes + zx]; the remainder of the block (up to the factor x^w, which
+// is invertible mod g) is the XOR of the weights of the set bits.  Two VALU
+// ops per column, against a byte-serial CRC and a 24-step remainder multiply
+// per lane.  Contains one workgroup barrier; every lane gets the result.
+template <int BG, int ZC>
+__device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32_t zx2,
+                                                     const uint32_t *wts, uint32_t *slots) {
+    constexpr int kLanes = kFixedLanes<ZC>;
+    const int zx = threadIdx.x;
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int c = 0; c < Graph<BG>::kInfoCols; ++c) {
+        const int q = *(const lds_i16 *)(lqb + 2u * (uint32_t)(c * ZC) + zx2);
+        acc ^= wts[c * kLanes + zx] & (uint32_t)(q >> 31);
+    }
+    acc = wave_xor_full(acc);
+    if ((zx & 63) == 0) slots[zx >> 6] = acc;
+    __syncthreads();
+    uint32_t total = 0u;
+#pragma unroll
+    for (int w = 0; w < kLanes / 64; ++w) total ^= slots[w];
+    return total;
+}
+
 template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple>
-__device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, const CrcCtx &crc,
-                             const uint32_t *tab, uint32_t *slots, int *status) {
+__device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, const uint32_t *wts,
+                                             uint32_t *slots, int *status) {
     // exact rows: one message per edge in global edge order; fast rows:
     // row-paired packed messages
     constexpr int kWords =
@@ -1030,7 +1061,7 @@ __device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, co
                                                       scale_row<BG, kTable>(p, it), alpha0, rows);
         }
         ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
+        if (fixed_crc_linear<BG, ZC>(lqb, 2u * (uint32_t)zx, wts, slots) == 0u) {
             *status = 0;
             break;
         }
@@ -1615,39 +1646,48 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
 // BG and lifting size ZC per workgroup (run_fixed_i16).  All of its LDS is
 // dynamic with the beliefs at offset 0, so every belief access is a DS
 // instruction with an immediate column offset: LQ (N + 2Zc int16), then the
-// CRC table and the per-wave CRC slots (fixed_lds_bytes).
+// per-wave CRC slots and the lanes' CRC bit weights (fixed_crc_linear).
 template <int BG, int ZC>
 struct FixedLds {
     static constexpr int kBeliefs = Graph<BG>::kCols * ZC;           // N + 2Zc
-    static constexpr int kTabOff = ((kBeliefs * 2 + 15) / 16) * 16;
-    static constexpr int kSlotsOff = kTabOff + 256 * 4;
-    static constexpr int kBytes = kSlotsOff + (kMaxThreads / 64) * 4;
+    static constexpr int kSlotsOff = ((kBeliefs * 2 + 15) / 16) * 16;
+    static constexpr int kWtsOff = kSlotsOff + 64;
+    static constexpr int kBytes = kWtsOff + Graph<BG>::kInfoCols * kFixedLanes<ZC> * 4;
 };
 
 template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple = false>
 __global__ void __launch_bounds__(kMaxThreads)
 ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     using LdsL = FixedLds<BG, ZC>;
+    constexpr int kLanes = kFixedLanes<ZC>;
     NRLDPC_DYNAMIC_LDS(smem);
     int16_t *lq = reinterpret_cast<int16_t *>(smem);
-    uint32_t *crc_tab = reinterpret_cast<uint32_t *>(smem + LdsL::kTabOff);
     uint32_t *slots = reinterpret_cast<uint32_t *>(smem + LdsL::kSlotsOff);
+    uint32_t *wts = reinterpret_cast<uint32_t *>(smem + LdsL::kWtsOff);
 
     const CbDesc d = o.desc[blockIdx.x];
     constexpr int n = LdsL::kBeliefs;
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    // CRC tables first (their own LDS region): the loads overlap the recovery,
-    // whose barriers publish the table
-    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
+    // CRC bit weights first (their own LDS region): the loads overlap the
+    // recovery, whose barriers publish them.  Bit i of the B-bit block has
+    // weight x^(B - 1 - i) mod g.
+    {
+        const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
+        const int nb = (int)d.B;
+#pragma unroll
+        for (int c = 0; c < Graph<BG>::kInfoCols; ++c) {
+            const int i = c * ZC + t;
+            wts[c * kLanes + t] = (t < ZC && i < nb) ? xb[nb - 1 - i] : 0u;
+        }
+    }
     recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     __syncthreads();
 
     int status = 1, it = 0;
     if (p.max_it > 0) {
-        it = run_fixed_i16<BG, ZC, kGeneral, kTable, kSimple>(lq, p, crc, crc_tab, slots,
-                                                              &status);
+        it = run_fixed_i16<BG, ZC, kGeneral, kTable, kSimple>(lq, p, wts, slots, &status);
         for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
     }
     if (t == 0) {
@@ -2051,7 +2091,9 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
         if (algo != 1 && (pv.scale_iters < 1 || !pv.scale[0] || !pv.scale[1]))
             return hipErrorInvalidValue;
         fn = i16_kernel(pv.general != 0, algo != 1);
-        if (uniform) {
+        // the specialized kernels run one lane per lifted check, Zc rounded
+        // up to whole waves
+        if (uniform && threads == (int)(((uniform & 0xFFFFu) + 63u) / 64u * 64u)) {
             int fl = 0;
             if (const void *fk = nrldpc_fixed_kernel(pv.general != 0, algo != 1, pv.simple != 0,
                                                      uniform, &fl)) {
