@@ -1003,8 +1003,9 @@ inline constexpr int kFixedLanes = (ZC + 63) / 64 * 64;
 // (0 past the B-bit block, and for lanes zx >= ZC) sits in LDS at
 // wts[c * lanes + zx]; the remainder of the block (up to the factor x^w, which
 // is invertible mod g) is the XOR of the weights of the set bits.  Two VALU
-// ops per column, against a byte-serial CRC and a 24-step remainder multiply
-// per lane.  Contains one workgroup barrier; every lane gets the result.
+// ops per column (sign-bit extract, then one v_bitop3 for acc ^ (w & m): the
+// truth-table byte 0x78 is src0 ^ (src1 & src2) with sources 0xF0/0xCC/0xAA),
+// against a byte-serial CRC and a 24-step remainder multiply per lane.  Contains one workgroup barrier; every lane gets the result.
 template <int BG, int ZC>
 __device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32_t zx2,
                                                      const uint32_t *wts, uint32_t *slots) {
@@ -1013,8 +1014,14 @@ __device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32
     uint32_t acc = 0u;
 #pragma unroll
     for (int c = 0; c < Graph<BG>::kInfoCols; ++c) {
-        const int q = *(const lds_i16 *)(lqb + 2u * (uint32_t)(c * ZC) + zx2);
-        acc ^= wts[c * kLanes + zx] & (uint32_t)(q >> 31);
+        const uint32_t q = (uint16_t)*(const lds_i16 *)(lqb + 2u * (uint32_t)(c * ZC) + zx2);
+        const uint32_t w = wts[c * kLanes + zx];
+        const uint32_t m = (uint32_t)sext_bit(q, 15);   // hard bit as 0 / -1
+#ifdef NRLDPC_CPU_EMU
+        acc ^= w & m;
+#else
+        acc = __builtin_amdgcn_bitop3_b32(acc, w, m, 0x78u);   // acc ^ (w & m)
+#endif
     }
     acc = wave_xor_full(acc);
     if ((zx & 63) == 0) slots[zx >> 6] = acc;
