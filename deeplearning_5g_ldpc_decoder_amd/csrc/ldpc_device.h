@@ -74,6 +74,9 @@ struct DecParams {
 constexpr int kTileWaves = 8;
 constexpr int kTileThreads = kTileWaves * 64;
 constexpr int kMaxTileCodeblocks = kTileThreads / 2;
+// CRC bit weights of a tile workgroup: one word per lane and systematic
+// column (22 for base graph 1).
+constexpr int kTileWtsBytes = 22 * kTileThreads * 4;
 struct TileDesc {
     uint32_t first;
     uint32_t etab_off;
@@ -89,6 +92,7 @@ static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 struct TileQueue {
     uint32_t *ctr;
     int32_t ntiles;
+    uint32_t wts_lds;   // byte offset of the CRC bit weights in dynamic LDS
 };
 
 // Per-codeblock result flags (status) follow c_ldpc_5g_decode: 0 CRC pass,

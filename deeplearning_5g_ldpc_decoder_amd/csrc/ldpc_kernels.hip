@@ -994,34 +994,64 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
     }
 }
 
-// Lanes of the specialized kernel: ZC rounded up to whole waves.
+// CRC zero test of the hard decisions, linear form.  Bit i of the B-bit block
+// contributes x^(B - 1 - i) mod g to the block's remainder (up to the factor
+// x^w, which is invertible mod g), so the remainder is the XOR of the weights
+// of the set bits.  A lane holds bit i = c * Zc + zx of each systematic column
+// c and the weights of those bits (0 past the block and for lanes without a
+// codeblock) in LDS, column-major with a stride of one workgroup
+// (crc_lin_weights); per column it extracts the hard bit as a 0 / -1 mask and
+// adds w & mask with one v_bitop3 (truth-table byte 0x78: src0 ^ (src1 &
+// src2) with sources 0xF0 / 0xCC / 0xAA).  Against a byte-serial table CRC and
+// a 24-step remainder multiply per lane, the test is two or three VALU ops
+// per column.
+__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, int kb,
+                                                const DecOut &o, const CbDesc &d, int zx,
+                                                int Zc, bool member) {
+    const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
+    const int nb = (int)d.B;
+    for (int c = 0; c < kb; ++c) {
+        const int i = c * Zc + zx;
+        wts[c * stride] = (member && i < nb) ? xb[nb - 1 - i] : 0u;
+    }
+}
+
+// Hard bit of a belief as 0 / -1: the sign bit of an int16, v < 0 for a float
+// (-0.0 is a 0 bit, as in hard_byte).
+__device__ __forceinline__ uint32_t crc_hard_mask(int16_t v) {
+    return (uint32_t)sext_bit((uint32_t)(uint16_t)v, 15);
+}
+__device__ __forceinline__ uint32_t crc_hard_mask(float v) { return v < 0.0f ? ~0u : 0u; }
+
+__device__ __forceinline__ uint32_t crc_lin_add(uint32_t acc, uint32_t w, uint32_t m) {
+#ifdef NRLDPC_CPU_EMU
+    return acc ^ (w & m);
+#else
+    return __builtin_amdgcn_bitop3_b32(acc, w, m, 0x78u);   // acc ^ (w & m)
+#endif
+}
+
+// Lanes of the specialized kernels: ZC rounded up to whole waves.
 template <int ZC>
 inline constexpr int kFixedLanes = (ZC + 63) / 64 * 64;
 
-// CRC zero test of the hard decisions, linear form: lane zx holds bit
-// i = c * ZC + zx of each systematic column c, whose weight x^(B - 1 - i) mod g
-// (0 past the B-bit block, and for lanes zx >= ZC) sits in LDS at
-// wts[c * lanes + zx]; the remainder of the block (up to the factor x^w, which
-// is invertible mod g) is the XOR of the weights of the set bits.  Two VALU
-// ops per column (sign-bit extract, then one v_bitop3 for acc ^ (w & m): the
-// truth-table byte 0x78 is src0 ^ (src1 & src2) with sources 0xF0/0xCC/0xAA),
-// against a byte-serial CRC and a 24-step remainder multiply per lane.  Contains one workgroup barrier; every lane gets the result.
-template <int BG, int ZC>
-__device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32_t zx2,
+// The specialized kernels' test: beliefs of type T at LDS address 0 (lqb),
+// zxb = zx * sizeof(T); one workgroup barrier; every lane gets the result.
+template <int BG, int ZC, typename T>
+__device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32_t zxb,
                                                      const uint32_t *wts, uint32_t *slots) {
+#ifdef NRLDPC_CPU_EMU
+    typedef T lds_t;
+#else
+    typedef __attribute__((address_space(3))) T lds_t;
+#endif
     constexpr int kLanes = kFixedLanes<ZC>;
     const int zx = threadIdx.x;
     uint32_t acc = 0u;
 #pragma unroll
     for (int c = 0; c < Graph<BG>::kInfoCols; ++c) {
-        const uint32_t q = (uint16_t)*(const lds_i16 *)(lqb + 2u * (uint32_t)(c * ZC) + zx2);
-        const uint32_t w = wts[c * kLanes + zx];
-        const uint32_t m = (uint32_t)sext_bit(q, 15);   // hard bit as 0 / -1
-#ifdef NRLDPC_CPU_EMU
-        acc ^= w & m;
-#else
-        acc = __builtin_amdgcn_bitop3_b32(acc, w, m, 0x78u);   // acc ^ (w & m)
-#endif
+        const T q = *(const lds_t *)(lqb + (uint32_t)(sizeof(T) * c * ZC) + zxb);
+        acc = crc_lin_add(acc, wts[c * kLanes + zx], crc_hard_mask(q));
     }
     acc = wave_xor_full(acc);
     if ((zx & 63) == 0) slots[zx >> 6] = acc;
@@ -1068,7 +1098,7 @@ __device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, co
                                                       scale_row<BG, kTable>(p, it), alpha0, rows);
         }
         ++it;
-        if (fixed_crc_linear<BG, ZC>(lqb, 2u * (uint32_t)zx, wts, slots) == 0u) {
+        if (fixed_crc_linear<BG, ZC, int16_t>(lqb, 2u * (uint32_t)zx, wts, slots) == 0u) {
             *status = 0;
             break;
         }
@@ -1445,8 +1475,8 @@ __device__ __forceinline__ void fixed_rows_f32ms(lds_char *lqb, StateF32<BG> &st
 }
 
 template <int BG, int ZC>
-__device__ __forceinline__ int run_fixed_f32ms(float *lq, const DecParams &p, const CrcCtx &crc,
-                               const uint32_t *tab, uint32_t *slots, int *status) {
+__device__ __forceinline__ int run_fixed_f32ms(float *lq, const DecParams &p, const uint32_t *wts,
+                                               uint32_t *slots, int *status) {
     StateF32<BG> st;
 #pragma unroll
     for (int r = 0; r < Graph<BG>::kRows; ++r) {
@@ -1463,7 +1493,7 @@ __device__ __forceinline__ int run_fixed_f32ms(float *lq, const DecParams &p, co
         NRLDPC_OPAQUE_LANE(zx4);
         fixed_rows_f32ms<BG, ZC, 0>(lqb, st, zx4, zx < ZC, p);
         ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
+        if (fixed_crc_linear<BG, ZC, float>(lqb, 4u * (uint32_t)zx, wts, slots) == 0u) {
             *status = 0;
             break;
         }
@@ -1680,15 +1710,7 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     // CRC bit weights first (their own LDS region): the loads overlap the
     // recovery, whose barriers publish them.  Bit i of the B-bit block has
     // weight x^(B - 1 - i) mod g.
-    {
-        const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
-        const int nb = (int)d.B;
-#pragma unroll
-        for (int c = 0; c < Graph<BG>::kInfoCols; ++c) {
-            const int i = c * ZC + t;
-            wts[c * kLanes + t] = (t < ZC && i < nb) ? xb[nb - 1 - i] : 0u;
-        }
-    }
+    crc_lin_weights(wts + t, kLanes, Graph<BG>::kInfoCols, o, d, t, ZC, t < ZC);
     recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     __syncthreads();
 
@@ -1709,13 +1731,13 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
 
 // Lifting-size specialized FP32 min-sum kernel (algo 4): one codeblock of
 // base graph BG and lifting size ZC per workgroup, beliefs as floats at LDS
-// address 0 (all of its LDS is dynamic), then the CRC table and slots.
+// address 0 (all of its LDS is dynamic), then the CRC slots and bit weights.
 template <int BG, int ZC>
 struct FixedLdsF32 {
     static constexpr int kBeliefs = Graph<BG>::kCols * ZC;           // N + 2Zc
-    static constexpr int kTabOff = ((kBeliefs * 4 + 15) / 16) * 16;
-    static constexpr int kSlotsOff = kTabOff + 256 * 4;
-    static constexpr int kBytes = kSlotsOff + (kMaxThreads / 64) * 4;
+    static constexpr int kSlotsOff = ((kBeliefs * 4 + 15) / 16) * 16;
+    static constexpr int kWtsOff = kSlotsOff + 64;
+    static constexpr int kBytes = kWtsOff + Graph<BG>::kInfoCols * kFixedLanes<ZC> * 4;
 };
 
 template <int BG, int ZC>
@@ -1724,21 +1746,21 @@ ldpc_decode_fixed_f32_kernel(DecOut o, DecParams p) {
     using LdsL = FixedLdsF32<BG, ZC>;
     NRLDPC_DYNAMIC_LDS(smem);
     float *lq = reinterpret_cast<float *>(smem);
-    uint32_t *crc_tab = reinterpret_cast<uint32_t *>(smem + LdsL::kTabOff);
     uint32_t *slots = reinterpret_cast<uint32_t *>(smem + LdsL::kSlotsOff);
+    uint32_t *wts = reinterpret_cast<uint32_t *>(smem + LdsL::kWtsOff);
 
     const CbDesc d = o.desc[blockIdx.x];
     constexpr int n = LdsL::kBeliefs;
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+    crc_lin_weights(wts + t, kFixedLanes<ZC>, Graph<BG>::kInfoCols, o, d, t, ZC, t < ZC);
     recover_fill<float>(lq, d, llr, soft, p.filler, t, nt, n);
-    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
     int status = 1, it = 0;
     if (p.max_it > 0) {
-        it = run_fixed_f32ms<BG, ZC>(lq, p, crc, crc_tab, slots, &status);
+        it = run_fixed_f32ms<BG, ZC>(lq, p, wts, slots, &status);
         for (int b = t; b < d.B / 8; b += nt) o.hard[d.hard_off + b] = (uint8_t)hard_byte(lq, b);
     }
     if (t == 0) {
@@ -1761,7 +1783,7 @@ ldpc_decode_fixed_f32_kernel(DecOut o, DecParams p) {
 // ---------------------------------------------------------------------------
 template <int BG, bool kGeneral, bool kTable, bool kSimple>
 __device__ __forceinline__ void run_tile_i16(int16_t *lq, const uint32_t *etab, int zx, int Zc, int slot,
-                             bool member, const CrcCtx &crc, const uint32_t *tab,
+                             bool member, const uint32_t *wts,
                              uint32_t (*bank)[kMaxTileCodeblocks], const DecParams &p,
                              int *status, int *iters) {
     StateI16<BG> st;
@@ -1775,7 +1797,6 @@ __device__ __forceinline__ void run_tile_i16(int16_t *lq, const uint32_t *etab, 
     const GenLane g = gen_lane(lq, zx, Zc);
     bool done = !member;
     bool tile_active = __syncthreads_or(member);
-    const Crc &c = crc.spec;
     for (int it = 0; it < p.max_it && tile_active; ++it) {
         if constexpr (kGeneral)
             rows_i16<BG, 0, kGeneral, kTable>(lq, st, etab, zx, Zc, !done, p,
@@ -1785,12 +1806,12 @@ __device__ __forceinline__ void run_tile_i16(int16_t *lq, const uint32_t *etab, 
                                                   scale_row<BG, kTable>(p, it));
         const int b = it & 1;
         if (!done) {
-            uint32_t r = 0;
-            for (int k = crc.b_lo; k < crc.b_hi; ++k) {
-                const uint32_t byte = hard_byte(lq, k);
-                r = ((r << 8) ^ tab[((r >> (c.w - 8)) ^ byte) & 0xFFu]) & c.mask;
-            }
-            if (r) atomicXor(&bank[b][slot], crc_mulmod_fast(r, crc.mult, c));
+            // linear CRC test (crc_lin_weights): this lane's bits zx + c * Zc
+            uint32_t r = 0u;
+#pragma unroll
+            for (int c = 0; c < Graph<BG>::kInfoCols; ++c)
+                r = crc_lin_add(r, wts[c * kTileThreads], crc_hard_mask(lq[c * Zc + zx]));
+            if (r) atomicXor(&bank[b][slot], r);
         }
         __syncthreads();
         if (!done) {
@@ -1812,8 +1833,7 @@ __device__ __forceinline__ void run_tile_i16(int16_t *lq, const uint32_t *etab, 
 template <bool kGeneral, bool kTable, bool kSimple>
 __device__ __forceinline__ void tile_body(const DecOut &o, const TileDesc *tiles,
                                           const uint32_t *order, const DecParams &p,
-                                          uint32_t tile, unsigned char *smem,
-                                          uint32_t (*crc_tab)[256],
+                                          uint32_t tile, unsigned char *smem, uint32_t *wts,
                                           uint32_t (*bank)[kMaxTileCodeblocks]) {
     const int lane = threadIdx.x, nt = blockDim.x;
     const int wave = NRLDPC_WAVE_UNIFORM(lane >> 6);
@@ -1830,30 +1850,21 @@ __device__ __forceinline__ void tile_body(const DecOut &o, const TileDesc *tiles
 
     for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
         bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
-    CrcCtx crc;
-    crc.spec = crc_spec(d.crctype);
-    crc.b_lo = crc.b_hi = 0;
-    crc.mult = 0u;
+    // the lane's CRC bit weights (the recovery's barriers publish them)
+    crc_lin_weights(wts + lane, kTileThreads, td.bgn == 1 ? Graph<1>::kInfoCols : Graph<2>::kInfoCols,
+                    o, d, zx, Zc, member);
     recover_i16(lq, d, o.llr + d.llr_off, (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr,
                 p.filler, zx, Zc, n, member);
-    if (member) {
-        const int nbytes = d.B / 8, per = (nbytes + Zc - 1) / Zc;
-        crc.b_lo = min(zx * per, nbytes);
-        crc.b_hi = min((zx + 1) * per, nbytes);
-        if (crc.b_hi > crc.b_lo)
-            crc.mult = o.crc[d.crctype * kCrcTabWords + 256 + nbytes - crc.b_hi];
-    }
     __syncthreads();
 
     int status = 1, iters = 0;
     const uint32_t *etab = o.etab + td.etab_off;
-    const uint32_t *tab = crc_tab[member ? d.crctype : 0];
     if (p.max_it > 0) {
         if (td.bgn == 1)
-            run_tile_i16<1, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, crc, tab,
+            run_tile_i16<1, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, wts + lane,
                                                        bank, p, &status, &iters);
         else
-            run_tile_i16<2, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, crc, tab,
+            run_tile_i16<2, kGeneral, kTable, kSimple>(lq, etab, zx, Zc, slot, member, wts + lane,
                                                        bank, p, &status, &iters);
     }
     if (!member) return;
@@ -1880,16 +1891,14 @@ __global__ void __launch_bounds__(kTileThreads)
 ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecParams p,
                  TileQueue q) {
     NRLDPC_DYNAMIC_LDS(smem);
-    __shared__ uint32_t crc_tab[4][256];
     __shared__ uint32_t bank[2][kMaxTileCodeblocks];
     __shared__ uint32_t next_tile;
 
-    const int lane = threadIdx.x, nt = blockDim.x;
-    // CRC tables of all four polynomials (codeblocks may differ in crctype);
-    // the recovery's barriers publish them
-    for (int i = lane; i < 4 * 256; i += nt) crc_tab[i >> 8][i & 255] = o.crc[(i >> 8) * kCrcTabWords + (i & 255)];
+    const int lane = threadIdx.x;
+    // CRC bit weights after the beliefs: kMaxInfoCols columns of kTileThreads
+    uint32_t *wts = reinterpret_cast<uint32_t *>(smem + q.wts_lds);
     if (!q.ctr) {
-        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, blockIdx.x, smem, crc_tab, bank);
+        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, blockIdx.x, smem, wts, bank);
         return;
     }
     for (;;) {
@@ -1897,7 +1906,7 @@ ldpc_tile_kernel(DecOut o, const TileDesc *tiles, const uint32_t *order, DecPara
         __syncthreads();
         const uint32_t tile = next_tile;
         if (tile >= (uint32_t)q.ntiles) break;   // uniform over the workgroup
-        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, tile, smem, crc_tab, bank);
+        tile_body<kGeneral, kTable, kSimple>(o, tiles, order, p, tile, smem, wts, bank);
         // every lane is done with this tile's LDS (and with next_tile)
         __syncthreads();
     }
@@ -2147,8 +2156,8 @@ extern "C" hipError_t nrldpc_configure_kernels() {
     e = nrldpc_configure_bp();
     if (e != hipSuccess) return e;
     // a tile workgroup holds at most 512 lanes' worth of beliefs (~70 KB for
-    // base graph 1) next to 6 KB of static LDS
-    const int tile_lds = 96 * 1024;
+    // base graph 1; at most 90 KB accepted) and the lanes' CRC bit weights
+    const int tile_lds = 90 * 1024 + kTileWtsBytes;
     for (int g = 0; g < 2 && e == hipSuccess; ++g)
         for (int t = 0; t < 2 && e == hipSuccess; ++t)
             for (int m = 0; m < 2 && e == hipSuccess; ++m)
@@ -2191,8 +2200,9 @@ extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
     if (algo < 1 || algo > 3) return hipErrorInvalidValue;
     if (algo != 1 && (p->scale_iters < 1 || !p->scale[0] || !p->scale[1]))
         return hipErrorInvalidValue;
-    const size_t lds = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
-    if ((size_t)lds_entries * sizeof(int16_t) > 90 * 1024) return hipErrorInvalidValue;
+    const size_t beliefs = ((size_t)lds_entries * sizeof(int16_t) + 15) & ~(size_t)15;
+    if (beliefs > 90 * 1024) return hipErrorInvalidValue;
+    const size_t lds = beliefs + kTileWtsBytes;
     const void *fn = tile_kernel(p->general != 0, algo != 1, p->simple != 0);
     int grid = ntiles;
     if (queue) {
@@ -2209,6 +2219,7 @@ extern "C" hipError_t nrldpc_launch_tiles(int algo, const nrldpc::DecOut *o,
     TileQueue q;
     q.ctr = queue;
     q.ntiles = ntiles;
+    q.wts_lds = (uint32_t)beliefs;
     void *args[] = {&ov, &tiles, &order, &pv, &q};
     const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(kTileThreads), args, lds, stream);
     if (e != hipSuccess) return e;

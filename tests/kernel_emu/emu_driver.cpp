@@ -271,13 +271,15 @@ extern "C" int emu_decode_tiles(int n, const int32_t *cfgs, const uint32_t *llr_
     const bool g = g_force_general >= 0 ? g_force_general != 0 : p.general != 0;
     const bool t = row[0] != 1;
     const unsigned nt = (unsigned)(tiles.size() / kTileWaves);
-    const size_t lds = (size_t)lds_entries * 2;
+    const size_t beliefs = ((size_t)lds_entries * 2 + 15) & ~(size_t)15;
+    const size_t lds = beliefs + kTileWtsBytes;
     const bool simple = p.simple != 0;
     // persistent launch: g_persistent_grid workgroups pull the tiles from a queue
     uint32_t ctr[2] = {0u, 0u};
     TileQueue q;
     q.ctr = g_persistent_grid > 0 ? ctr : nullptr;
     q.ntiles = (int)nt;
+    q.wts_lds = (uint32_t)beliefs;
     const unsigned grid = g_persistent_grid > 0 ? std::min(nt, (unsigned)g_persistent_grid) : nt;
     if (!g && !t && simple)
         emu_launch(grid, kTileThreads, lds, [&] { ldpc_tile_kernel<false, false, true>(o, td, ord, p, q); });
