@@ -1005,15 +1005,22 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
 // src2) with sources 0xF0 / 0xCC / 0xAA).  Against a byte-serial table CRC and
 // a 24-step remainder multiply per lane, the test is two or three VALU ops
 // per column.
-__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, int kb,
-                                                const DecOut &o, const CbDesc &d, int zx,
-                                                int Zc, bool member) {
+// KB columns: the loads are unconditional (an index clamped into the table)
+// so that all of them are in flight together, then one wait, then the stores.
+template <int KB>
+__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, const DecOut &o,
+                                                const CbDesc &d, int zx, int Zc, bool member) {
     const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
     const int nb = (int)d.B;
-    for (int c = 0; c < kb; ++c) {
-        const int i = c * Zc + zx;
-        wts[c * stride] = (member && i < nb) ? xb[nb - 1 - i] : 0u;
+    uint32_t w[KB];
+#pragma unroll
+    for (int c = 0; c < KB; ++c) {
+        const int m = nb - 1 - (c * Zc + zx);   // bits after this one
+        w[c] = xb[m < 0 ? 0 : m];
+        if (!member || m < 0) w[c] = 0u;
     }
+#pragma unroll
+    for (int c = 0; c < KB; ++c) wts[c * stride] = w[c];
 }
 
 // Hard bit of a belief as 0 / -1: the sign bit of an int16, v < 0 for a float
@@ -1710,7 +1717,7 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     // CRC bit weights first (their own LDS region): the loads overlap the
     // recovery, whose barriers publish them.  Bit i of the B-bit block has
     // weight x^(B - 1 - i) mod g.
-    crc_lin_weights(wts + t, kLanes, Graph<BG>::kInfoCols, o, d, t, ZC, t < ZC);
+    crc_lin_weights<Graph<BG>::kInfoCols>(wts + t, kLanes, o, d, t, ZC, t < ZC);
     recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
     __syncthreads();
 
@@ -1754,7 +1761,7 @@ ldpc_decode_fixed_f32_kernel(DecOut o, DecParams p) {
     const int t = threadIdx.x, nt = blockDim.x;
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    crc_lin_weights(wts + t, kFixedLanes<ZC>, Graph<BG>::kInfoCols, o, d, t, ZC, t < ZC);
+    crc_lin_weights<Graph<BG>::kInfoCols>(wts + t, kFixedLanes<ZC>, o, d, t, ZC, t < ZC);
     recover_fill<float>(lq, d, llr, soft, p.filler, t, nt, n);
     __syncthreads();
 
@@ -1851,8 +1858,10 @@ __device__ __forceinline__ void tile_body(const DecOut &o, const TileDesc *tiles
     for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
         bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
     // the lane's CRC bit weights (the recovery's barriers publish them)
-    crc_lin_weights(wts + lane, kTileThreads, td.bgn == 1 ? Graph<1>::kInfoCols : Graph<2>::kInfoCols,
-                    o, d, zx, Zc, member);
+    if (td.bgn == 1)
+        crc_lin_weights<Graph<1>::kInfoCols>(wts + lane, kTileThreads, o, d, zx, Zc, member);
+    else
+        crc_lin_weights<Graph<2>::kInfoCols>(wts + lane, kTileThreads, o, d, zx, Zc, member);
     recover_i16(lq, d, o.llr + d.llr_off, (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr,
                 p.filler, zx, Zc, n, member);
     __syncthreads();
