@@ -185,10 +185,16 @@ __device__ __forceinline__ void recover_scatter_x8(int16_t *lq, const CbDesc &d,
 // lanes first, first + stride, ... .  Every lane of the workgroup calls it
 // (it holds two workgroup barriers); lanes with active = false only join
 // them.  The caller's next barrier orders the soft-buffer write-back before
-// decoding changes the beliefs.
-template <int kBatch = kScatterBatch>
+// decoding changes the beliefs.  mid() runs after the first barrier, before
+// the scatter issues its loads: loads it issues share the scatter's memory
+// round trip.
+struct NoHook {
+    __device__ void operator()() const {}
+};
+template <int kBatch = kScatterBatch, typename Mid = NoHook>
 __device__ __forceinline__ void recover_i16(int16_t *lq, const CbDesc &d, const int16_t *llr, int16_t *soft,
-                            int filler, int first, int stride, int n, bool active) {
+                            int filler, int first, int stride, int n, bool active,
+                            Mid &&mid = NoHook()) {
     const bool scatter = active && d.E <= d.N - (d.K - d.B);
     if (active) {
         if (scatter)
@@ -197,6 +203,7 @@ __device__ __forceinline__ void recover_i16(int16_t *lq, const CbDesc &d, const 
             recover_fill<int16_t>(lq, d, llr, soft, filler, first, stride, n);
     }
     __syncthreads();
+    mid();
     if (scatter) {
         const bool harq = d.harq_on && soft;
         if (((uintptr_t)llr & 15u) == 0 && (d.E & 7) == 0)
@@ -1006,21 +1013,35 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
 // a 24-step remainder multiply per lane, the test is two or three VALU ops
 // per column.
 // KB columns: the loads are unconditional (an index clamped into the table)
-// so that all of them are in flight together, then one wait, then the stores.
+// so that all of them are in flight together (crc_lin_load); the stores
+// follow once they have arrived (crc_lin_store).
 template <int KB>
-__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, const DecOut &o,
-                                                const CbDesc &d, int zx, int Zc, bool member) {
+struct CrcLinRegs {
+    uint32_t w[KB];
+};
+template <int KB>
+__device__ __forceinline__ void crc_lin_load(CrcLinRegs<KB> &r, const DecOut &o, const CbDesc &d,
+                                             int zx, int Zc, bool member) {
     const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
     const int nb = (int)d.B;
-    uint32_t w[KB];
 #pragma unroll
     for (int c = 0; c < KB; ++c) {
         const int m = nb - 1 - (c * Zc + zx);   // bits after this one
-        w[c] = xb[m < 0 ? 0 : m];
-        if (!member || m < 0) w[c] = 0u;
+        r.w[c] = xb[m < 0 ? 0 : m];
+        if (!member || m < 0) r.w[c] = 0u;
     }
+}
+template <int KB>
+__device__ __forceinline__ void crc_lin_store(const CrcLinRegs<KB> &r, uint32_t *wts, int stride) {
 #pragma unroll
-    for (int c = 0; c < KB; ++c) wts[c * stride] = w[c];
+    for (int c = 0; c < KB; ++c) wts[c * stride] = r.w[c];
+}
+template <int KB>
+__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, const DecOut &o,
+                                                const CbDesc &d, int zx, int Zc, bool member) {
+    CrcLinRegs<KB> r;
+    crc_lin_load<KB>(r, o, d, zx, Zc, member);
+    crc_lin_store<KB>(r, wts, stride);
 }
 
 // Hard bit of a belief as 0 / -1: the sign bit of an int16, v < 0 for a float
@@ -1717,8 +1738,11 @@ ldpc_decode_fixed_kernel(DecOut o, DecParams p) {
     // CRC bit weights first (their own LDS region): the loads overlap the
     // recovery, whose barriers publish them.  Bit i of the B-bit block has
     // weight x^(B - 1 - i) mod g.
-    crc_lin_weights<Graph<BG>::kInfoCols>(wts + t, kLanes, o, d, t, ZC, t < ZC);
-    recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
+    // CRC bit weights: loaded with the recovery's LLR loads, stored after it
+    CrcLinRegs<Graph<BG>::kInfoCols> wr;
+    recover_i16<kScatterBatch>(lq, d, llr, soft, p.filler, t, nt, n, true,
+                               [&]() { crc_lin_load(wr, o, d, t, ZC, t < ZC); });
+    crc_lin_store(wr, wts + t, kLanes);
     __syncthreads();
 
     int status = 1, it = 0;
@@ -1857,13 +1881,21 @@ __device__ __forceinline__ void tile_body(const DecOut &o, const TileDesc *tiles
 
     for (int i = lane; i < 2 * kMaxTileCodeblocks; i += nt)
         bank[i / kMaxTileCodeblocks][i % kMaxTileCodeblocks] = 0u;
-    // the lane's CRC bit weights (the recovery's barriers publish them)
-    if (td.bgn == 1)
-        crc_lin_weights<Graph<1>::kInfoCols>(wts + lane, kTileThreads, o, d, zx, Zc, member);
-    else
-        crc_lin_weights<Graph<2>::kInfoCols>(wts + lane, kTileThreads, o, d, zx, Zc, member);
-    recover_i16(lq, d, o.llr + d.llr_off, (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr,
-                p.filler, zx, Zc, n, member);
+    // the lane's CRC bit weights: loaded with the recovery's LLR loads, stored
+    // after it (the barrier below publishes them)
+    const int16_t *llr = o.llr + d.llr_off;
+    int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
+    if (td.bgn == 1) {
+        CrcLinRegs<Graph<1>::kInfoCols> wr;
+        recover_i16<kScatterBatch>(lq, d, llr, soft, p.filler, zx, Zc, n, member,
+                                   [&]() { crc_lin_load(wr, o, d, zx, Zc, member); });
+        crc_lin_store(wr, wts + lane, kTileThreads);
+    } else {
+        CrcLinRegs<Graph<2>::kInfoCols> wr;
+        recover_i16<kScatterBatch>(lq, d, llr, soft, p.filler, zx, Zc, n, member,
+                                   [&]() { crc_lin_load(wr, o, d, zx, Zc, member); });
+        crc_lin_store(wr, wts + lane, kTileThreads);
+    }
     __syncthreads();
 
     int status = 1, iters = 0;
