@@ -833,9 +833,15 @@ __device__ __forceinline__ PackedScale packed_finalize(const PackedRowState<PAIR
                                                        const DecParams &p, int alpha, int beta) {
     // merge the halves: m1 = min of the four, m2 = second smallest
     const uint32_t m1l = rs.m1 & 0xFFFFu, m1h = rs.m1 >> 16;
-    const uint32_t m2l = rs.m2 & 0xFFFFu, m2h = rs.m2 >> 16;
     const int m1 = (int)min(m1l, m1h);
-    const int m2 = (int)min(max(m1l, m1h), min(m2l, m2h));
+    // the halves' second minima as one 16-bit min (an SDWA op; a min3 over
+    // the extracted halves takes three)
+#ifdef NRLDPC_CPU_EMU
+    const uint32_t m2s = min(rs.m2 & 0xFFFFu, rs.m2 >> 16);
+#else
+    const uint32_t m2s = __builtin_elementwise_min((uint16_t)rs.m2, (uint16_t)(rs.m2 >> 16));
+#endif
+    const int m2 = (int)min(max(m1l, m1h), m2s);
     int n1, n2;
     if constexpr (kSimple) {
         n1 = scale_simple(m1, alpha);
