@@ -77,6 +77,9 @@ constexpr int kMaxTileCodeblocks = kTileThreads / 2;
 // CRC bit weights of a tile workgroup: one word per lane and systematic
 // column (22 for base graph 1).
 constexpr int kTileWtsBytes = 22 * kTileThreads * 4;
+// CRC bit-weight columns of the one-codeblock-per-workgroup min-sum kernels
+// (the larger base graph's systematic columns), per lane.
+constexpr int kGenericWtsCols = 22;
 struct TileDesc {
     uint32_t first;
     uint32_t etab_off;

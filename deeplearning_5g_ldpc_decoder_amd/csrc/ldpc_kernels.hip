@@ -1096,6 +1096,26 @@ __device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32
     return total;
 }
 
+// The one-codeblock-per-workgroup kernels' test: beliefs lq (generic
+// pointer, lifting size Zc at run time), weights wts[c * blockDim.x + zx];
+// lanes zx >= Zc read another column's belief under a zero weight (still
+// inside the N + 2Zc beliefs for every Zc >= 2).
+template <int BG, typename T>
+__device__ __forceinline__ uint32_t block_crc_linear(const T *lq, int Zc, const uint32_t *wts,
+                                                     uint32_t *slots) {
+    const int zx = threadIdx.x, nt = blockDim.x;
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int c = 0; c < Graph<BG>::kInfoCols; ++c)
+        acc = crc_lin_add(acc, wts[c * nt + zx], crc_hard_mask(lq[c * Zc + zx]));
+    acc = wave_xor_full(acc);   // block sizes are whole waves
+    if ((zx & 63) == 0) slots[zx >> 6] = acc;
+    __syncthreads();
+    uint32_t total = 0u;
+    for (int w = 0; w < (nt >> 6); ++w) total ^= slots[w];
+    return total;
+}
+
 template <int BG, int ZC, bool kGeneral, bool kTable, bool kSimple>
 __device__ __forceinline__ int run_fixed_i16(int16_t *lq, const DecParams &p, const uint32_t *wts,
                                              uint32_t *slots, int *status) {
@@ -1301,7 +1321,7 @@ constexpr int kPackedWords = packed_word0<BG>(Graph<BG>::kRows);
 
 template <int BG, bool kGeneral, bool kTable>
 __device__ __forceinline__ int run_i16(int16_t *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
-                       const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
+                       const uint32_t *wts, uint32_t *slots, int *status) {
     const int zx = threadIdx.x;
     int it = 0;
     *status = 1;
@@ -1323,7 +1343,7 @@ __device__ __forceinline__ int run_i16(int16_t *lq, const uint32_t *etab, const 
             gprows_iteration<BG, kTable, false>(g, msg, etab, zx < d.Zc, p,
                                                 scale_row<BG, kTable>(p, it));
         ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
+        if (block_crc_linear<BG>(lq, d.Zc, wts, slots) == 0u) {
             *status = 0;
             break;
         }
@@ -1448,7 +1468,7 @@ __device__ __forceinline__ void rows_f32ms(float *lq, StateF32<BG> &st, etab_cpt
 
 template <int BG>
 __device__ __forceinline__ int run_f32ms(float *lq, const uint32_t *etab, const CbDesc &d, const DecParams &p,
-                         const CrcCtx &crc, const uint32_t *tab, uint32_t *slots, int *status) {
+                         const uint32_t *wts, uint32_t *slots, int *status) {
     StateF32<BG> st;
 #pragma unroll
     for (int r = 0; r < Graph<BG>::kRows; ++r) {
@@ -1465,7 +1485,7 @@ __device__ __forceinline__ int run_f32ms(float *lq, const uint32_t *etab, const 
         NRLDPC_OPAQUE_UNIFORM(ce);
         rows_f32ms<BG, 0>(lq, st, ce, zx, d.Zc, p);
         ++it;
-        if (block_crc(lq, crc, tab, slots) == 0u) {
+        if (block_crc_linear<BG>(lq, d.Zc, wts, slots) == 0u) {
             *status = 0;
             break;
         }
@@ -1675,14 +1695,37 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     const int n = d.N + 2 * d.Zc;
     const int t = threadIdx.x, nt = blockDim.x;
 
-    // fused rate recovery + HARQ combine into LDS; soft buffer write-back
+    // fused rate recovery + HARQ combine into LDS; soft buffer write-back.
+    // The min-sum kernels' CRC bit weights follow this codeblock's beliefs in
+    // the dynamic LDS (the launch adds kGenericWtsCols * blockDim.x words).
     const int16_t *llr = o.llr + d.llr_off;
     int16_t *soft = (d.soft_off != kNone) ? o.soft + d.soft_off : nullptr;
-    if constexpr (A == Algo::kMinSumI16 || A == Algo::kMinSumI16Edge)
+    uint32_t *wts = reinterpret_cast<uint32_t *>(smem + (((size_t)n * sizeof(T) + 15) & ~(size_t)15));
+    if constexpr (A == Algo::kMinSumI16) {
+        if (d.bgn == 1) {
+            CrcLinRegs<Graph<1>::kInfoCols> wr;
+            recover_i16<kScatterBatch>(lq, d, llr, soft, p.filler, t, nt, n, true,
+                                       [&]() { crc_lin_load(wr, o, d, t, d.Zc, t < d.Zc); });
+            crc_lin_store(wr, wts + t, nt);
+        } else {
+            CrcLinRegs<Graph<2>::kInfoCols> wr;
+            recover_i16<kScatterBatch>(lq, d, llr, soft, p.filler, t, nt, n, true,
+                                       [&]() { crc_lin_load(wr, o, d, t, d.Zc, t < d.Zc); });
+            crc_lin_store(wr, wts + t, nt);
+        }
+    } else if constexpr (A == Algo::kMinSumI16Edge) {
         recover_i16(lq, d, llr, soft, p.filler, t, nt, n, true);
-    else
+    } else {
+        if (d.bgn == 1)
+            crc_lin_weights<Graph<1>::kInfoCols>(wts + t, nt, o, d, t, d.Zc, t < d.Zc);
+        else
+            crc_lin_weights<Graph<2>::kInfoCols>(wts + t, nt, o, d, t, d.Zc, t < d.Zc);
         recover_fill<T>(lq, d, llr, soft, p.filler, t, nt, n);
-    const CrcCtx crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
+    }
+    // the lane-per-edge schedule keeps the byte-form test (its lanes are
+    // edges, not lifted checks)
+    CrcCtx crc{};
+    if constexpr (A == Algo::kMinSumI16Edge) crc = crc_setup_tab(d.crctype, d.B / 8, crc_tab, o.crc);
     __syncthreads();
 
     const uint32_t *etab = o.etab + d.etab_off;
@@ -1690,11 +1733,11 @@ ldpc_decode_kernel(DecOut o, DecParams p) {
     if (p.max_it > 0) {
         if constexpr (A == Algo::kMinSumI16) {
             it = (d.bgn == 1)
-                     ? run_i16<1, kGeneral, kTable>(lq, etab, d, p, crc, crc_tab, slots, &status)
-                     : run_i16<2, kGeneral, kTable>(lq, etab, d, p, crc, crc_tab, slots, &status);
+                     ? run_i16<1, kGeneral, kTable>(lq, etab, d, p, wts, slots, &status)
+                     : run_i16<2, kGeneral, kTable>(lq, etab, d, p, wts, slots, &status);
         } else if constexpr (A == Algo::kMinSumF32) {
-            it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
-                              : run_f32ms<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
+            it = (d.bgn == 1) ? run_f32ms<1>(lq, etab, d, p, wts, slots, &status)
+                              : run_f32ms<2>(lq, etab, d, p, wts, slots, &status);
         } else if constexpr (A == Algo::kMinSumI16Edge) {
             it = (d.bgn == 1) ? run_lpe_i16<1>(lq, etab, d, p, crc, crc_tab, slots, &status)
                               : run_lpe_i16<2>(lq, etab, d, p, crc, crc_tab, slots, &status);
@@ -2141,7 +2184,9 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
     using namespace nrldpc;
     const bool i16 = algo >= 1 && algo <= 3;
     const size_t esz = i16 ? sizeof(int16_t) : sizeof(float);
-    size_t lds = ((size_t)max_soft * esz + 15) & ~(size_t)15;
+    // beliefs, then the CRC bit weights of the min-sum kernels
+    size_t lds = (((size_t)max_soft * esz + 15) & ~(size_t)15) +
+                 (size_t)kGenericWtsCols * (size_t)threads * 4;
     const dim3 grid(ncb), block(threads);
     DecOut ov = *o;
     DecParams pv = *p;

@@ -164,7 +164,10 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
     const DecOut o = outputs(b, llr, soft_pool, hard, status, iters, lq, work);
     const DecParams p = b.p;
     const unsigned threads = (unsigned)workgroup_threads(b.max_zc);
-    const size_t lds16 = (size_t)b.max_soft * 2;
+    // beliefs, then the min-sum kernels' CRC bit weights (nrldpc_launch_decode)
+    const size_t wts_bytes = (size_t)kGenericWtsCols * threads * 4;
+    const size_t lds16 = (((size_t)b.max_soft * 2 + 15) & ~(size_t)15) + wts_bytes;
+    const size_t lds32 = (((size_t)b.max_soft * 4 + 15) & ~(size_t)15) + wts_bytes;
     switch (row[0]) {
     case 1:
     case 2:
@@ -208,7 +211,7 @@ extern "C" int emu_decode_batch(int n, const int32_t *cfgs, const uint32_t *llr_
                        [&] { ldpc_decode_fixed_f32_kernel<1, 15>(o, p); });
             return 0;
         }
-        emu_launch(n, threads, (size_t)b.max_soft * 4,
+        emu_launch(n, threads, lds32,
                    [&] { ldpc_decode_kernel<Algo::kMinSumF32, float>(o, p); });
         return 0;
     case 0:
