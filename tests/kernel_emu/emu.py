@@ -42,6 +42,8 @@ def lib():
         L.emu_force_general.argtypes = [I]
         L.emu_force_general.restype = None
         L.emu_use_fixed.argtypes = [I]
+        L.emu_crc_tables.argtypes = [P, I]
+        L.emu_crc_tables.restype = I
         L.emu_use_fixed.restype = None
         L.emu_set_persistent_tiles.argtypes = [I]
         L.emu_set_persistent_tiles.restype = None
@@ -226,3 +228,13 @@ def plan_tiles(cfgs, k=0):
     if rc < 0:
         raise RuntimeError(f"emu_plan_tiles returned {rc}")
     return tiles[:rc * TILE_WAVES].reshape(rc, TILE_WAVES), order[:n], lds.value
+
+
+def crc_tables():
+    """The device CRC tables (4 CRC types x words per type) and the offset of
+    the per-bit weights x^m mod g within a type's words."""
+    L = lib()
+    words = L.emu_crc_tab_words()
+    out = np.zeros(4 * words, np.uint32)
+    assert L.emu_crc_tables(_p(out), out.size) == out.size
+    return out.reshape(4, words), L.emu_crc_bit_off()

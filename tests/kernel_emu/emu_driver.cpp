@@ -312,6 +312,17 @@ extern "C" void emu_set_degree_alpha(const int16_t *values) { set_degree_alpha(v
 // 0: never run a lifting-size specialized kernel (generic kernels only).
 extern "C" void emu_use_fixed(int on) { g_use_fixed = on; }
 
+// The device CRC tables as the library builds them (crc_device_tables):
+// 4 * kCrcTabWords words; returns the number copied, or -1 if n is short.
+extern "C" int emu_crc_tables(uint32_t *out, int n) {
+    const std::vector<uint32_t> &t = tables().crc;
+    if (n < (int)t.size()) return -1;
+    std::copy(t.begin(), t.end(), out);
+    return (int)t.size();
+}
+extern "C" int emu_crc_tab_words() { return kCrcTabWords; }
+extern "C" int emu_crc_bit_off() { return kCrcBitOff; }
+
 // Forces the int16 row form (0 fast, 1 exact) for negative controls; -1
 // restores the library's choice.
 extern "C" void emu_force_general(int g) { g_force_general = g; }

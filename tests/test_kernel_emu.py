@@ -601,3 +601,45 @@ def test_tile_plan_invariants(oracle_mod):
     assert np.all(seen == 1)
     assert mixed_wgs > 0
     assert lds * 2 <= 90 * 1024
+
+
+def test_crc_bit_weights_give_the_reference_crc(oracle_mod):
+    """The per-bit weight table behind the kernels' linear CRC test: entry m
+    of a CRC type is x^m mod g, and for random byte strings the XOR of the
+    set bits' weights (bit i of B weighs x^(B-1-i)), times x^w mod g, equals
+    the reference's byte-serial CRC of the string (the oracle's, pinned to
+    the reference's vectors).  So a block passes the kernels' zero test
+    exactly when its CRC remainder is zero."""
+    tabs, off = emu.crc_tables()
+    polys = {0: (0x1021, 16), 1: (0x864CFB, 24), 2: (0x800063, 24), 3: (0xB2B117, 24)}
+    rng = np.random.default_rng(7)
+    for t, (poly, w) in polys.items():
+        g = poly | (1 << w)
+        weights = tabs[t][off:].astype(np.int64)
+        r = 1
+        for m in range(weights.size):
+            assert weights[m] == r, (t, m)
+            r <<= 1
+            if (r >> w) & 1:
+                r ^= g
+        for nbytes in (1, 3, 66, 1056):
+            data = rng.integers(0, 256, nbytes, dtype=np.uint8)
+            bits = np.unpackbits(data)
+            acc = 0
+            for i in np.flatnonzero(bits):
+                acc ^= int(weights[bits.size - 1 - i])
+            for _ in range(w):
+                acc <<= 1
+                if (acc >> w) & 1:
+                    acc ^= g
+            assert acc == oracle_mod.crc(data, t), (t, nbytes)
+        # a string with its CRC appended has every weight sum zero
+        data = rng.integers(0, 256, 40, dtype=np.uint8)
+        c = oracle_mod.crc(data, t)
+        full = np.concatenate([data, np.array([(c >> s) & 0xFF for s in range(w - 8, -1, -8)],
+                                              np.uint8)])
+        bits = np.unpackbits(full)
+        acc = 0
+        for i in np.flatnonzero(bits):
+            acc ^= int(weights[bits.size - 1 - i])
+        assert acc == 0, t
