@@ -743,6 +743,15 @@ def main():
             "lds": {"achieved": lds_bytes(iters) / avg_kernel_s / 1e9,
                     "peak": LDS_PEAK / 1e9, "unit": "GB/s",
                     "frac": lds_bytes(iters) / avg_kernel_s / LDS_PEAK}}
+    if vi and inflight > 1:
+        # informational: the same instruction count per step over the
+        # pipelined region's time per step, where the next launch's
+        # workgroups fill the CUs that early-stopping codeblocks free
+        roof["pipelined"] = {
+            "achieved": vi / (ms_per_step / 1e3) / 1e9,
+            "frac": vi / (ms_per_step / 1e3) / VALU_PEAK,
+            "note": "wave-VALU instructions per step / pipelined ms_per_step (the step's "
+                    "decode overlapped with the previous one); not a per-launch figure"}
 
     log("per-codeblock path done")
     cpu = None
