@@ -138,6 +138,7 @@ bool try_fixed(uint32_t uniform, unsigned n, unsigned threads, const DecOut &o,
     case (1u << 16) | 384u: launch_fixed<1, 384>(n, threads, o, p, g, t); return true;
     case (2u << 16) | 26u: launch_fixed<2, 26>(n, threads, o, p, g, t); return true;
     case (1u << 16) | 15u: launch_fixed<1, 15>(n, threads, o, p, g, t); return true;
+    case (1u << 16) | 16u: launch_fixed<1, 16>(n, threads, o, p, g, t); return true;
     default: return false;
     }
 }

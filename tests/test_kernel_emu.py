@@ -389,7 +389,7 @@ def test_emu_synth_matches_oracle(oracle_mod, cfg):
     assert O.syndrome_ok(c, gen["cw"][0])
 
 
-@pytest.mark.parametrize("pair", [(2, 26), (1, 15)])
+@pytest.mark.parametrize("pair", [(2, 26), (1, 15), (1, 16)])
 @pytest.mark.parametrize("row,tab", [
     ([1, 8, 22936, 0, 1280, 32767, -32767], 2),      # packed, simple scale
     ([1, 8, 26000, 32, -900, 32767, -32767], 2),     # packed, offset min-sum
