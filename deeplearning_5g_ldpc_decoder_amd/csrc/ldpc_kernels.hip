@@ -1007,64 +1007,6 @@ __device__ __forceinline__ void fixed_rows(lds_char *lqb, uint32_t (&msg)[N], ui
     }
 }
 
-// CRC zero test of the hard decisions, linear form.  Bit i of the B-bit block
-// contributes x^(B - 1 - i) mod g to the block's remainder (up to the factor
-// x^w, which is invertible mod g), so the remainder is the XOR of the weights
-// of the set bits.  A lane holds bit i = c * Zc + zx of each systematic column
-// c and the weights of those bits (0 past the block and for lanes without a
-// codeblock) in LDS, column-major with a stride of one workgroup
-// (crc_lin_weights); per column it extracts the hard bit as a 0 / -1 mask and
-// adds w & mask with one v_bitop3 (truth-table byte 0x78: src0 ^ (src1 &
-// src2) with sources 0xF0 / 0xCC / 0xAA).  Against a byte-serial table CRC and
-// a 24-step remainder multiply per lane, the test is two or three VALU ops
-// per column.
-// KB columns: the loads are unconditional (an index clamped into the table)
-// so that all of them are in flight together (crc_lin_load); the stores
-// follow once they have arrived (crc_lin_store).
-template <int KB>
-struct CrcLinRegs {
-    uint32_t w[KB];
-};
-template <int KB>
-__device__ __forceinline__ void crc_lin_load(CrcLinRegs<KB> &r, const DecOut &o, const CbDesc &d,
-                                             int zx, int Zc, bool member) {
-    const uint32_t *xb = o.crc + (size_t)d.crctype * kCrcTabWords + kCrcBitOff;
-    const int nb = (int)d.B;
-#pragma unroll
-    for (int c = 0; c < KB; ++c) {
-        const int m = nb - 1 - (c * Zc + zx);   // bits after this one
-        r.w[c] = xb[m < 0 ? 0 : m];
-        if (!member || m < 0) r.w[c] = 0u;
-    }
-}
-template <int KB>
-__device__ __forceinline__ void crc_lin_store(const CrcLinRegs<KB> &r, uint32_t *wts, int stride) {
-#pragma unroll
-    for (int c = 0; c < KB; ++c) wts[c * stride] = r.w[c];
-}
-template <int KB>
-__device__ __forceinline__ void crc_lin_weights(uint32_t *wts, int stride, const DecOut &o,
-                                                const CbDesc &d, int zx, int Zc, bool member) {
-    CrcLinRegs<KB> r;
-    crc_lin_load<KB>(r, o, d, zx, Zc, member);
-    crc_lin_store<KB>(r, wts, stride);
-}
-
-// Hard bit of a belief as 0 / -1: the sign bit of an int16, v < 0 for a float
-// (-0.0 is a 0 bit, as in hard_byte).
-__device__ __forceinline__ uint32_t crc_hard_mask(int16_t v) {
-    return (uint32_t)sext_bit((uint32_t)(uint16_t)v, 15);
-}
-__device__ __forceinline__ uint32_t crc_hard_mask(float v) { return v < 0.0f ? ~0u : 0u; }
-
-__device__ __forceinline__ uint32_t crc_lin_add(uint32_t acc, uint32_t w, uint32_t m) {
-#ifdef NRLDPC_CPU_EMU
-    return acc ^ (w & m);
-#else
-    return __builtin_amdgcn_bitop3_b32(acc, w, m, 0x78u);   // acc ^ (w & m)
-#endif
-}
-
 // Lanes of the specialized kernels: ZC rounded up to whole waves.
 template <int ZC>
 inline constexpr int kFixedLanes = (ZC + 63) / 64 * 64;
@@ -1093,26 +1035,6 @@ __device__ __forceinline__ uint32_t fixed_crc_linear(const lds_char *lqb, uint32
     uint32_t total = 0u;
 #pragma unroll
     for (int w = 0; w < kLanes / 64; ++w) total ^= slots[w];
-    return total;
-}
-
-// The one-codeblock-per-workgroup kernels' test: beliefs lq (generic
-// pointer, lifting size Zc at run time), weights wts[c * blockDim.x + zx];
-// lanes zx >= Zc read another column's belief under a zero weight (still
-// inside the N + 2Zc beliefs for every Zc >= 2).
-template <int BG, typename T>
-__device__ __forceinline__ uint32_t block_crc_linear(const T *lq, int Zc, const uint32_t *wts,
-                                                     uint32_t *slots) {
-    const int zx = threadIdx.x, nt = blockDim.x;
-    uint32_t acc = 0u;
-#pragma unroll
-    for (int c = 0; c < Graph<BG>::kInfoCols; ++c)
-        acc = crc_lin_add(acc, wts[c * nt + zx], crc_hard_mask(lq[c * Zc + zx]));
-    acc = wave_xor_full(acc);   // block sizes are whole waves
-    if ((zx & 63) == 0) slots[zx >> 6] = acc;
-    __syncthreads();
-    uint32_t total = 0u;
-    for (int w = 0; w < (nt >> 6); ++w) total ^= slots[w];
     return total;
 }
 
@@ -2184,9 +2106,11 @@ extern "C" hipError_t nrldpc_launch_decode(int algo, const nrldpc::DecOut *o,
     using namespace nrldpc;
     const bool i16 = algo >= 1 && algo <= 3;
     const size_t esz = i16 ? sizeof(int16_t) : sizeof(float);
-    // beliefs, then the CRC bit weights of the min-sum kernels
+    // beliefs, then the CRC bit weights of the min-sum kernels: one column
+    // per systematic column of the launch's base graph (both when mixed)
+    const int wcols = (uniform >> 16) == 2u ? Graph<2>::kInfoCols : kGenericWtsCols;
     size_t lds = (((size_t)max_soft * esz + 15) & ~(size_t)15) +
-                 (size_t)kGenericWtsCols * (size_t)threads * 4;
+                 (size_t)wcols * (size_t)threads * 4;
     const dim3 grid(ncb), block(threads);
     DecOut ov = *o;
     DecParams pv = *p;
