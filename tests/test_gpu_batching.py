@@ -176,8 +176,12 @@ def test_overlapping_launches_of_one_plan(torch_dev, oracle_mod, uniform, flags)
     plan.close()
 
 
+# CRC24B as in config 2, plus CRC16 and CRC24A codeblocks (and filler bits)
+# for the CRC stop test's per-type bit weights
 _SPECIALIZED_CFGS = [(2, 1016, 2, 2080, 0, 4, 0), (2, 2056, 2, 10400, 1, 2, 0),
-                     (1, 8424, 2, 25344, 0, 2, 0)]
+                     (1, 8424, 2, 25344, 0, 2, 0), (1, 8432, 0, 25344, 0, 2, 0),
+                     (1, 8424, 1, 25344, 0, 2, 0), (2, 1024, 0, 2080, 0, 4, 0),
+                     (2, 2040, 1, 10400, 1, 2, 0)]
 _SPECIALIZED_ROWS = [[1, 8, 22936, 0, 1280, 32767, -32767],
                      [1, 8, 26000, 40, -900, 2000, -1500],
                      [4, 20, 22936, 0, 1280, 32767, -32767]]
