@@ -30,6 +30,10 @@ struct EmuDim3 {
 struct uint4 {
     uint32_t x, y, z, w;
 };
+struct uint2 {
+    uint32_t x, y;
+};
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
 struct float4 {
     float x, y, z, w;
 };

@@ -56,3 +56,14 @@ def test_bp_kernel_no_scratch_no_long_branch(tmp_path):
     sizes = [int(line.split(":")[1]) for line in notes.splitlines()
              if ".private_segment_fixed_size:" in line]
     assert sizes and all(s == 0 for s in sizes), sizes
+
+
+def test_specialized_kernels_no_scratch(tmp_path):
+    """The lifting-size specialized kernels (the headline decode) keep their
+    state in VGPRs and LDS: a spill to scratch inside the row loop costs a
+    memory round trip per access, so any private segment fails the build's
+    tests."""
+    _, notes = _device_code(tmp_path, "ldpc_fixed")
+    sizes = [int(line.split(":")[1]) for line in notes.splitlines()
+             if ".private_segment_fixed_size:" in line]
+    assert sizes and all(s == 0 for s in sizes), sizes
